@@ -1,0 +1,249 @@
+/*
+ * jvmc_hip.h -- C ABI of libjvmc_hip.so, the gfx950 (MI355X) implementation of
+ * jVMC's VMC inner loop (one TDVP / MinSR right-hand-side evaluation).
+ *
+ * Conventions
+ *   - Every entry point returns an int status: JV_OK (0) or a negative JV_E_*
+ *     code; jv_last_error() returns the message of the last failure on the
+ *     calling thread.  No C++ exception crosses this boundary.
+ *   - Array arguments are DEVICE pointers owned by the caller, except where a
+ *     parameter is documented as host memory.  Complex numbers are interleaved
+ *     (re, im) doubles.  Matrices are row-major.
+ *   - All calls are asynchronous on the given HIP stream (NULL = default
+ *     stream), except jv_eigh/jv_minsr_solve/jv_tdvp_solve, which read a small
+ *     status word back to the host.
+ *   - The library keeps no per-job state; a per-device workspace for rocSOLVER
+ *     and solver temporaries is grown on demand and released by jv_shutdown().
+ *
+ * Each entry point names the jVMC 1.5.8 interface it replaces (file:line).
+ */
+#ifndef JVMC_HIP_H
+#define JVMC_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define JV_OK 0
+#define JV_E_ARG (-1)      /* invalid argument / unsupported configuration */
+#define JV_E_HIP (-2)      /* HIP runtime error */
+#define JV_E_SOLVER (-3)   /* rocSOLVER failure (info != 0) */
+#define JV_E_NOMEM (-4)    /* allocation failure or LDS budget exceeded */
+
+#define JV_NET_CPXRBM 1
+#define JV_NET_RBM 2
+#define JV_NET_CNN 3
+
+#define JV_MAX_LAYERS 8
+
+/* Network descriptor.  The caller fills the "input" fields; jv_net_init()
+ * validates them and fills the "derived" fields (parameter layout).  The flat
+ * real parameter vector follows jVMC's layout (vqs.py:430-466): leaves in JAX
+ * tree_flatten order (dict keys sorted as strings), each complex leaf stored as
+ * [Re(ravel), Im(ravel)]. */
+typedef struct jv_net {
+  /* input */
+  int32_t kind;                       /* JV_NET_* */
+  int32_t ndim;                       /* lattice rank, 1 or 2 */
+  int32_t shape[2];                   /* lattice extents; shape[1] = 1 when ndim == 1 */
+  int32_t num_hidden;                 /* RBMs: hidden units */
+  int32_t hidden_bias;                /* RBMs: 0/1 (rbm.py:32 use_bias) */
+  int32_t visible_bias;               /* CpxRBM only: 0/1 (extension, not in jVMC) */
+  int32_t num_layers;                 /* CNN: number of conv layers (cnn.py:159) */
+  int32_t filter[2];                  /* CNN: filter extents F (cnn.py:122) */
+  int32_t channels[JV_MAX_LAYERS];    /* CNN: output channels per layer */
+  int32_t layer_bias[JV_MAX_LAYERS];  /* CNN: bias per layer (cnn.py:144-145) */
+  /* derived (jv_net_init) */
+  int32_t num_sites;                  /* shape[0] * shape[1] */
+  int32_t is_complex;                 /* complex (holomorphic) parameters */
+  int64_t num_params;                 /* parameters as the network sees them (complex count for CpxRBM) */
+  int64_t num_real_params;            /* length of the flat real vector */
+  int64_t off_hidden_bias;            /* RBMs: offset of leaf in the flat vector, -1 if absent */
+  int64_t off_kernel;                 /* RBMs */
+  int64_t off_visible_bias;           /* CpxRBM */
+  int64_t off_conv_kernel[JV_MAX_LAYERS]; /* CNN */
+  int64_t off_conv_bias[JV_MAX_LAYERS];   /* CNN, -1 if absent */
+} jv_net;
+
+/* Compiled operator table (branch_free.py:348-414).  All pointers are device
+ * pointers.  Strings are padded to max_len factors with the identity on site 0;
+ * factor k of string o is applied k-th (right-to-left order of the string). */
+typedef struct jv_optable {
+  int32_t num_ops;
+  int32_t max_len;
+  int32_t l_dim;             /* local dimension, must be 2 */
+  int32_t num_diag;
+  const int32_t* idx;        /* [num_ops][max_len] site of each factor */
+  const int32_t* map;        /* [num_ops][max_len][l_dim] new local state */
+  const double* matel;       /* [num_ops][max_len][l_dim] complex factor */
+  const int32_t* diag;       /* [num_diag] indices of all-diagonal strings */
+} jv_optable;
+
+/* Metropolis arguments (sampler.py:103-104, 195-358). */
+typedef struct jv_mc_args {
+  int32_t num_chains;
+  int32_t therm_steps;        /* thermalizationSweeps * sweepSteps */
+  int32_t samples_per_chain;  /* from distribute_sampling (mpi_wrapper.py:53-95) */
+  int32_t sweep_steps;
+  double mu;
+  uint64_t seed;              /* Philox4x32-10 key */
+  uint32_t call_index;        /* Philox counter word 2; advance once per call */
+  uint32_t stream;            /* Philox counter word 3 */
+} jv_mc_args;
+
+/* Solver outputs copied to host memory by the solve calls. */
+typedef struct jv_solve_info {
+  double residual;           /* tdvp_residual (tdvp.py:209) */
+  double pinv_cutoff;        /* max(cutoff, pinvCutoff) (tdvp.py:213) */
+  int32_t iterations;        /* regulariser iterations taken (tdvp.py:198) */
+  int32_t rank;              /* MinSR: eigenvalues kept by the pseudo-inverse */
+  double e_var;              /* TDVP: sum_n |e_n|^2 = Var E_loc (stats.py:248-252) */
+  double tdvp_error;         /* TDVP: tdvp_error metadata (tdvp.py:102-104) */
+} jv_solve_info;
+
+/* ------------------------------------------------------------------ misc */
+int jv_version(void);
+const char* jv_last_error(void);
+/* Selects the device for the calling thread.  Replaces the rank-to-device pick
+ * of global_defs.py:15-19. */
+int jv_set_device(int device);
+/* Releases the per-device workspaces and solver handles. */
+int jv_shutdown(void);
+
+/* ------------------------------------------------------------------ networks */
+/* Validates a descriptor and computes the flat parameter layout (host only).
+ * Replaces NQS.init_net's shape bookkeeping (vqs.py:187-218). */
+int jv_net_init(jv_net* net);
+
+/* log psi(s) for n configurations (int32 [n][num_sites]); out: complex [n].
+ * Replaces NQS.__call__ / NQS._eval (vqs.py:223-251) with CpxRBM.__call__
+ * (rbm.py:30-38), RBM.__call__ (rbm.py:80-88) or CNN.__call__ (cnn.py:131-171). */
+int jv_net_forward(const jv_net* net, const double* params, const int32_t* configs,
+                   int64_t n, double* logpsi, void* stream);
+
+/* Per-sample O_k = d log psi / d theta_k.  layout 0 ("compact"): complex nets
+ * write complex [n][num_params] (one derivative per complex parameter), real
+ * nets write real [n][num_params].  layout 1 ("jvmc"): complex [n][num_real_params]
+ * in jVMC's flat layout ([g, i*g] per leaf for holomorphic nets, vqs.py:66-69;
+ * g + 0i for real nets, vqs.py:46-51).  logpsi may be NULL.
+ * Replaces NQS.gradients / NQS._get_gradients (vqs.py:256-287). */
+int jv_net_gradients(const jv_net* net, const double* params, const int32_t* configs,
+                     int64_t n, int32_t layout, double* grad, double* logpsi, void* stream);
+
+/* ------------------------------------------------------------------ operators */
+/* s' enumeration with nonzero compaction.  Outputs, per sample, all num_ops
+ * slots: the first count[n] slots hold the strings with |m| > 1e-6 in string
+ * order, the rest repeat the last string's s' with m = 0; sp: int32
+ * [n][num_ops][num_sites], matel: complex [n][num_ops], count: int32 [n].
+ * prefactor: complex [num_ops] (device), the host-evaluated f(*args).
+ * Replaces BranchFreeOperator._get_s_primes (branch_free.py:443-487) and
+ * Operator.get_s_primes / _find_nonzero / set_zero_to_zero (base.py:91-160). */
+int jv_get_s_primes(const jv_optable* op, const double* prefactor, const int32_t* configs,
+                    int64_t n, int32_t num_sites, int32_t* sp, double* matel, int32_t* count,
+                    void* stream);
+
+/* Fused local estimator: O_loc(s) = sum_j m_j exp(log psi(s'_j) - logpsi_s) with
+ * the s' enumerated in-kernel, never materialised, and log psi(s') obtained by
+ * incremental (RBM) or receptive-field-limited (CNN) re-evaluation.
+ * logpsi_s: complex [n]; out: complex [n].
+ * Replaces Operator.get_O_loc / _get_O_loc / get_O_loc_batched (base.py:162-278). */
+int jv_eloc(const jv_net* net, const double* params, const jv_optable* op,
+            const double* prefactor, const int32_t* configs, const double* logpsi_s,
+            int64_t n, double* out, void* stream);
+
+/* ------------------------------------------------------------------ sampling */
+/* Metropolis sampling with single-spin-flip proposals.  walkers: int32
+ * [num_chains][num_sites], updated in place.  configs: int32
+ * [samples_per_chain * num_chains][num_sites] with sample index t*C + c;
+ * logpsi: complex [samples_per_chain * num_chains]; counters: int64 [2]
+ * (proposed, accepted), overwritten.
+ * Replaces MCSampler._get_samples_mcmc / _get_samples / _sweep / _mc_init and
+ * propose_spin_flip (sampler.py:15-19, 264-367). */
+int jv_mc_sample(const jv_net* net, const double* params, const jv_mc_args* args,
+                 int32_t* walkers, int32_t* configs, double* logpsi, int64_t* counters,
+                 void* stream);
+
+/* Exact-sampler weights: p_k = exp(Re(logpsi_k - last_norm)/log_prob_factor),
+ * normalised; *norm_out (host) receives the unnormalised sum.
+ * Replaces ExactSampler._compute_probabilities / sample (sampler.py:474-526). */
+int jv_exact_weights(const double* logpsi, int64_t n, double last_norm, double log_prob_factor,
+                     double* p, double* norm_out, void* stream);
+
+/* Metropolis weights p = exp((1/logProbFactor - mu) Re logpsi), normalised
+ * (sampler.py:234-235). */
+int jv_mc_weights(const double* logpsi, int64_t n, double mu, double log_prob_factor,
+                  double* p, void* stream);
+
+/* ------------------------------------------------------------------ statistics */
+/* SampledObs centring: mean = sum_n w_n O_n, data_n = sqrt(w_n) (O_n - mean).
+ * obs: [n][p] real (is_complex 0) or complex (1); data may alias obs.
+ * Replaces SampledObs._compute_data_and_mean (stats.py:197-205). */
+int jv_center(const double* obs, const double* w, int64_t n, int64_t p, int32_t is_complex,
+              double* mean, double* data, void* stream);
+
+/* Column sums of squares: out[j] = sum_n |data[n][j]|^2.
+ * Replaces SampledObs.var (stats.py:248-252). */
+int jv_col_norm2(const double* data, int64_t n, int64_t p, int32_t is_complex, double* out, void* stream);
+
+/* General FP64 GEMM on MFMA: C[M][N] = alpha * opA opB with opA(m, k) =
+ * A[m*sam + k*sak], opB(k, n) = B[k*sbk + n*sbn] (element strides; complex
+ * elements are two doubles), optional conjugation of either operand.  Used for
+ * the products of stats.py:50-83 (covar, covar_data + transform) and
+ * tdvp.py:175-179. */
+int jv_gemm(int64_t M, int64_t N, int64_t K, const double* A, int64_t sam, int64_t sak, int32_t conjA,
+            const double* B, int64_t sbk, int64_t sbn, int32_t conjB, double* C, int64_t ldc,
+            int32_t is_complex, double alpha, void* stream);
+
+/* y = alpha * op(M) x for row-major M [rows][cols]: trans 0 -> M x, trans 1 ->
+ * M^T x (conj 1: M^H x).  Replaces the jnp.dot / matmul GEMVs of tdvp.py:171,211
+ * and minsr.py:65,78. */
+int jv_gemv(int32_t trans, const double* M, const double* x, int64_t rows, int64_t cols, int32_t is_complex,
+            int32_t conj, double alpha, double* y, void* stream);
+
+/* Tangent kernel T = D D^dagger ([n][n]) of centred data D [n][p].
+ * Replaces SampledObs.tangent_kernel (stats.py:332-336) and the T build of
+ * MinSR.solve (minsr.py:59-71). */
+int jv_gram_nn(const double* data, int64_t n, int64_t p, int32_t is_complex, double* T,
+               void* stream);
+
+/* Covariance S = D^dagger D ([p][p]) of centred data D [n][p].
+ * Replaces SampledObs.covar (stats.py:235-245) for the S matrix. */
+int jv_gram_pp(const double* data, int64_t n, int64_t p, int32_t is_complex, double* S,
+               void* stream);
+
+/* Symmetric / Hermitian eigendecomposition (ascending eigenvalues) in FP64 via
+ * rocSOLVER syevd / heevd.  A [n][n] is overwritten by eigenvectors stored as
+ * COLUMNS of the row-major matrix (A[i][k] = component i of eigenvector k).
+ * Replaces jnp.linalg.eigh (tdvp.py:153-171). */
+int jv_eigh(int64_t n, int32_t is_complex, double* A, double* w, void* stream);
+
+/* MinSR update (minsr.py:53-80) from centred data.  Holomorphic (is_complex
+ * with holo=1): T = D D^dagger, update = -D^dagger pinv(T) e with the jVMC flat
+ * layout handled by the caller.  Real-parameter nets (is_complex 0): D real
+ * [n][p], e complex [n]; T = D D^T + shift I (the imaginary stacked half of
+ * minsr.py:68-78 is identically zero and drops out exactly), update real [p].
+ * pinv keeps |lambda| > rcond * max|lambda|.  info (host) may be NULL. */
+int jv_minsr_solve(const double* data, const double* e, int64_t n, int64_t p,
+                   int32_t is_complex, double rcond, double diag_shift, double* update,
+                   jv_solve_info* info, void* stream);
+
+/* TDVP solve (tdvp.py:134-213).  data: centred O in jVMC's flat layout, complex
+ * [n][pr]; e: centred E_loc, complex [n]; w: weights [n]; x = rhsPrefactor
+ * (complex, host); make_real 0 = 'real', 1 = 'imag'.  use_snr selects the SNR
+ * term of the regulariser (MCSampler) and glob_num_samples is mpi.globNumSamples.
+ * Outputs (device): update real [pr], S (makeReal'd, complex [pr][pr]), F (complex
+ * [pr]), ev [pr], snr [pr]; E mean / variance complex (device, 2 values each).
+ * Replaces TDVP.get_tdvp_equation / _transform_to_eigenbasis / _get_snr / solve. */
+int jv_tdvp_solve(const double* data, const double* e, const double* w, int64_t n, int64_t pr,
+                  const double* x, int32_t make_real, double diag_shift, int32_t use_snr,
+                  double glob_num_samples, double snr_tol, double pinv_tol, double pinv_cutoff,
+                  double* update, double* S, double* F, double* ev, double* snr,
+                  jv_solve_info* info, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* JVMC_HIP_H */

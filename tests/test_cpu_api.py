@@ -1,0 +1,140 @@
+"""CPU-side checks of the boundary: the C-ABI library loads, exports every
+symbol declared in include/jvmc_hip.h, and its host-only layout logic agrees
+with the oracle (no GPU compute calls here)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import nets as onets
+from oracle import operator as oop
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "jvmc_hip.h")
+LIB = os.path.join(ROOT, "vmc_jax_amd", "libjvmc_hip.so")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(jv_\w+)\s*\(", text, flags=re.M)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(LIB):
+        pytest.fail("libjvmc_hip.so is not built; run __graft_entry__.build()")
+    from vmc_jax_amd import _lib
+    return _lib
+
+
+def test_library_exports_every_declared_symbol(lib):
+    syms = declared_symbols()
+    assert len(syms) >= 20
+    out = subprocess.run(["nm", "-D", "--defined-only", LIB], capture_output=True, text=True, check=True).stdout
+    exported = set(re.findall(r"\bT (jv_\w+)", out))
+    missing = [s for s in syms if s not in exported]
+    assert not missing, missing
+    for s in syms:
+        assert hasattr(lib.lib, s)
+    # the Python binding declares exactly the header's entry points
+    assert sorted(lib.exported_symbols()) == syms
+
+
+def test_no_torch_types_in_header():
+    text = open(HEADER).read()
+    assert "torch" not in text.lower() and "at::" not in text
+
+
+def _init(lib, d):
+    rc = lib.lib.jv_net_init(ctypes.byref(d))
+    assert rc == 0, lib.lib.jv_last_error()
+    return d
+
+
+def test_cnn_layout_matches_oracle(lib):
+    from vmc_jax_amd import nets
+    for shape, F, ch in [((10, 10), (3, 3), (8, 8, 8)), ((6,), (3,), (3, 2)),
+                         ((4, 4), (2, 2), tuple([2] * 11 if False else [2] * 8))]:
+        net = nets.CNN(F=F, channels=ch, strides=(1,) * len(shape))
+        d = _init(lib, net.descriptor(shape))
+        o = onets.CNN(shape, F=F, channels=ch, strides=(1,) * len(shape))
+        assert d.num_real_params == o.num_real_params()
+        off = 0
+        for leaf in o.leaves:
+            layer = int(leaf.name.split("/")[0].split("_")[1])
+            if leaf.name.endswith("kernel"):
+                assert d.off_conv_kernel[layer] == off, leaf.name
+            else:
+                assert d.off_conv_bias[layer] == off, leaf.name
+            off += leaf.size
+
+
+def test_rbm_layout_matches_oracle(lib):
+    from vmc_jax_amd import nets
+    for bias, vb in [(False, False), (True, False), (True, True)]:
+        d = _init(lib, nets.CpxRBM(numHidden=20, bias=bias, visibleBias=vb).descriptor((10,)))
+        o = onets.CpxRBM((10,), numHidden=20, bias=bias, visibleBias=vb)
+        assert d.num_real_params == o.num_real_params()
+        assert d.num_params == o.num_params()
+    d = _init(lib, nets.CpxRBM(numHidden=20, bias=True, visibleBias=True).descriptor((10,)))
+    assert (d.off_hidden_bias, d.off_kernel, d.off_visible_bias) == (0, 40, 440)
+    assert d.num_real_params == 460  # C1: 230 complex parameters
+
+
+def test_init_rejects_bad_descriptors(lib):
+    from vmc_jax_amd import _lib as L
+    d = L.JvNet()
+    d.kind = 99
+    d.ndim = 1
+    d.shape[0] = 4
+    assert lib.lib.jv_net_init(ctypes.byref(d)) == -1
+    assert b"unknown" in lib.lib.jv_last_error()
+    d = L.JvNet()
+    d.kind = L.JV_NET_CNN
+    d.ndim = 2
+    d.shape[0] = d.shape[1] = 64
+    d.num_layers = 3
+    d.filter[0] = d.filter[1] = 3
+    for l in range(3):
+        d.channels[l] = 64
+        d.layer_bias[l] = 1
+    assert lib.lib.jv_net_init(ctypes.byref(d)) == -4  # exceeds one workgroup's LDS
+
+
+def test_operator_tables_match_oracle():
+    from vmc_jax_amd import operator as dop
+    L = 5
+    H, Ho = dop.BranchFreeOperator(), oop.BranchFreeOperator()
+    for i in range(L):
+        H.add(dop.scal_opstr(2., (dop.Sx(i),)))
+        H.add(dop.scal_opstr(2., (dop.Sy(i), dop.Sz((i + 1) % L))))
+        H.add((dop.Sz(i),))
+        Ho.add(oop.scal_opstr(2., (oop.Sx(i),)))
+        Ho.add(oop.scal_opstr(2., (oop.Sy(i), oop.Sz((i + 1) % L))))
+        Ho.add((oop.Sz(i),))
+    H.compile()
+    Ho.compile()
+    np.testing.assert_array_equal(H.idxH, Ho.idxC)
+    np.testing.assert_array_equal(H.mapH, Ho.mapC)
+    np.testing.assert_array_equal(H.matElsH, Ho.matElsC)
+    np.testing.assert_array_equal(H.diagH, Ho.diag)
+    f = lambda t: 3.0 * t
+    H2 = dop.BranchFreeOperator()
+    H2.add(dop.scal_opstr(f, (dop.Sp(0),)))
+    H2.add(dop.scal_opstr(0.5, (dop.Sx(1),)))
+    H2.compile()
+    res = np.ones(2, np.complex128)
+    for i, fn in H2.prefactor:
+        res[i] = fn(2.0)
+    np.testing.assert_allclose(res, [6.0, 0.5])
+
+
+def test_distribute_sampling_semantics():
+    from vmc_jax_amd import mpi_wrapper as mpi
+    assert mpi.distribute_sampling(8192, 1, 512) == (16, 8192)
+    assert mpi.globNumSamples == 8192
+    assert mpi.distribute_sampling(1000, 1, 300) == (4, 1200)
+    assert mpi.distribute_sampling(1024) == (1024, 1024)

@@ -1,0 +1,125 @@
+"""ctypes binding of libjvmc_hip.so (the gfx950 HIP implementation).
+
+The shared library is built in-tree by ``vmc_jax_amd/csrc/build.sh`` (or
+``__graft_entry__.build()``).  There is no fallback: every compute entry point
+of this package goes through the library, and importing it fails loudly when
+the library is missing.  torch is imported first so that the HIP runtime the
+library binds to is the one torch already loaded.
+"""
+import ctypes
+import os
+
+import torch  # noqa: F401  (loads the HIP runtime first)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libjvmc_hip.so")
+
+JV_NET_CPXRBM = 1
+JV_NET_RBM = 2
+JV_NET_CNN = 3
+JV_MAX_LAYERS = 8
+
+_i32 = ctypes.c_int32
+_i64 = ctypes.c_int64
+_dbl = ctypes.c_double
+_vp = ctypes.c_void_p
+
+
+class JvNet(ctypes.Structure):
+    _fields_ = [
+        ("kind", _i32), ("ndim", _i32), ("shape", _i32 * 2), ("num_hidden", _i32),
+        ("hidden_bias", _i32), ("visible_bias", _i32), ("num_layers", _i32), ("filter", _i32 * 2),
+        ("channels", _i32 * JV_MAX_LAYERS), ("layer_bias", _i32 * JV_MAX_LAYERS),
+        ("num_sites", _i32), ("is_complex", _i32), ("num_params", _i64), ("num_real_params", _i64),
+        ("off_hidden_bias", _i64), ("off_kernel", _i64), ("off_visible_bias", _i64),
+        ("off_conv_kernel", _i64 * JV_MAX_LAYERS), ("off_conv_bias", _i64 * JV_MAX_LAYERS),
+    ]
+
+
+class JvOpTable(ctypes.Structure):
+    _fields_ = [("num_ops", _i32), ("max_len", _i32), ("l_dim", _i32), ("num_diag", _i32),
+                ("idx", _vp), ("map", _vp), ("matel", _vp), ("diag", _vp)]
+
+
+class JvMcArgs(ctypes.Structure):
+    _fields_ = [("num_chains", _i32), ("therm_steps", _i32), ("samples_per_chain", _i32),
+                ("sweep_steps", _i32), ("mu", _dbl), ("seed", ctypes.c_uint64),
+                ("call_index", ctypes.c_uint32), ("stream", ctypes.c_uint32)]
+
+
+class JvSolveInfo(ctypes.Structure):
+    _fields_ = [("residual", _dbl), ("pinv_cutoff", _dbl), ("iterations", _i32), ("rank", _i32)]
+
+
+# name -> argtypes; every function returns int status
+_SIGS = {
+    "jv_version": [],
+    "jv_set_device": [ctypes.c_int],
+    "jv_shutdown": [],
+    "jv_net_init": [ctypes.POINTER(JvNet)],
+    "jv_net_forward": [ctypes.POINTER(JvNet), _vp, _vp, _i64, _vp, _vp],
+    "jv_net_gradients": [ctypes.POINTER(JvNet), _vp, _vp, _i64, _i32, _vp, _vp, _vp],
+    "jv_get_s_primes": [ctypes.POINTER(JvOpTable), _vp, _vp, _i64, _i32, _vp, _vp, _vp, _vp],
+    "jv_eloc": [ctypes.POINTER(JvNet), _vp, ctypes.POINTER(JvOpTable), _vp, _vp, _vp, _i64, _vp, _vp],
+    "jv_mc_sample": [ctypes.POINTER(JvNet), _vp, ctypes.POINTER(JvMcArgs), _vp, _vp, _vp, _vp, _vp],
+    "jv_exact_weights": [_vp, _i64, _dbl, _dbl, _vp, ctypes.POINTER(_dbl), _vp],
+    "jv_mc_weights": [_vp, _i64, _dbl, _dbl, _vp, _vp],
+    "jv_center": [_vp, _vp, _i64, _i64, _i32, _vp, _vp, _vp],
+    "jv_col_norm2": [_vp, _i64, _i64, _i32, _vp, _vp],
+    "jv_gram_nn": [_vp, _i64, _i64, _i32, _vp, _vp],
+    "jv_gram_pp": [_vp, _i64, _i64, _i32, _vp, _vp],
+    "jv_gemm": [_i64, _i64, _i64, _vp, _i64, _i64, _i32, _vp, _i64, _i64, _i32, _vp, _i64, _i32, _dbl, _vp],
+    "jv_gemv": [_i32, _vp, _vp, _i64, _i64, _i32, _i32, _dbl, _vp, _vp],
+    "jv_eigh": [_i64, _i32, _vp, _vp, _vp],
+    "jv_minsr_solve": [_vp, _vp, _i64, _i64, _i32, _dbl, _dbl, _vp, ctypes.POINTER(JvSolveInfo), _vp],
+    "jv_tdvp_solve": [_vp, _vp, _vp, _i64, _i64, ctypes.POINTER(_dbl), _i32, _dbl, _i32, _dbl, _dbl, _dbl,
+                      _dbl, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(JvSolveInfo), _vp],
+}
+
+
+class HipError(RuntimeError):
+    pass
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("libjvmc_hip.so is not built (%s); run vmc_jax_amd/csrc/build.sh or "
+                          "__graft_entry__.build()" % LIB_PATH)
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, args in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_int
+    lib.jv_last_error.argtypes = []
+    lib.jv_last_error.restype = ctypes.c_char_p
+    return lib
+
+
+lib = _load()
+
+
+def check(rc, what=""):
+    if rc != 0:
+        msg = lib.jv_last_error().decode(errors="replace")
+        raise HipError("%s failed (%d): %s" % (what, rc, msg))
+
+
+def call(name, *args):
+    check(getattr(lib, name)(*args), name)
+
+
+def ptr(t):
+    """Device pointer of a torch tensor (None -> NULL)."""
+    if t is None:
+        return None
+    if not t.is_contiguous():
+        raise ValueError("tensor must be contiguous")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def stream_ptr():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def exported_symbols():
+    return sorted(list(_SIGS.keys()) + ["jv_last_error"])

@@ -1,0 +1,150 @@
+// Shared device helpers for libjvmc_hip: error plumbing, FP64 complex
+// arithmetic, log-cosh, Philox4x32-10 and wavefront reductions (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+
+#include "../../include/jvmc_hip.h"
+
+namespace jv {
+
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+#define JV_HIP(call)                                   \
+  do {                                                 \
+    hipError_t e_ = (call);                            \
+    if (e_ != hipSuccess) return jv::hip_fail(e_, #call); \
+  } while (0)
+
+#define JV_LAUNCH_CHECK(what)                          \
+  do {                                                 \
+    hipError_t e_ = hipGetLastError();                 \
+    if (e_ != hipSuccess) return jv::hip_fail(e_, what); \
+  } while (0)
+
+// Per-device scratch arena (grown on demand, released by jv_shutdown).
+void* workspace(size_t bytes, int slot);
+
+constexpr int WAVE = 64;
+
+// ------------------------------------------------------------------ complex
+struct cd {
+  double re, im;
+};
+__host__ __device__ inline cd mk(double r, double i) { return {r, i}; }
+__host__ __device__ inline cd operator+(cd a, cd b) { return {a.re + b.re, a.im + b.im}; }
+__host__ __device__ inline cd operator-(cd a, cd b) { return {a.re - b.re, a.im - b.im}; }
+__host__ __device__ inline cd operator*(cd a, cd b) {
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+__host__ __device__ inline cd operator*(double s, cd a) { return {s * a.re, s * a.im}; }
+__host__ __device__ inline cd cconj(cd a) { return {a.re, -a.im}; }
+__host__ __device__ inline double cabs2(cd a) { return a.re * a.re + a.im * a.im; }
+
+__device__ inline cd cexp(cd z) {
+  double s, c;
+  sincos(z.im, &s, &c);
+  double e = exp(z.re);
+  return {e * c, e * s};
+}
+
+__device__ inline cd cdiv(cd a, cd b) {
+  // Smith's algorithm
+  if (fabs(b.re) >= fabs(b.im)) {
+    double r = b.im / b.re, d = b.re + b.im * r;
+    return {(a.re + a.im * r) / d, (a.im - a.re * r) / d};
+  }
+  double r = b.re / b.im, d = b.re * r + b.im;
+  return {(a.re * r + a.im) / d, (a.im * r - a.re) / d};
+}
+
+// log(1 + w) for |w| <= 1 (argument of the log-cosh tail).
+__device__ inline cd clog1p(cd w) {
+  double x = w.re, y = w.im;
+  // |1+w|^2 - 1 = 2x + x^2 + y^2, evaluated without cancellation for small w
+  double t = fma(x, x, fma(y, y, 2.0 * x));
+  return {0.5 * log1p(t), atan2(y, 1.0 + x)};
+}
+
+constexpr double LN2 = 0.69314718055994530942;
+
+// log(cosh(x)) = x*sgn + log1p(exp(-2 x sgn)) - log 2, sgn from the sign bit of
+// Re x (activation_functions.py:19-22).
+__device__ inline cd log_cosh(cd x) {
+  double sg = signbit(x.re) ? -1.0 : 1.0;
+  cd y = {x.re * sg, x.im * sg};
+  cd e = cexp({-2.0 * y.re, -2.0 * y.im});
+  cd l = clog1p(e);
+  return {y.re + l.re - LN2, y.im + l.im};
+}
+// Real part only: Re log cosh(x) = a + 0.5 log1p(2 e^{-2a} cos 2b + e^{-4a}) - log 2
+__device__ inline double re_log_cosh(cd x) {
+  double a = fabs(x.re);
+  double b = x.im;
+  double e2 = exp(-2.0 * a);
+  double t = fma(2.0 * e2, cos(2.0 * b), e2 * e2);
+  return a + 0.5 * log1p(t) - LN2;
+}
+__device__ inline double log_cosh(double x) {
+  double a = fabs(x);
+  return a + log1p(exp(-2.0 * a)) - LN2;
+}
+// tanh = d log_cosh / dx, via exp(-2|Re x|) so it never overflows.
+__device__ inline cd ctanh(cd x) {
+  double sg = signbit(x.re) ? -1.0 : 1.0;
+  cd e = cexp({-2.0 * sg * x.re, -2.0 * sg * x.im});
+  cd t = cdiv({1.0 - e.re, -e.im}, {1.0 + e.re, e.im});
+  return {sg * t.re, sg * t.im};
+}
+
+// ------------------------------------------------------------------ Philox4x32-10
+// Salmon et al., SC'11.  Same counter layout as the oracle.
+__device__ inline void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+    uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+    uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+    c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+}
+
+// Site in [0, n) and a 53-bit uniform in [0, 1) for (step, chain, call, stream).
+__device__ inline void mc_draw(uint64_t seed, uint32_t step, uint32_t chain, uint32_t call,
+                               uint32_t stream, int n, int& site, double& u) {
+  uint32_t c[4] = {step, chain, call, stream};
+  philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+  site = (int)(((uint64_t)c[0] * (uint64_t)n) >> 32);
+  double a = (double)(c[1] >> 5), b = (double)(c[2] >> 6);
+  u = (a * 67108864.0 + b) * (1.0 / 9007199254740992.0);
+}
+
+// ------------------------------------------------------------------ reductions
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Block-wide sum; red must hold blockDim.x / 64 doubles.  All threads get the sum.
+__device__ inline double block_sum(double v, double* red) {
+  v = wave_sum(v);
+  int w = threadIdx.x / WAVE, l = threadIdx.x % WAVE;
+  __syncthreads();
+  if (l == 0) red[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  int nw = blockDim.x / WAVE;
+  for (int i = 0; i < nw; ++i) s += red[i];
+  __syncthreads();
+  return s;
+}
+
+}  // namespace jv

@@ -1,0 +1,111 @@
+// Error plumbing, per-device workspace and solver handles for libjvmc_hip.
+#include <rocblas/rocblas.h>
+
+#include <mutex>
+#include <vector>
+
+#include "common.hpp"
+
+namespace jv {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+  g_last_error = std::string(what) + ": " + hipGetErrorString(e);
+  return JV_E_HIP;
+}
+
+namespace {
+struct DeviceState {
+  std::vector<void*> ptr;
+  std::vector<size_t> size;
+  rocblas_handle blas = nullptr;
+};
+std::mutex g_mu;
+std::vector<DeviceState> g_dev;
+
+DeviceState& state() {
+  int d = 0;
+  (void)hipGetDevice(&d);
+  if ((int)g_dev.size() <= d) g_dev.resize(d + 1);
+  return g_dev[d];
+}
+}  // namespace
+
+// Grow-only scratch buffer `slot` on the current device.  Freeing an old buffer
+// synchronises the device first so in-flight kernels never see it vanish.
+void* workspace(size_t bytes, int slot) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  DeviceState& s = state();
+  if ((int)s.ptr.size() <= slot) {
+    s.ptr.resize(slot + 1, nullptr);
+    s.size.resize(slot + 1, 0);
+  }
+  if (s.size[slot] < bytes) {
+    if (s.ptr[slot]) {
+      (void)hipDeviceSynchronize();
+      (void)hipFree(s.ptr[slot]);
+      s.ptr[slot] = nullptr;
+      s.size[slot] = 0;
+    }
+    size_t b = bytes < 256 ? 256 : bytes;
+    if (hipMalloc(&s.ptr[slot], b) != hipSuccess) {
+      s.ptr[slot] = nullptr;
+      return nullptr;
+    }
+    s.size[slot] = b;
+  }
+  return s.ptr[slot];
+}
+
+rocblas_handle blas_handle(hipStream_t stream) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  DeviceState& s = state();
+  if (!s.blas) {
+    if (rocblas_create_handle(&s.blas) != rocblas_status_success) return nullptr;
+  }
+  rocblas_set_stream(s.blas, stream);
+  return s.blas;
+}
+
+}  // namespace jv
+
+extern "C" {
+
+int jv_version(void) { return 100; }
+
+const char* jv_last_error(void) { return jv::g_last_error.c_str(); }
+
+int jv_set_device(int device) {
+  JV_HIP(hipSetDevice(device));
+  return JV_OK;
+}
+
+int jv_shutdown(void) {
+  std::lock_guard<std::mutex> lk(jv::g_mu);
+  int cur = 0;
+  (void)hipGetDevice(&cur);
+  for (size_t d = 0; d < jv::g_dev.size(); ++d) {
+    auto& s = jv::g_dev[d];
+    if (s.ptr.empty() && !s.blas) continue;
+    (void)hipSetDevice((int)d);
+    (void)hipDeviceSynchronize();
+    for (void* p : s.ptr)
+      if (p) (void)hipFree(p);
+    s.ptr.clear();
+    s.size.clear();
+    if (s.blas) rocblas_destroy_handle(s.blas);
+    s.blas = nullptr;
+  }
+  (void)hipSetDevice(cur);
+  return JV_OK;
+}
+
+}  // extern "C"

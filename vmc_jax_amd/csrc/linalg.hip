@@ -1,0 +1,794 @@
+// Sample statistics, FP64 MFMA Gram / GEMM kernels, rocSOLVER eigendecompositions
+// and the MinSR / TDVP solves.
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "common.hpp"
+
+namespace jv {
+
+rocblas_handle blas_handle(hipStream_t stream);
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+// ------------------------------------------------------------------ centring
+// Column partial sums of w_n * O[n][j] over row chunks (deterministic order).
+constexpr int CHUNK_ROWS = 256;
+
+__global__ void col_wsum_partial(const double* __restrict__ O, const double* __restrict__ w, int64_t n,
+                                 int64_t p, int cpx, double* __restrict__ part) {
+  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= p) return;
+  int64_t r0 = (int64_t)blockIdx.y * CHUNK_ROWS, r1 = std::min<int64_t>(n, r0 + CHUNK_ROWS);
+  double ar = 0.0, ai = 0.0;
+  if (cpx) {
+    for (int64_t r = r0; r < r1; ++r) {
+      ar = fma(w[r], O[2 * (r * p + j)], ar);
+      ai = fma(w[r], O[2 * (r * p + j) + 1], ai);
+    }
+    part[2 * (blockIdx.y * p + j)] = ar;
+    part[2 * (blockIdx.y * p + j) + 1] = ai;
+  } else {
+    for (int64_t r = r0; r < r1; ++r) ar = fma(w[r], O[r * p + j], ar);
+    part[blockIdx.y * p + j] = ar;
+  }
+}
+
+__global__ void col_reduce(const double* __restrict__ part, int nchunks, int64_t len, double* __restrict__ out) {
+  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= len) return;
+  double a = 0.0;
+  for (int c = 0; c < nchunks; ++c) a += part[c * len + j];
+  out[j] = a;
+}
+
+__global__ void center_apply(const double* __restrict__ O, const double* __restrict__ w,
+                             const double* __restrict__ mean, int64_t n, int64_t p, int cpx,
+                             double* __restrict__ D) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t len = p * (cpx ? 2 : 1);
+  if (t >= n * len) return;
+  int64_t r = t / len, j = t % len;
+  D[t] = sqrt(w[r]) * (O[t] - mean[j]);
+}
+
+int center(const double* O, const double* w, int64_t n, int64_t p, int cpx, double* mean, double* D,
+           hipStream_t st) {
+  int nchunks = (int)((n + CHUNK_ROWS - 1) / CHUNK_ROWS);
+  int64_t len = p * (cpx ? 2 : 1);
+  double* part = (double*)workspace(sizeof(double) * (size_t)nchunks * len, 2);
+  if (!part) return fail(JV_E_NOMEM, "jv_center: workspace allocation failed");
+  hipLaunchKernelGGL(col_wsum_partial, dim3((unsigned)((p + 255) / 256), nchunks), dim3(256), 0, st, O, w, n,
+                     p, cpx, part);
+  JV_LAUNCH_CHECK("col_wsum_partial");
+  hipLaunchKernelGGL(col_reduce, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, st, part, nchunks, len, mean);
+  JV_LAUNCH_CHECK("col_reduce");
+  hipLaunchKernelGGL(center_apply, dim3((unsigned)((n * len + 255) / 256)), dim3(256), 0, st, O, w, mean, n, p,
+                     cpx, D);
+  JV_LAUNCH_CHECK("center_apply");
+  return JV_OK;
+}
+
+// ------------------------------------------------------------------ GEMM (FP64 MFMA)
+// C[M][N] = alpha * opA * opB (+ beta C), opA(m, k) = A[m*sam + k*sak],
+// opB(k, n) = B[k*sbk + n*sbn] (strides in elements; complex elements are two
+// doubles), optional conjugation of either operand.  sym: C is Hermitian;
+// only tiles with tm <= tn are computed and mirrored.
+struct GemmArgs {
+  int64_t M, N, K;
+  const double* A;
+  int64_t sam, sak;
+  int conjA;
+  const double* B;
+  int64_t sbk, sbn;
+  int conjB;
+  double* C;
+  int64_t ldc;
+  int cpx;
+  int sym;
+  double alpha;
+};
+
+constexpr int BM = 64, BN = 64, BK = 16;
+
+// MFMA 16x16x4 f64 operand / result lane layout (measured on gfx950): A[row =
+// lane%16][k = lane/16], B[k = lane/16][col = lane%16], result element j of lane
+// l is C[4j + l/16][l%16].
+__device__ inline int mf_arow(int lane) { return lane % 16; }
+__device__ inline int mf_ak(int lane) { return lane / 16; }
+__device__ inline int mf_bcol(int lane) { return lane % 16; }
+__device__ inline int mf_bk(int lane) { return lane / 16; }
+__device__ inline int mf_crow(int lane, int j) { return 4 * j + lane / 16; }
+__device__ inline int mf_ccol(int lane, int j) { return lane % 16; }
+
+template <bool CPX>
+__global__ void __launch_bounds__(256) gemm_f64(GemmArgs g) {
+  __shared__ double As[2][BK][BM + 1];
+  __shared__ double Bs[2][BK][BN + 1];
+  int tm, tn;
+  const int64_t ntn = (g.N + BN - 1) / BN;
+  if (g.sym) {
+    // linear block id -> (tm, tn) with tm <= tn
+    int64_t b = blockIdx.x, row = 0, rowlen = ntn;
+    while (b >= rowlen) { b -= rowlen; ++row; --rowlen; }
+    tm = (int)row;
+    tn = (int)(row + b);
+  } else {
+    tm = blockIdx.x / ntn;
+    tn = blockIdx.x % ntn;
+  }
+  const int tid = threadIdx.x, lane = tid % 64, wv = tid / 64;
+  const int wm = (wv / 2) * 32, wn = (wv % 2) * 32;  // wave sub-tile origin
+  const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
+  v4d accr[2][2], acci[2][2];
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) {
+      accr[a][b] = v4d{0, 0, 0, 0};
+      acci[a][b] = v4d{0, 0, 0, 0};
+    }
+  const double sa_im = g.conjA ? -1.0 : 1.0, sb_im = g.conjB ? -1.0 : 1.0;
+  const bool a_k_fast = g.sak == 1, b_k_fast = g.sbk == 1;
+  for (int64_t k0 = 0; k0 < g.K; k0 += BK) {
+    // global -> LDS (zero padded), each thread moves 4 elements of each tile
+    for (int r = 0; r < (BM * BK) / 256; ++r) {
+      int e = tid + 256 * r;
+      int mm = a_k_fast ? e / BK : e % BM;
+      int kk = a_k_fast ? e % BK : e / BM;
+      int64_t gm = m0 + mm, gk = k0 + kk;
+      double vr = 0.0, vi = 0.0;
+      if (gm < g.M && gk < g.K) {
+        int64_t off = gm * g.sam + gk * g.sak;
+        if (CPX) { vr = g.A[2 * off]; vi = sa_im * g.A[2 * off + 1]; }
+        else vr = g.A[off];
+      }
+      As[0][kk][mm] = vr;
+      if (CPX) As[1][kk][mm] = vi;
+      int nn = b_k_fast ? e / BK : e % BN;
+      int kb = b_k_fast ? e % BK : e / BN;
+      int64_t gn = n0 + nn, gkb = k0 + kb;
+      vr = 0.0; vi = 0.0;
+      if (gn < g.N && gkb < g.K) {
+        int64_t off = gkb * g.sbk + gn * g.sbn;
+        if (CPX) { vr = g.B[2 * off]; vi = sb_im * g.B[2 * off + 1]; }
+        else vr = g.B[off];
+      }
+      Bs[0][kb][nn] = vr;
+      if (CPX) Bs[1][kb][nn] = vi;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < BK; ks += 4) {
+      double ar[2], ai[2], br[2], bi[2];
+      for (int a = 0; a < 2; ++a) {
+        ar[a] = As[0][ks + mf_ak(lane)][wm + 16 * a + mf_arow(lane)];
+        if (CPX) ai[a] = As[1][ks + mf_ak(lane)][wm + 16 * a + mf_arow(lane)];
+      }
+      for (int b = 0; b < 2; ++b) {
+        br[b] = Bs[0][ks + mf_bk(lane)][wn + 16 * b + mf_bcol(lane)];
+        if (CPX) bi[b] = Bs[1][ks + mf_bk(lane)][wn + 16 * b + mf_bcol(lane)];
+      }
+      for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) {
+          accr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], br[b], accr[a][b], 0, 0, 0);
+          if (CPX) {
+            accr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[a], bi[b], accr[a][b], 0, 0, 0);
+            acci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], bi[b], acci[a][b], 0, 0, 0);
+            acci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[a], br[b], acci[a][b], 0, 0, 0);
+          }
+        }
+    }
+    __syncthreads();
+  }
+  // epilogue
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b)
+      for (int j = 0; j < 4; ++j) {
+        int64_t i = m0 + wm + 16 * a + mf_crow(lane, j);
+        int64_t c = n0 + wn + 16 * b + mf_ccol(lane, j);
+        if (i >= g.M || c >= g.N) continue;
+        double vr = g.alpha * accr[a][b][j];
+        double vi = CPX ? g.alpha * acci[a][b][j] : 0.0;
+        if (g.sym && tm == tn && i > c) continue;
+        if (CPX) {
+          g.C[2 * (i * g.ldc + c)] = vr;
+          g.C[2 * (i * g.ldc + c) + 1] = vi;
+          if (g.sym && i != c) {
+            g.C[2 * (c * g.ldc + i)] = vr;
+            g.C[2 * (c * g.ldc + i) + 1] = -vi;
+          }
+        } else {
+          g.C[i * g.ldc + c] = vr;
+          if (g.sym && i != c) g.C[c * g.ldc + i] = vr;
+        }
+      }
+}
+
+int gemm(const GemmArgs& g, hipStream_t st) {
+  if (g.M <= 0 || g.N <= 0) return JV_OK;
+  const int64_t ntm = (g.M + BM - 1) / BM, ntn = (g.N + BN - 1) / BN;
+  int64_t blocks = g.sym ? ntn * (ntn + 1) / 2 : ntm * ntn;
+  if (g.sym && (g.M != g.N)) return fail(JV_E_ARG, "symmetric GEMM needs a square output");
+  if (g.cpx) hipLaunchKernelGGL(gemm_f64<true>, dim3((unsigned)blocks), dim3(256), 0, st, g);
+  else hipLaunchKernelGGL(gemm_f64<false>, dim3((unsigned)blocks), dim3(256), 0, st, g);
+  JV_LAUNCH_CHECK("gemm_f64");
+  return JV_OK;
+}
+
+// ------------------------------------------------------------------ GEMV
+// y[i] = alpha * sum_j M[i][j] x[j] (op 'N'), one wave per row.
+template <bool CPX>
+__global__ void gemv_n(const double* __restrict__ M, const double* __restrict__ x, int64_t rows, int64_t cols,
+                       int conjM, double alpha, double* __restrict__ y) {
+  const int lane = threadIdx.x % 64;
+  const int64_t r = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+  if (r >= rows) return;
+  double ar = 0.0, ai = 0.0;
+  for (int64_t j = lane; j < cols; j += 64) {
+    if (CPX) {
+      double mr = M[2 * (r * cols + j)], mi = M[2 * (r * cols + j) + 1];
+      if (conjM) mi = -mi;
+      double xr = x[2 * j], xi = x[2 * j + 1];
+      ar += mr * xr - mi * xi;
+      ai += mr * xi + mi * xr;
+    } else {
+      ar = fma(M[r * cols + j], x[j], ar);
+    }
+  }
+  ar = wave_sum(ar);
+  if (CPX) ai = wave_sum(ai);
+  if (lane == 0) {
+    if (CPX) { y[2 * r] = alpha * ar; y[2 * r + 1] = alpha * ai; }
+    else y[r] = alpha * ar;
+  }
+}
+
+// y[j] = alpha * sum_i op(M[i][j]) x[i] (op 'T' or 'C'), row-chunk partials.
+template <bool CPX>
+__global__ void gemv_t_partial(const double* __restrict__ M, const double* __restrict__ x, int64_t rows,
+                               int64_t cols, int conjM, double* __restrict__ part) {
+  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= cols) return;
+  int64_t r0 = (int64_t)blockIdx.y * CHUNK_ROWS, r1 = std::min<int64_t>(rows, r0 + CHUNK_ROWS);
+  double ar = 0.0, ai = 0.0;
+  for (int64_t r = r0; r < r1; ++r) {
+    if (CPX) {
+      double mr = M[2 * (r * cols + j)], mi = M[2 * (r * cols + j) + 1];
+      if (conjM) mi = -mi;
+      double xr = x[2 * r], xi = x[2 * r + 1];
+      ar += mr * xr - mi * xi;
+      ai += mr * xi + mi * xr;
+    } else {
+      ar = fma(M[r * cols + j], x[r], ar);
+    }
+  }
+  if (CPX) {
+    part[2 * (blockIdx.y * cols + j)] = ar;
+    part[2 * (blockIdx.y * cols + j) + 1] = ai;
+  } else {
+    part[blockIdx.y * cols + j] = ar;
+  }
+}
+
+__global__ void scale_vec(double* __restrict__ v, int64_t len, double alpha) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < len) v[i] *= alpha;
+}
+
+int gemv(bool trans, const double* M, const double* x, int64_t rows, int64_t cols, int cpx, int conjM,
+         double alpha, double* y, hipStream_t st, int ws_slot) {
+  if (!trans) {
+    int64_t blocks = (rows + 3) / 4;
+    if (cpx) hipLaunchKernelGGL(gemv_n<true>, dim3((unsigned)blocks), dim3(256), 0, st, M, x, rows, cols, conjM, alpha, y);
+    else hipLaunchKernelGGL(gemv_n<false>, dim3((unsigned)blocks), dim3(256), 0, st, M, x, rows, cols, conjM, alpha, y);
+    JV_LAUNCH_CHECK("gemv_n");
+    return JV_OK;
+  }
+  int nchunks = (int)((rows + CHUNK_ROWS - 1) / CHUNK_ROWS);
+  int64_t len = cols * (cpx ? 2 : 1);
+  double* part = (double*)workspace(sizeof(double) * (size_t)nchunks * len, ws_slot);
+  if (!part) return fail(JV_E_NOMEM, "gemv: workspace allocation failed");
+  if (cpx) hipLaunchKernelGGL(gemv_t_partial<true>, dim3((unsigned)((cols + 255) / 256), nchunks), dim3(256), 0, st, M, x, rows, cols, conjM, part);
+  else hipLaunchKernelGGL(gemv_t_partial<false>, dim3((unsigned)((cols + 255) / 256), nchunks), dim3(256), 0, st, M, x, rows, cols, conjM, part);
+  JV_LAUNCH_CHECK("gemv_t_partial");
+  hipLaunchKernelGGL(col_reduce, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, st, part, nchunks, len, y);
+  JV_LAUNCH_CHECK("col_reduce");
+  if (alpha != 1.0) {
+    hipLaunchKernelGGL(scale_vec, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, st, y, len, alpha);
+    JV_LAUNCH_CHECK("scale_vec");
+  }
+  return JV_OK;
+}
+
+// ------------------------------------------------------------------ eigh
+// rocSOLVER works column-major; a row-major symmetric / Hermitian A is seen as
+// A^T = conj(A).  On return the row-major buffer holds z_k in row k, where
+// conj(A) z_k = lambda_k z_k, i.e. the eigenvectors of A are conj(row k).
+int eigh_raw(int64_t n, int cpx, double* A, double* w, hipStream_t st) {
+  rocblas_handle h = blas_handle(st);
+  if (!h) return fail(JV_E_SOLVER, "rocblas_create_handle failed");
+  double* E = (double*)workspace(sizeof(double) * (size_t)n + 64, 3);
+  rocblas_int* info = (rocblas_int*)workspace(64, 4);
+  if (!E || !info) return fail(JV_E_NOMEM, "eigh: workspace allocation failed");
+  rocblas_status s;
+  if (cpx)
+    s = rocsolver_zheevd(h, rocblas_evect_original, rocblas_fill_upper, (rocblas_int)n,
+                         (rocblas_double_complex*)A, (rocblas_int)n, w, E, info);
+  else
+    s = rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, (rocblas_int)n, A, (rocblas_int)n, w, E,
+                         info);
+  if (s != rocblas_status_success) return fail(JV_E_SOLVER, "rocsolver syevd/heevd status " + std::to_string((int)s));
+  rocblas_int hinfo = 0;
+  JV_HIP(hipMemcpyAsync(&hinfo, info, sizeof(hinfo), hipMemcpyDeviceToHost, st));
+  JV_HIP(hipStreamSynchronize(st));
+  if (hinfo != 0) return fail(JV_E_SOLVER, "eigendecomposition did not converge (info " + std::to_string(hinfo) + ")");
+  return JV_OK;
+}
+
+// out[i][k] = conj(in[k][i])
+__global__ void conj_transpose(const double* __restrict__ in, double* __restrict__ out, int64_t n, int cpx) {
+  __shared__ double tr[32][33], ti[32][33];
+  int64_t bx = (int64_t)blockIdx.x * 32, by = (int64_t)blockIdx.y * 32;
+  int tx = threadIdx.x % 32, ty = threadIdx.x / 32;
+  for (int yy = ty; yy < 32; yy += 8) {
+    int64_t r = by + yy, c = bx + tx;
+    if (r < n && c < n) {
+      if (cpx) { tr[yy][tx] = in[2 * (r * n + c)]; ti[yy][tx] = in[2 * (r * n + c) + 1]; }
+      else tr[yy][tx] = in[r * n + c];
+    }
+  }
+  __syncthreads();
+  for (int yy = ty; yy < 32; yy += 8) {
+    int64_t r = bx + yy, c = by + tx;
+    if (r < n && c < n) {
+      if (cpx) { out[2 * (r * n + c)] = tr[tx][yy]; out[2 * (r * n + c) + 1] = -ti[tx][yy]; }
+      else out[r * n + c] = tr[tx][yy];
+    }
+  }
+}
+
+// ------------------------------------------------------------------ pinv apply
+// y[k] *= 1/lambda_k when |lambda_k| > rcond * max|lambda| else 0 (jnp.linalg.pinv,
+// hermitian=True).  One block.
+__global__ void pinv_scale(const double* __restrict__ ev, int64_t n, double rcond, int cpx, double* __restrict__ y,
+                           int* __restrict__ rank) {
+  __shared__ double red[16];
+  double mx = 0.0;
+  for (int64_t k = threadIdx.x; k < n; k += blockDim.x) mx = fmax(mx, fabs(ev[k]));
+  // block max
+  for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+  if (threadIdx.x % 64 == 0) red[threadIdx.x / 64] = mx;
+  __syncthreads();
+  mx = 0.0;
+  for (int i = 0; i < (int)blockDim.x / 64; ++i) mx = fmax(mx, red[i]);
+  const double cut = rcond * mx;
+  int kept = 0;
+  for (int64_t k = threadIdx.x; k < n; k += blockDim.x) {
+    double l = ev[k];
+    double inv = fabs(l) > cut ? 1.0 / l : 0.0;
+    kept += fabs(l) > cut;
+    if (cpx) { y[2 * k] *= inv; y[2 * k + 1] *= inv; }
+    else y[k] *= inv;
+  }
+  __syncthreads();
+  double kk = block_sum((double)kept, red);
+  if (threadIdx.x == 0) *rank = (int)kk;
+}
+
+__global__ void add_diag(double* __restrict__ T, int64_t n, double shift, int cpx) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) T[(i * n + i) * (cpx ? 2 : 1)] += shift;
+}
+
+__global__ void real_part(const double* __restrict__ in, int64_t n, double* __restrict__ out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = in[2 * i];
+}
+
+}  // namespace jv
+
+using namespace jv;
+
+extern "C" {
+
+int jv_center(const double* obs, const double* w, int64_t n, int64_t p, int32_t is_complex, double* mean,
+              double* data, void* stream) {
+  if (!obs || !w || !mean || !data || n < 1 || p < 1) return fail(JV_E_ARG, "jv_center: bad arguments");
+  return center(obs, w, n, p, is_complex, mean, data, (hipStream_t)stream);
+}
+
+int jv_gram_nn(const double* data, int64_t n, int64_t p, int32_t is_complex, double* T, void* stream) {
+  if (!data || !T || n < 1 || p < 1) return fail(JV_E_ARG, "jv_gram_nn: bad arguments");
+  // T = D D^dagger: opA = D (m = row, k = col), opB(k, n) = conj(D[n][k])
+  GemmArgs g{n, n, p, data, p, 1, 0, data, 1, p, 1, T, n, is_complex, 1, 1.0};
+  return gemm(g, (hipStream_t)stream);
+}
+
+int jv_gram_pp(const double* data, int64_t n, int64_t p, int32_t is_complex, double* S, void* stream) {
+  if (!data || !S || n < 1 || p < 1) return fail(JV_E_ARG, "jv_gram_pp: bad arguments");
+  // S = D^dagger D: opA(m, k) = conj(D[k][m]), opB(k, n) = D[k][n]
+  GemmArgs g{p, p, n, data, 1, p, 1, data, p, 1, 0, S, p, is_complex, 1, 1.0};
+  return gemm(g, (hipStream_t)stream);
+}
+
+int jv_eigh(int64_t n, int32_t is_complex, double* A, double* w, void* stream) {
+  if (!A || !w || n < 1) return fail(JV_E_ARG, "jv_eigh: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  int rc = eigh_raw(n, is_complex, A, w, st);
+  if (rc) return rc;
+  size_t bytes = sizeof(double) * (size_t)n * n * (is_complex ? 2 : 1);
+  double* tmp = (double*)workspace(bytes, 5);
+  if (!tmp) return fail(JV_E_NOMEM, "jv_eigh: workspace allocation failed");
+  JV_HIP(hipMemcpyAsync(tmp, A, bytes, hipMemcpyDeviceToDevice, st));
+  dim3 grid((unsigned)((n + 31) / 32), (unsigned)((n + 31) / 32));
+  hipLaunchKernelGGL(conj_transpose, grid, dim3(256), 0, st, tmp, A, n, is_complex);
+  JV_LAUNCH_CHECK("conj_transpose");
+  return JV_OK;
+}
+
+int jv_minsr_solve(const double* data, const double* e, int64_t n, int64_t p, int32_t is_complex, double rcond,
+                   double diag_shift, double* update, jv_solve_info* info, void* stream) {
+  if (!data || !e || !update || n < 1 || p < 1) return fail(JV_E_ARG, "jv_minsr_solve: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  const int cpx = is_complex;
+  // T = D D^dagger (+ shift I for real-parameter nets, minsr.py:72)
+  double* T = (double*)workspace(sizeof(double) * (size_t)n * n * (cpx ? 2 : 1), 6);
+  double* ev = (double*)workspace(sizeof(double) * (size_t)n, 7);
+  double* y = (double*)workspace(sizeof(double) * (size_t)n * 2, 8);
+  double* x = (double*)workspace(sizeof(double) * (size_t)n * 2, 9);
+  double* er = (double*)workspace(sizeof(double) * (size_t)n, 10);
+  int* rank = (int*)workspace(64, 11);
+  if (!T || !ev || !y || !x || !er || !rank) return fail(JV_E_NOMEM, "jv_minsr_solve: workspace allocation failed");
+  int rc = jv_gram_nn(data, n, p, cpx, T, stream);
+  if (rc) return rc;
+  if (!cpx && diag_shift != 0.0) {
+    hipLaunchKernelGGL(add_diag, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, T, n, diag_shift, 0);
+    JV_LAUNCH_CHECK("add_diag");
+  }
+  rc = eigh_raw(n, cpx, T, ev, st);
+  if (rc) return rc;
+  // T's rows hold z_k (eigenvectors u_k = conj(z_k)); y = R e, scale, x = R^H y
+  const double* rhs = e;
+  if (!cpx) {  // real-parameter nets: only Re(e) couples to the real data (minsr.py:75-78)
+    hipLaunchKernelGGL(real_part, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, e, n, er);
+    JV_LAUNCH_CHECK("real_part");
+    rhs = er;
+  }
+  rc = gemv(false, T, rhs, n, n, cpx, 0, 1.0, y, st, 12);
+  if (rc) return rc;
+  hipLaunchKernelGGL(pinv_scale, dim3(1), dim3(1024), 0, st, ev, n, rcond, cpx, y, rank);
+  JV_LAUNCH_CHECK("pinv_scale");
+  rc = gemv(true, T, y, n, n, cpx, 1, 1.0, x, st, 12);
+  if (rc) return rc;
+  // update = -D^dagger x (holomorphic: the caller applies the layout factor)
+  rc = gemv(true, data, x, n, p, cpx, 1, -1.0, update, st, 12);
+  if (rc) return rc;
+  if (info) {
+    int hr = 0;
+    JV_HIP(hipMemcpyAsync(&hr, rank, sizeof(int), hipMemcpyDeviceToHost, st));
+    JV_HIP(hipStreamSynchronize(st));
+    info->rank = hr;
+    info->iterations = 0;
+    info->residual = 0.0;
+    info->pinv_cutoff = rcond;
+  }
+  return JV_OK;
+}
+
+}  // extern "C"
+
+// ====================================================================== TDVP
+namespace jv {
+
+__global__ void col_norm2_partial(const double* __restrict__ D, int64_t n, int64_t p, int cpx,
+                                  double* __restrict__ part) {
+  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= p) return;
+  int64_t r0 = (int64_t)blockIdx.y * CHUNK_ROWS, r1 = std::min<int64_t>(n, r0 + CHUNK_ROWS);
+  double a = 0.0;
+  for (int64_t r = r0; r < r1; ++r) {
+    if (cpx) {
+      double x = D[2 * (r * p + j)], y = D[2 * (r * p + j) + 1];
+      a = fma(x, x, fma(y, y, a));
+    } else {
+      double x = D[r * p + j];
+      a = fma(x, x, a);
+    }
+  }
+  part[blockIdx.y * p + j] = a;
+}
+
+int col_norm2(const double* D, int64_t n, int64_t p, int cpx, double* out, hipStream_t st, int slot) {
+  int nchunks = (int)((n + CHUNK_ROWS - 1) / CHUNK_ROWS);
+  double* part = (double*)workspace(sizeof(double) * (size_t)nchunks * p, slot);
+  if (!part) return fail(JV_E_NOMEM, "col_norm2: workspace allocation failed");
+  hipLaunchKernelGGL(col_norm2_partial, dim3((unsigned)((p + 255) / 256), nchunks), dim3(256), 0, st, D, n, p, cpx,
+                     part);
+  JV_LAUNCH_CHECK("col_norm2_partial");
+  hipLaunchKernelGGL(col_reduce, dim3((unsigned)((p + 255) / 256)), dim3(256), 0, st, part, nchunks, p, out);
+  JV_LAUNCH_CHECK("col_reduce");
+  return JV_OK;
+}
+
+// F0 = -x * raw (raw = D^dagger e); F = makeReal(F0): 'real' -> Re, 'imag' -> i Im
+__global__ void tdvp_force(const double* __restrict__ raw, int64_t pr, double xr, double xi, int imag,
+                           double* __restrict__ F0, double* __restrict__ F) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= pr) return;
+  double a = raw[2 * i], b = raw[2 * i + 1];
+  double fr = -(xr * a - xi * b), fi = -(xr * b + xi * a);
+  F0[2 * i] = fr;
+  F0[2 * i + 1] = fi;
+  F[2 * i] = imag ? 0.0 : fr;
+  F[2 * i + 1] = imag ? fi : 0.0;
+}
+
+// S = makeReal(S0) (+ diag(shift * diag(S)), tdvp.py:145-146).  'real': real
+// matrix; 'imag': complex matrix i Im(S0).
+__global__ void tdvp_project_S(const double* __restrict__ S0, int64_t pr, int imag, double shift,
+                               double* __restrict__ S) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= pr * pr) return;
+  int64_t i = t / pr, j = t % pr;
+  if (imag) {
+    S[2 * t] = 0.0;
+    S[2 * t + 1] = S0[2 * t + 1];
+  } else {
+    double v = S0[2 * t];
+    if (i == j && shift > 1e-10) v += shift * v;
+    S[t] = v;
+  }
+}
+
+// Per-sample SNR data Q[n][k] = makeReal(-x conj(D[n][k]) e_n / w_n)
+// (stats.py:255-265 covar_data followed by transform, tdvp.py:175-178).
+__global__ void tdvp_snr_data(const double* __restrict__ D, const double* __restrict__ e,
+                              const double* __restrict__ w, int64_t n, int64_t pr, double xr, double xi, int imag,
+                              double* __restrict__ Q) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * pr) return;
+  int64_t r = t / pr;
+  double dr = D[2 * t], di = -D[2 * t + 1];  // conj(D)
+  double er = e[2 * r], ei = e[2 * r + 1];
+  double pr_ = dr * er - di * ei, pi_ = dr * ei + di * er;
+  double inv = 1.0 / w[r];
+  pr_ *= inv;
+  pi_ *= inv;
+  double qr = -(xr * pr_ - xi * pi_), qi = -(xr * pi_ + xi * pr_);
+  if (imag) { Q[2 * t] = 0.0; Q[2 * t + 1] = qi; }
+  else Q[t] = qr;
+}
+
+__global__ void tdvp_snr(const double* __restrict__ VtF, const double* __restrict__ var, int64_t pr, int cpx,
+                         double nsamp, double* __restrict__ snr) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= pr) return;
+  double a2 = cpx ? VtF[2 * k] * VtF[2 * k] + VtF[2 * k + 1] * VtF[2 * k + 1] : VtF[k] * VtF[k];
+  snr[k] = sqrt(fabs(nsamp * a2 / var[k]));
+}
+
+// One block per cutoff candidate c: residual_c = ||(pinvEv ev - 1) VtF|| / ||F||
+// with pinvEv = invEv * 1/(1+(max(c, pinvCutoff)/|ev/ev_max|)^6) [* 1/(1+(snrTol/snr)^6)]
+// (tdvp.py:193-209).
+__device__ inline double tdvp_coef(double ev, double evmax, double cut, double pinv_cutoff, int use_snr,
+                                   double snr_tol, double snr) {
+  double rel = fabs(ev / evmax);
+  double inv = rel > 1e-14 ? 1.0 / ev : 0.0;
+  double q = fmax(cut, pinv_cutoff) / rel;
+  double q2 = q * q;
+  double r = 1.0 / (1.0 + q2 * q2 * q2);
+  if (use_snr) {
+    double s = snr_tol / snr;
+    double s2 = s * s;
+    r *= 1.0 / (1.0 + s2 * s2 * s2);
+  }
+  return inv * r;
+}
+
+__global__ void tdvp_residuals(const double* __restrict__ ev, const double* __restrict__ VtF,
+                               const double* __restrict__ snr, const double* __restrict__ cuts, int64_t pr, int cpx,
+                               double pinv_cutoff, int use_snr, double snr_tol, const double* __restrict__ fnorm,
+                               double* __restrict__ res) {
+  __shared__ double red[16];
+  const double cut = cuts[blockIdx.x];
+  const double evmax = ev[pr - 1];
+  double acc = 0.0;
+  for (int64_t k = threadIdx.x; k < pr; k += blockDim.x) {
+    double c = tdvp_coef(ev[k], evmax, cut, pinv_cutoff, use_snr, snr_tol, snr[k]);
+    double f = c * ev[k] - 1.0;
+    double a2 = cpx ? VtF[2 * k] * VtF[2 * k] + VtF[2 * k + 1] * VtF[2 * k + 1] : VtF[k] * VtF[k];
+    acc = fma(f * f, a2, acc);
+  }
+  double s = block_sum(acc, red);
+  if (threadIdx.x == 0) res[blockIdx.x] = sqrt(s) / *fnorm;
+}
+
+__global__ void vec_norm(const double* __restrict__ v, int64_t len, double* __restrict__ out) {
+  __shared__ double red[16];
+  double acc = 0.0;
+  for (int64_t i = threadIdx.x; i < len; i += blockDim.x) acc = fma(v[i], v[i], acc);
+  double s = block_sum(acc, red);
+  if (threadIdx.x == 0) *out = sqrt(s);
+}
+
+// c_k = pinvEv_k * VtF_k for the selected candidate
+__global__ void tdvp_coeffs(const double* __restrict__ ev, const double* __restrict__ VtF,
+                            const double* __restrict__ snr, int64_t pr, int cpx, double cut, double pinv_cutoff,
+                            int use_snr, double snr_tol, double* __restrict__ c) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= pr) return;
+  double f = tdvp_coef(ev[k], ev[pr - 1], cut, pinv_cutoff, use_snr, snr_tol, snr[k]);
+  if (cpx) { c[2 * k] = f * VtF[2 * k]; c[2 * k + 1] = f * VtF[2 * k + 1]; }
+  else c[k] = f * VtF[k];
+}
+
+__global__ void to_complex(const double* __restrict__ in, int64_t n, double* __restrict__ out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) { out[2 * i] = in[i]; out[2 * i + 1] = 0.0; }
+}
+
+// tdvp_error = |1 + (Re(u^T S0 u) - 2 Re(u^T F0)) / Var E|  (tdvp.py:102-104)
+__global__ void tdvp_error_k(const double* __restrict__ u, const double* __restrict__ S0u,
+                             const double* __restrict__ F0, int64_t pr, const double* __restrict__ evar,
+                             double* __restrict__ out) {
+  __shared__ double red[16];
+  double a = 0.0;
+  for (int64_t i = threadIdx.x; i < pr; i += blockDim.x) a += u[i] * (S0u[2 * i] - 2.0 * F0[2 * i]);
+  double s = block_sum(a, red);
+  if (threadIdx.x == 0) *out = fabs(1.0 + s / *evar);
+}
+
+__global__ void sum_vec(const double* __restrict__ v, int64_t n, double* __restrict__ out) {
+  __shared__ double red[16];
+  double a = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) a += v[i];
+  double s = block_sum(a, red);
+  if (threadIdx.x == 0) *out = s;
+}
+
+}  // namespace jv
+
+extern "C" {
+
+int jv_col_norm2(const double* data, int64_t n, int64_t p, int32_t is_complex, double* out, void* stream) {
+  if (!data || !out || n < 1 || p < 1) return fail(JV_E_ARG, "jv_col_norm2: bad arguments");
+  return col_norm2(data, n, p, is_complex, out, (hipStream_t)stream, 13);
+}
+
+int jv_gemm(int64_t M, int64_t N, int64_t K, const double* A, int64_t sam, int64_t sak, int32_t conjA,
+            const double* B, int64_t sbk, int64_t sbn, int32_t conjB, double* C, int64_t ldc, int32_t is_complex,
+            double alpha, void* stream) {
+  if (!A || !B || !C || M < 0 || N < 0 || K < 1) return fail(JV_E_ARG, "jv_gemm: bad arguments");
+  GemmArgs g{M, N, K, A, sam, sak, conjA, B, sbk, sbn, conjB, C, ldc, is_complex, 0, alpha};
+  return gemm(g, (hipStream_t)stream);
+}
+
+int jv_gemv(int32_t trans, const double* M, const double* x, int64_t rows, int64_t cols, int32_t is_complex,
+            int32_t conj, double alpha, double* y, void* stream) {
+  if (!M || !x || !y || rows < 1 || cols < 1) return fail(JV_E_ARG, "jv_gemv: bad arguments");
+  return gemv(trans != 0, M, x, rows, cols, is_complex, conj, alpha, y, (hipStream_t)stream, 14);
+}
+
+int jv_tdvp_solve(const double* D, const double* e, const double* w, int64_t n, int64_t pr, const double* x,
+                  int32_t make_real, double diag_shift, int32_t use_snr, double glob_num_samples, double snr_tol,
+                  double pinv_tol, double pinv_cutoff, double* update, double* S, double* F, double* ev,
+                  double* snr, jv_solve_info* info, void* stream) {
+  if (!D || !e || !w || !x || !update || !S || !F || !ev || !snr || n < 1 || pr < 1)
+    return fail(JV_E_ARG, "jv_tdvp_solve: bad arguments");
+  if (!(pinv_tol < 1.0)) return fail(JV_E_ARG, "pinvTol >= 1 leaves the regulariser undefined (tdvp.py:195-198)");
+  hipStream_t st = (hipStream_t)stream;
+  const int imag = make_real != 0;
+  const double xr = x[0], xi = x[1];
+  const size_t P2 = (size_t)pr * pr;
+  double* S0 = (double*)workspace(sizeof(double) * 2 * P2, 20);
+  double* Z = (double*)workspace(sizeof(double) * (imag ? 2 : 1) * P2, 21);
+  double* raw = (double*)workspace(sizeof(double) * 2 * pr, 22);
+  double* F0 = (double*)workspace(sizeof(double) * 2 * pr, 23);
+  double* VtF = (double*)workspace(sizeof(double) * 2 * pr, 24);
+  double* Q = (double*)workspace(sizeof(double) * (imag ? 2 : 1) * (size_t)n * pr, 25);
+  double* R = (double*)workspace(sizeof(double) * (imag ? 2 : 1) * (size_t)n * pr, 26);
+  double* mean = (double*)workspace(sizeof(double) * 2 * pr, 27);
+  double* var = (double*)workspace(sizeof(double) * pr, 28);
+  double* scal = (double*)workspace(sizeof(double) * 8, 29);  // fnorm, evar, tdvp_error
+  double* cvec = (double*)workspace(sizeof(double) * 2 * pr, 30);
+  double* ucpx = (double*)workspace(sizeof(double) * 2 * pr, 31);
+  double* S0u = (double*)workspace(sizeof(double) * 2 * pr, 32);
+  double* Fr = (double*)workspace(sizeof(double) * pr, 33);
+  if (!S0 || !Z || !raw || !F0 || !VtF || !Q || !R || !mean || !var || !scal || !cvec || !ucpx || !S0u || !Fr)
+    return fail(JV_E_NOMEM, "jv_tdvp_solve: workspace allocation failed");
+  const unsigned bp = (unsigned)((pr + 255) / 256);
+  int rc;
+  // S0 = D^dagger D, F0 = -x D^dagger e  (tdvp.py:139-142)
+  if ((rc = jv_gram_pp(D, n, pr, 1, S0, stream))) return rc;
+  if ((rc = gemv(true, D, e, n, pr, 1, 1, 1.0, raw, st, 34))) return rc;
+  hipLaunchKernelGGL(tdvp_force, dim3(bp), dim3(256), 0, st, raw, pr, xr, xi, imag, F0, F);
+  JV_LAUNCH_CHECK("tdvp_force");
+  hipLaunchKernelGGL(tdvp_project_S, dim3((unsigned)((P2 + 255) / 256)), dim3(256), 0, st, S0, pr, imag,
+                     diag_shift, S);
+  JV_LAUNCH_CHECK("tdvp_project_S");
+  // eigendecomposition of S (tdvp.py:153-171); Z rows hold z_k
+  JV_HIP(hipMemcpyAsync(Z, S, sizeof(double) * (imag ? 2 : 1) * P2, hipMemcpyDeviceToDevice, st));
+  if ((rc = eigh_raw(pr, imag, Z, ev, st))) return rc;
+  // VtF_k = v_k^dagger F = sum_i z_k[i] F_i
+  if (imag) {
+    if ((rc = gemv(false, Z, F, pr, pr, 1, 0, 1.0, VtF, st, 34))) return rc;
+  } else {
+    hipLaunchKernelGGL(real_part, dim3(bp), dim3(256), 0, st, F, pr, Fr);
+    JV_LAUNCH_CHECK("real_part");
+    if ((rc = gemv(false, Z, Fr, pr, pr, 0, 0, 1.0, VtF, st, 34))) return rc;
+  }
+  // SNR (tdvp.py:173-181): rho = Q conj(V) = Q Z^T, weighted variance per column
+  hipLaunchKernelGGL(tdvp_snr_data, dim3((unsigned)((n * pr + 255) / 256)), dim3(256), 0, st, D, e, w, n, pr, xr,
+                     xi, imag, Q);
+  JV_LAUNCH_CHECK("tdvp_snr_data");
+  {
+    GemmArgs g{n, pr, pr, Q, pr, 1, 0, Z, 1, pr, 0, R, pr, imag, 0, 1.0};
+    if ((rc = gemm(g, st))) return rc;
+  }
+  if ((rc = center(R, w, n, pr, imag, mean, R, st))) return rc;
+  if ((rc = col_norm2(R, n, pr, imag, var, st, 35))) return rc;
+  hipLaunchKernelGGL(tdvp_snr, dim3(bp), dim3(256), 0, st, VtF, var, pr, imag, glob_num_samples, snr);
+  JV_LAUNCH_CHECK("tdvp_snr");
+  // ||F||, Var E = sum |e|^2
+  if (imag) {
+    hipLaunchKernelGGL(vec_norm, dim3(1), dim3(1024), 0, st, F, 2 * pr, scal);
+  } else {
+    hipLaunchKernelGGL(vec_norm, dim3(1), dim3(1024), 0, st, Fr, pr, scal);
+  }
+  JV_LAUNCH_CHECK("vec_norm");
+  if ((rc = col_norm2(e, n, 1, 1, scal + 1, st, 36))) return rc;
+  // regulariser candidates (host computes the same cutoff sequence as tdvp.py:196-199)
+  std::vector<double> cuts;
+  double cutoff = 1e-2;
+  do {
+    cutoff *= 0.8;
+    cuts.push_back(cutoff);
+  } while (cutoff > pinv_cutoff);
+  const int K = (int)cuts.size();
+  double* dcuts = (double*)workspace(sizeof(double) * 2 * K, 37);
+  if (!dcuts) return fail(JV_E_NOMEM, "jv_tdvp_solve: workspace allocation failed");
+  double* dres = dcuts + K;
+  JV_HIP(hipMemcpyAsync(dcuts, cuts.data(), sizeof(double) * K, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(tdvp_residuals, dim3(K), dim3(256), 0, st, ev, VtF, snr, dcuts, pr, imag, pinv_cutoff, use_snr,
+                     snr_tol, scal, dres);
+  JV_LAUNCH_CHECK("tdvp_residuals");
+  std::vector<double> res(K);
+  JV_HIP(hipMemcpyAsync(res.data(), dres, sizeof(double) * K, hipMemcpyDeviceToHost, st));
+  JV_HIP(hipStreamSynchronize(st));
+  int ks = K - 1;
+  for (int k = 0; k < K; ++k)
+    if (!(res[k] > pinv_tol)) { ks = k; break; }
+  hipLaunchKernelGGL(tdvp_coeffs, dim3(bp), dim3(256), 0, st, ev, VtF, snr, pr, imag, cuts[ks], pinv_cutoff, use_snr,
+                     snr_tol, cvec);
+  JV_LAUNCH_CHECK("tdvp_coeffs");
+  // update = Re(V c): 'real' -> Z^T c ; 'imag' -> Re(Z^H c)
+  if (imag) {
+    if ((rc = gemv(true, Z, cvec, pr, pr, 1, 1, 1.0, ucpx, st, 34))) return rc;
+    hipLaunchKernelGGL(real_part, dim3(bp), dim3(256), 0, st, ucpx, pr, update);
+    JV_LAUNCH_CHECK("real_part");
+  } else {
+    if ((rc = gemv(true, Z, cvec, pr, pr, 0, 0, 1.0, update, st, 34))) return rc;
+  }
+  // tdvp_error from the unprojected S0, F0
+  hipLaunchKernelGGL(to_complex, dim3(bp), dim3(256), 0, st, update, pr, ucpx);
+  JV_LAUNCH_CHECK("to_complex");
+  if ((rc = gemv(false, S0, ucpx, pr, pr, 1, 0, 1.0, S0u, st, 34))) return rc;
+  hipLaunchKernelGGL(tdvp_error_k, dim3(1), dim3(1024), 0, st, update, S0u, F0, pr, scal + 1, scal + 2);
+  JV_LAUNCH_CHECK("tdvp_error_k");
+  double hs[3];
+  JV_HIP(hipMemcpyAsync(hs, scal, sizeof(double) * 3, hipMemcpyDeviceToHost, st));
+  JV_HIP(hipStreamSynchronize(st));
+  if (info) {
+    info->residual = res[ks];
+    info->pinv_cutoff = std::max(cuts[ks], pinv_cutoff);
+    info->iterations = ks + 1;
+    info->rank = 0;
+    info->e_var = hs[1];
+    info->tdvp_error = hs[2];
+  }
+  return JV_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,315 @@
+// Network descriptors, forward pass and fused per-sample log-derivatives O_k.
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "nets.hpp"
+
+namespace jv {
+
+// ------------------------------------------------------------------ host geometry
+bool make_rbm_geom(const jv_net* net, RbmGeom& g) {
+  if (!net || (net->kind != JV_NET_CPXRBM && net->kind != JV_NET_RBM)) return false;
+  g.L = net->num_sites;
+  g.H = net->num_hidden;
+  g.cpx = net->kind == JV_NET_CPXRBM;
+  g.ob = net->off_hidden_bias;
+  g.ow = net->off_kernel;
+  g.oa = net->off_visible_bias;
+  return true;
+}
+
+bool make_cnn_geom(const jv_net* net, CnnGeom& g) {
+  if (!net || net->kind != JV_NET_CNN) return false;
+  g.Lx = net->shape[0];
+  g.Ly = net->ndim == 2 ? net->shape[1] : 1;
+  g.n = g.Lx * g.Ly;
+  g.Fx = net->filter[0];
+  g.Fy = net->ndim == 2 ? net->filter[1] : 1;
+  g.FF = g.Fx * g.Fy;
+  g.nl = net->num_layers;
+  int off = 0;
+  for (int l = 0; l < g.nl; ++l) {
+    g.C[l] = net->channels[l];
+    g.Cin[l] = l == 0 ? 1 : net->channels[l - 1];
+    g.aoff[l] = off;
+    off += g.n * g.C[l];
+    g.koff[l] = net->off_conv_kernel[l];
+    g.boff[l] = net->off_conv_bias[l];
+  }
+  g.atot = off;
+  g.P = (int)net->num_real_params;
+  g.inv_nrm = 1.0 / sqrt((double)g.n * g.C[g.nl - 1]);
+  return true;
+}
+
+// LDS bytes of the CNN forward / gradient kernel.
+size_t cnn_fwd_lds(const CnnGeom& g, bool grad) {
+  size_t d = (size_t)g.P + g.n + (size_t)g.atot * (grad ? 2 : 1) + 8;
+  size_t s = 2 * (size_t)g.n * g.FF;
+  return d * 8 + s * 2 + 16;
+}
+
+int set_lds_limit(const void* fn, size_t bytes) {
+  if (bytes > 160 * 1024) return fail(JV_E_NOMEM, "network needs " + std::to_string(bytes) + " bytes of LDS per workgroup (limit 160 KiB)");
+  if (bytes > 64 * 1024) {
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute(MaxDynamicSharedMemorySize)");
+  }
+  return JV_OK;
+}
+
+// ------------------------------------------------------------------ RBM kernel
+// One wavefront per sample; lanes own hidden units h = lane + 64 c.
+__global__ void __launch_bounds__(256) rbm_fwd_grad(RbmGeom g, const double* __restrict__ P,
+                                                    const int32_t* __restrict__ cfg, int64_t n,
+                                                    int layout, double* __restrict__ grad,
+                                                    double* __restrict__ logpsi) {
+  const int lane = threadIdx.x % WAVE;
+  const int64_t wstride = (int64_t)gridDim.x * (blockDim.x / WAVE);
+  const int64_t Pc = (g.ob >= 0 ? g.H : 0) + (int64_t)g.L * g.H + (g.oa >= 0 ? g.L : 0);
+  // doubles per gradient row
+  const int64_t rowlen = layout == 0 ? (g.cpx ? 2 * Pc : Pc) : 2 * (g.cpx ? 2 * Pc : Pc);
+  for (int64_t s = (int64_t)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE; s < n; s += wstride) {
+    const int32_t* c = cfg + s * g.L;
+    double* row = grad ? grad + s * rowlen : nullptr;
+    // column writers: compact (complex or real) or jVMC layout (complex; [g, i g] per complex leaf)
+    auto put = [&](int64_t leaf_off_flat, int64_t leaf_off_compact, int64_t leaf_size, int64_t j, cd v) {
+      if (layout == 0) {
+        if (g.cpx) {
+          row[2 * (leaf_off_compact + j)] = v.re;
+          row[2 * (leaf_off_compact + j) + 1] = v.im;
+        } else {
+          row[leaf_off_compact + j] = v.re;
+        }
+      } else {
+        int64_t col = leaf_off_flat + j;
+        row[2 * col] = v.re;
+        row[2 * col + 1] = v.im;
+        if (g.cpx) {  // i * g
+          int64_t col2 = leaf_off_flat + leaf_size + j;
+          row[2 * col2] = -v.im;
+          row[2 * col2 + 1] = v.re;
+        }
+      }
+    };
+    const int64_t cb = 0, cw = (g.ob >= 0 ? g.H : 0), ca = cw + (int64_t)g.L * g.H;
+    double accr = 0.0, acci = 0.0;
+    for (int h = lane; h < g.H; h += WAVE) {
+      cd th = rbm_theta(g, P, c, h);
+      if (g.cpx) {
+        cd lc = log_cosh(th);
+        accr += lc.re;
+        acci += lc.im;
+      } else {
+        accr += log_cosh(th.re);
+      }
+      if (row) {
+        cd t = g.cpx ? ctanh(th) : cd{tanh(th.re), 0.0};
+        if (g.ob >= 0) put(g.ob, cb, g.H, h, t);
+        for (int i = 0; i < g.L; ++i) {
+          double sg = (double)(2 * c[i] - 1);
+          put(g.ow, cw, (int64_t)g.L * g.H, (int64_t)i * g.H + h, {sg * t.re, sg * t.im});
+        }
+      }
+    }
+    if (g.oa >= 0) {
+      for (int i = lane; i < g.L; i += WAVE) {
+        double sg = (double)(2 * c[i] - 1);
+        accr = fma(sg, P[g.oa + i], accr);
+        acci = fma(sg, P[g.oa + g.L + i], acci);
+        if (row) put(g.oa, ca, g.L, i, {sg, 0.0});
+      }
+    }
+    accr = wave_sum(accr);
+    acci = wave_sum(acci);
+    if (logpsi && lane == 0) {
+      logpsi[2 * s] = accr;
+      logpsi[2 * s + 1] = acci;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ CNN kernel
+// One workgroup per sample (grid-stride): forward into LDS, then reverse pass
+// for dlogpsi/dtheta (cnn.py:131-171).
+__global__ void __launch_bounds__(256) cnn_fwd_grad(CnnGeom g, const double* __restrict__ Pg,
+                                                    const int32_t* __restrict__ cfg, int64_t n,
+                                                    int layout, double* __restrict__ grad,
+                                                    double* __restrict__ logpsi) {
+  extern __shared__ double lds[];
+  const bool want_grad = grad != nullptr;
+  double* P = lds;
+  double* x0 = P + g.P;
+  double* A = x0 + g.n;
+  double* D = want_grad ? A + g.atot : nullptr;
+  double* red = A + g.atot * (want_grad ? 2 : 1);
+  short* nbr = (short*)(red + 8);
+  short* inv = nbr + g.n * g.FF;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int i = tid; i < g.P; i += nt) P[i] = Pg[i];
+  cnn_build_tables(g, nbr, inv, tid, nt);
+  __syncthreads();
+  auto sync = [] { __syncthreads(); };
+  for (int64_t s = blockIdx.x; s < n; s += gridDim.x) {
+    const int32_t* c = cfg + s * g.n;
+    for (int i = tid; i < g.n; i += nt) x0[i] = (double)(2 * c[i] - 1);
+    __syncthreads();
+    cnn_forward_group(g, P, x0, A, D, nbr, tid, nt, sync);
+    const int L1 = g.nl - 1;
+    double part = 0.0;
+    for (int t = tid; t < g.n * g.C[L1]; t += nt) part += A[g.aoff[L1] + t];
+    double lp = block_sum(part, red) * g.inv_nrm;
+    if (logpsi && tid == 0) {
+      logpsi[2 * s] = lp;
+      logpsi[2 * s + 1] = 0.0;
+    }
+    if (!want_grad) continue;
+    // reverse pass; D_l is overwritten by delta_l = d out / d z_l
+    double* row = grad + s * (int64_t)g.P * (layout == 0 ? 1 : 2);
+    auto put = [&](int64_t col, double v) {
+      if (layout == 0) row[col] = v;
+      else { row[2 * col] = v; row[2 * col + 1] = 0.0; }
+    };
+    for (int t = tid; t < g.n * g.C[L1]; t += nt) D[g.aoff[L1] + t] *= g.inv_nrm;
+    __syncthreads();
+    for (int l = L1; l >= 0; --l) {
+      const int cin = g.Cin[l], cout = g.C[l];
+      const double* dl = D + g.aoff[l];
+      const double* X = l == 0 ? x0 : A + g.aoff[l - 1];
+      // kernel gradient: sum_p delta[p, co] X[nbr(p, k), ci]
+      for (int t = tid; t < g.FF * cin * cout; t += nt) {
+        int co = t % cout, kc = t / cout, ci = kc % cin, k = kc / cin;
+        double acc = 0.0;
+        for (int p = 0; p < g.n; ++p) acc = fma(dl[p * cout + co], X[nbr[p * g.FF + k] * cin + ci], acc);
+        put(g.koff[l] + t, acc);
+      }
+      if (g.boff[l] >= 0) {
+        for (int co = tid; co < cout; co += nt) {
+          double acc = 0.0;
+          for (int p = 0; p < g.n; ++p) acc += dl[p * cout + co];
+          put(g.boff[l] + co, acc);
+        }
+      }
+      if (l > 0) {
+        // delta_{l-1}[q, ci] = tanh(z_{l-1}[q, ci]) * sum_{k, co} delta_l[inv(q, k), co] K[k, ci, co]
+        const double* K = P + g.koff[l];
+        double* dp = D + g.aoff[l - 1];
+        for (int t = tid; t < g.n * cin; t += nt) {
+          int q = t / cin, ci = t % cin;
+          double acc = 0.0;
+          for (int k = 0; k < g.FF; ++k) {
+            const double* d = dl + (int)inv[q * g.FF + k] * cout;
+            const double* w = K + (k * cin + ci) * cout;
+            for (int co = 0; co < cout; ++co) acc = fma(d[co], w[co], acc);
+          }
+          dp[t] *= acc;
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+}  // namespace jv
+
+using namespace jv;
+
+extern "C" {
+
+int jv_net_init(jv_net* net) {
+  if (!net) return fail(JV_E_ARG, "jv_net_init: null descriptor");
+  if (net->ndim < 1 || net->ndim > 2) return fail(JV_E_ARG, "ndim must be 1 or 2");
+  if (net->ndim == 1) net->shape[1] = 1;
+  if (net->shape[0] < 1 || net->shape[1] < 1) return fail(JV_E_ARG, "lattice extents must be positive");
+  net->num_sites = net->shape[0] * net->shape[1];
+  net->off_hidden_bias = net->off_kernel = net->off_visible_bias = -1;
+  for (int l = 0; l < JV_MAX_LAYERS; ++l) net->off_conv_kernel[l] = net->off_conv_bias[l] = -1;
+  if (net->kind == JV_NET_CPXRBM || net->kind == JV_NET_RBM) {
+    const int L = net->num_sites, H = net->num_hidden;
+    if (H < 1) return fail(JV_E_ARG, "num_hidden must be positive");
+    if (net->kind == JV_NET_RBM && net->visible_bias) return fail(JV_E_ARG, "the real RBM has no visible bias");
+    const int mult = net->kind == JV_NET_CPXRBM ? 2 : 1;
+    int64_t off = 0, np = 0;
+    // tree_flatten order: Dense_0/bias, Dense_0/kernel, visible_bias
+    if (net->hidden_bias) { net->off_hidden_bias = off; off += (int64_t)mult * H; np += H; }
+    net->off_kernel = off; off += (int64_t)mult * L * H; np += (int64_t)L * H;
+    if (net->visible_bias) { net->off_visible_bias = off; off += (int64_t)mult * L; np += L; }
+    net->is_complex = net->kind == JV_NET_CPXRBM;
+    net->num_params = np;
+    net->num_real_params = off;
+    return JV_OK;
+  }
+  if (net->kind == JV_NET_CNN) {
+    const int nl = net->num_layers;
+    if (nl < 1 || nl > JV_MAX_LAYERS) return fail(JV_E_ARG, "num_layers must be in [1, 8]");
+    if (net->ndim == 1) net->filter[1] = 1;
+    if (net->filter[0] < 1 || net->filter[1] < 1) return fail(JV_E_ARG, "filter extents must be positive");
+    if (net->num_sites > 32767) return fail(JV_E_ARG, "CNN lattices are limited to 32767 sites");
+    const int FF = net->filter[0] * net->filter[1];
+    // entries (name, leaf, size) sorted as JAX sorts dict keys
+    struct E { std::string name; std::string leaf; int layer; int64_t size; };
+    std::vector<E> es;
+    for (int l = 0; l < nl; ++l) {
+      if (net->channels[l] < 1) return fail(JV_E_ARG, "channels must be positive");
+      int cin = l == 0 ? 1 : net->channels[l - 1];
+      std::string nm = "Conv_" + std::to_string(l);
+      if (net->layer_bias[l]) es.push_back({nm, "bias", l, net->channels[l]});
+      es.push_back({nm, "kernel", l, (int64_t)FF * cin * net->channels[l]});
+    }
+    std::sort(es.begin(), es.end(), [](const E& a, const E& b) {
+      return a.name != b.name ? a.name < b.name : a.leaf < b.leaf;
+    });
+    int64_t off = 0;
+    for (auto& e : es) {
+      if (e.leaf == "bias") net->off_conv_bias[e.layer] = off;
+      else net->off_conv_kernel[e.layer] = off;
+      off += e.size;
+    }
+    net->is_complex = 0;
+    net->num_params = off;
+    net->num_real_params = off;
+    CnnGeom g;
+    make_cnn_geom(net, g);
+    if (cnn_fwd_lds(g, true) > 160 * 1024)
+      return fail(JV_E_NOMEM, "CNN too large for one workgroup's LDS");
+    return JV_OK;
+  }
+  return fail(JV_E_ARG, "unknown network kind");
+}
+
+int jv_net_gradients(const jv_net* net, const double* params, const int32_t* configs, int64_t n,
+                     int32_t layout, double* grad, double* logpsi, void* stream) {
+  if (!net || !params || !configs || n < 0) return fail(JV_E_ARG, "jv_net_gradients: bad arguments");
+  if (n == 0) return JV_OK;
+  if (layout != 0 && layout != 1) return fail(JV_E_ARG, "layout must be 0 or 1");
+  hipStream_t st = (hipStream_t)stream;
+  RbmGeom rg;
+  CnnGeom cg;
+  if (make_rbm_geom(net, rg)) {
+    int64_t blocks = std::min<int64_t>((n + 3) / 4, 4096);
+    hipLaunchKernelGGL(rbm_fwd_grad, dim3((unsigned)blocks), dim3(256), 0, st, rg, params, configs, n,
+                       layout, grad, logpsi);
+    JV_LAUNCH_CHECK("rbm_fwd_grad");
+    return JV_OK;
+  }
+  if (make_cnn_geom(net, cg)) {
+    size_t lds = cnn_fwd_lds(cg, grad != nullptr);
+    int rc = set_lds_limit((const void*)cnn_fwd_grad, lds);
+    if (rc) return rc;
+    int64_t blocks = std::min<int64_t>(n, 8192);
+    hipLaunchKernelGGL(cnn_fwd_grad, dim3((unsigned)blocks), dim3(256), lds, st, cg, params, configs, n,
+                       layout, grad, logpsi);
+    JV_LAUNCH_CHECK("cnn_fwd_grad");
+    return JV_OK;
+  }
+  return fail(JV_E_ARG, "jv_net_gradients: unknown network kind (call jv_net_init first)");
+}
+
+int jv_net_forward(const jv_net* net, const double* params, const int32_t* configs, int64_t n,
+                   double* logpsi, void* stream) {
+  if (!logpsi) return fail(JV_E_ARG, "jv_net_forward: null output");
+  return jv_net_gradients(net, params, configs, n, 0, nullptr, logpsi, stream);
+}
+
+}  // extern "C"

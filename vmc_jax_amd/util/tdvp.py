@@ -1,0 +1,140 @@
+"""TDVP right-hand side (tdvp.py:26-326) with the solve on the GPU.
+
+The S / F build, FP64 eigendecomposition, SNR projection and the regulariser
+loop (all cutoff candidates at once) run inside ``jv_tdvp_solve``.
+"""
+import ctypes
+import time
+
+import numpy as np
+import torch
+
+from .. import _lib
+from .. import mpi_wrapper as mpi
+from ..sampler import ExactSampler
+
+
+def realFun(x):
+    return torch.real(x) if torch.is_tensor(x) else np.real(x)
+
+
+def imagFun(x):
+    return 0.5 * (x - x.conj())
+
+
+class _Timer:
+    def __init__(self, outp):
+        self.outp = outp
+
+    def start(self, name):
+        if self.outp is not None:
+            torch.cuda.synchronize()
+            self.outp.start_timing(name)
+
+    def stop(self, name):
+        if self.outp is not None:
+            torch.cuda.synchronize()
+            self.outp.stop_timing(name)
+
+
+class TDVP:
+    def __init__(self, sampler, snrTol=2, pinvTol=1e-14, pinvCutoff=1e-8, makeReal='imag', rhsPrefactor=1.j,
+                 diagonalShift=0., crossValidation=False, diagonalizeOnDevice=True):
+        if crossValidation:
+            raise NotImplementedError("crossValidation is not implemented")
+        self.sampler = sampler
+        self.snrTol = snrTol
+        self.pinvTol = pinvTol
+        self.pinvCutoff = pinvCutoff
+        self.diagonalShift = diagonalShift
+        self.rhsPrefactor = rhsPrefactor
+        self.crossValidation = crossValidation
+        self.diagonalizeOnDevice = diagonalizeOnDevice
+        self.makeReal = imagFun if makeReal == 'imag' else realFun
+        self._imag = makeReal == 'imag'
+        self.metaData = None
+
+    def set_diagonal_shift(self, delta):
+        self.diagonalShift = delta
+
+    def set_cross_validation(self, crossValidation=True):
+        if crossValidation:
+            raise NotImplementedError("crossValidation is not implemented")
+
+    def get_residuals(self):
+        return self.metaData["tdvp_error"], self.metaData["tdvp_residual"]
+
+    def get_snr(self):
+        return self.metaData["SNR"]
+
+    def get_spectrum(self):
+        return self.metaData["spectrum"]
+
+    def get_metadata(self):
+        return self.metaData
+
+    def get_energy_variance(self):
+        return self.ElocVar0
+
+    def get_energy_mean(self):
+        return float(np.real(self.ElocMean0))
+
+    def get_S(self):
+        return self.S
+
+    def solve(self, Eloc, gradients):
+        """tdvp.py:183-213 on centred data (SampledObs of E_loc and O)."""
+        D = gradients._data
+        if not torch.is_complex(D):
+            D = D.to(torch.complex128)
+        D = D.contiguous()
+        e = Eloc._data.reshape(-1).to(torch.complex128).contiguous()
+        w = gradients._weights
+        n, pr = D.shape
+        dev = D.device
+        update = torch.empty(pr, dtype=torch.float64, device=dev)
+        S = torch.empty((pr, pr), dtype=torch.complex128 if self._imag else torch.float64, device=dev)
+        F = torch.empty(pr, dtype=torch.complex128, device=dev)
+        ev = torch.empty(pr, dtype=torch.float64, device=dev)
+        snr = torch.empty(pr, dtype=torch.float64, device=dev)
+        x = (ctypes.c_double * 2)(complex(self.rhsPrefactor).real, complex(self.rhsPrefactor).imag)
+        info = _lib.JvSolveInfo()
+        _lib.call("jv_tdvp_solve", _lib.ptr(D), _lib.ptr(e), _lib.ptr(w), n, pr, x, int(self._imag),
+                  float(self.diagonalShift), int(not isinstance(self.sampler, ExactSampler)),
+                  float(mpi.globNumSamples), float(self.snrTol), float(self.pinvTol), float(self.pinvCutoff),
+                  _lib.ptr(update), _lib.ptr(S), _lib.ptr(F), _lib.ptr(ev), _lib.ptr(snr), ctypes.byref(info),
+                  _lib.stream_ptr())
+        self.S, self.F, self.ev, self.snr = S, F, ev, snr
+        self.ElocMean = complex(Eloc.mean()[0].item())
+        self.ElocVar = info.e_var
+        self._tdvp_error = info.tdvp_error
+        return update, info.residual, info.pinv_cutoff
+
+    def __call__(self, netParameters, t, *, psi, hamiltonian, **rhsArgs):
+        """tdvp.py:219-326."""
+        from ..stats import SampledObs
+        tmpParameters = psi.get_parameters()
+        psi.set_parameters(netParameters)
+        timer = _Timer(rhsArgs.get("outp", None))
+        numSamples = rhsArgs.get("numSamples", None)
+        timer.start("sampling")
+        sampleConfigs, sampleLogPsi, p = self.sampler.sample(numSamples=numSamples)
+        timer.stop("sampling")
+        timer.start("compute Eloc")
+        Eloc = hamiltonian.get_O_loc(sampleConfigs, psi, sampleLogPsi, t)
+        timer.stop("compute Eloc")
+        Eloc = SampledObs(Eloc, p)
+        timer.start("compute gradients")
+        sampleGradients = psi.gradients(sampleConfigs)
+        timer.stop("compute gradients")
+        sampleGradients = SampledObs(sampleGradients, p)
+        timer.start("solve TDVP eqn.")
+        update, solverResidual, pinvCutoff = self.solve(Eloc, sampleGradients)
+        timer.stop("solve TDVP eqn.")
+        psi.set_parameters(tmpParameters)
+        if rhsArgs.get("intStep", None) == 0:
+            self.ElocMean0 = self.ElocMean
+            self.ElocVar0 = self.ElocVar
+            self.metaData = {"tdvp_error": self._tdvp_error, "tdvp_residual": solverResidual,
+                             "pinv_cutoff": pinvCutoff, "SNR": self.snr, "spectrum": self.ev}
+        return update

@@ -1,0 +1,186 @@
+"""Variational state (vqs.py:85-491) with the network evaluated by HIP kernels.
+
+Shapes follow jVMC: configurations carry a leading device axis of size 1,
+``(1, B, *sampleShape)`` int32, and ``__call__`` returns ``(1, B)`` (complex128
+for complex nets, float64 for real ones); ``gradients`` returns
+``(1, B, numRealParameters)`` complex128 in jVMC's flat layout.  Parameters live
+on the GPU as one flat float64 vector in jVMC's tree_flatten order.
+"""
+import numpy as np
+import torch
+
+from . import _lib
+from . import global_defs
+
+
+def _as_configs(s):
+    if not torch.is_tensor(s):
+        s = torch.as_tensor(np.asarray(s))
+    s = s.to(device=global_defs.device(), dtype=torch.int32)
+    return s.contiguous()
+
+
+class NQS:
+    def __init__(self, net, logarithmic=True, batchSize=1000, seed=1234, orbit=None, avgFun=None):
+        if isinstance(net, (tuple, list)):
+            raise NotImplementedError("two-network ansatz (TwoNets) is not implemented")
+        if orbit is not None:
+            raise NotImplementedError("symmetrised networks (SymNet) are not implemented")
+        if not logarithmic:
+            raise NotImplementedError("only logarithmic networks are implemented")
+        self.net = net
+        self.logarithmic = True
+        self.batchSize = batchSize
+        self.seed = seed
+        self.initialized = False
+        self._isGenerator = False
+        self.desc = None
+        self._params = None
+
+    # ------------------------------------------------------------ init
+    def init_net(self, s):
+        """vqs.py:187-218: lattice shape from the first configuration seen."""
+        if self.initialized:
+            return
+        shape = tuple(int(x) for x in s.shape[2:])
+        self.sampleShape = shape
+        self.desc = self.net.descriptor(shape)
+        _lib.check(_lib.lib.jv_net_init(self.desc), "jv_net_init")
+        self.holomorphic = bool(self.desc.is_complex)
+        self.realParams = not self.holomorphic
+        rng = np.random.default_rng(self.seed)
+        P = self.net.init_params(self.desc, rng)
+        self._params = torch.as_tensor(P, dtype=torch.float64, device=global_defs.device())
+        self.initialized = True
+
+    @property
+    def numParameters(self):
+        return int(self.desc.num_params)
+
+    @property
+    def numRealParameters(self):
+        return int(self.desc.num_real_params)
+
+    @property
+    def is_generator(self):
+        return False
+
+    @property
+    def params(self):
+        return self._params
+
+    @params.setter
+    def params(self, P):
+        self._params = P
+
+    @property
+    def parameters(self):
+        return self._params
+
+    # ------------------------------------------------------------ evaluation
+    def _flat(self, s):
+        s = _as_configs(s)
+        self.init_net(s)
+        return s, s.reshape(-1, self.desc.num_sites)
+
+    def logpsi_complex(self, s):
+        """log psi as complex128 (B,) for flat configurations (B, L)."""
+        s = _as_configs(s)
+        n = s.shape[0]
+        out = torch.empty(n, dtype=torch.complex128, device=s.device)
+        _lib.call("jv_net_forward", self.desc, _lib.ptr(self._params), _lib.ptr(s), n, _lib.ptr(out),
+                  _lib.stream_ptr())
+        return out
+
+    def __call__(self, s):
+        """vqs.py:223-251."""
+        s, flat = self._flat(s)
+        out = self.logpsi_complex(flat).reshape(s.shape[:2])
+        return out if self.holomorphic else out.real.contiguous()
+
+    def gradients(self, s, layout=1):
+        """vqs.py:271-287: O_k = d log psi / d theta_k, jVMC flat layout (layout=1)
+        or the compact per-parameter layout (layout=0)."""
+        s, flat = self._flat(s)
+        n = flat.shape[0]
+        if layout == 1:
+            g = torch.empty((n, self.numRealParameters), dtype=torch.complex128, device=flat.device)
+        elif self.holomorphic:
+            g = torch.empty((n, self.numParameters), dtype=torch.complex128, device=flat.device)
+        else:
+            g = torch.empty((n, self.numParameters), dtype=torch.float64, device=flat.device)
+        _lib.call("jv_net_gradients", self.desc, _lib.ptr(self._params), _lib.ptr(flat), n, layout, _lib.ptr(g),
+                  None, _lib.stream_ptr())
+        return g.reshape(s.shape[:2] + g.shape[1:])
+
+    def get_sampler_net(self):
+        """vqs.py:337-352: Re log psi and the current parameters."""
+        return (lambda p, x: self.__call__(x).real), self._params
+
+    # ------------------------------------------------------------ parameters
+    def _to_dev(self, P):
+        if torch.is_tensor(P):
+            return P.to(global_defs.device())
+        return torch.as_tensor(np.asarray(P), device=global_defs.device())
+
+    def _param_unflatten(self, P):
+        """vqs.py:430-444 for a flat vector that may be complex (the holomorphic
+        MinSR update is complex, minsr.py:65): leaf = P[Re slot] + i P[Im slot]."""
+        P = self._to_dev(P)
+        if not torch.is_complex(P):
+            return P.to(torch.float64).contiguous()
+        if not self.holomorphic:
+            return P.real.to(torch.float64).contiguous()
+        out = torch.empty(self.numRealParameters, dtype=torch.float64, device=P.device)
+        for off, size in self._complex_leaves():
+            re = P[off:off + size]
+            im = P[off + size:off + 2 * size]
+            leaf = re + 1j * im
+            out[off:off + size] = leaf.real
+            out[off + size:off + 2 * size] = leaf.imag
+        return out
+
+    def _complex_leaves(self):
+        d = self.desc
+        leaves = []
+        if d.off_hidden_bias >= 0:
+            leaves.append((d.off_hidden_bias, d.num_hidden))
+        leaves.append((d.off_kernel, d.num_sites * d.num_hidden))
+        if d.off_visible_bias >= 0:
+            leaves.append((d.off_visible_bias, d.num_sites))
+        return leaves
+
+    def set_parameters(self, P):
+        """vqs.py:409-427."""
+        if not self.initialized:
+            raise RuntimeError("Error in NQS.set_parameters(): Network not initialized. Evaluate net on example "
+                               "input for initialization.")
+        P = self._param_unflatten(P)
+        if P.numel() != self.numRealParameters:
+            raise ValueError("expected %d parameters, got %d" % (self.numRealParameters, P.numel()))
+        self._params = P.clone()
+
+    def update_parameters(self, deltaP):
+        """vqs.py:384-406."""
+        if not self.initialized:
+            self.set_parameters(deltaP)
+            return
+        self._params = self._params + self._param_unflatten(deltaP)
+
+    def get_parameters(self):
+        """vqs.py:449-466: flat real vector (a device tensor)."""
+        if not self.initialized:
+            return None
+        return self._params.clone()
+
+    def expand_compact(self, v):
+        """Compact complex update (one entry per complex parameter) -> jVMC flat
+        layout [v_leaf, -i v_leaf] per leaf (the holomorphic MinSR update of
+        minsr.py:65 in the [O, iO] gradient layout of vqs.py:66-69)."""
+        out = torch.empty(self.numRealParameters, dtype=torch.complex128, device=v.device)
+        c = 0
+        for off, size in self._complex_leaves():
+            out[off:off + size] = v[c:c + size]
+            out[off + size:off + 2 * size] = -1j * v[c:c + size]
+            c += size
+        return out
