@@ -111,6 +111,14 @@ const char* jv_last_error(void);
 int jv_set_device(int device);
 /* Releases the per-device workspaces and solver handles. */
 int jv_shutdown(void);
+/* Kernel timing (no reference counterpart; measurement support): when enabled,
+ * HIP events are recorded on the launch stream around the named kernels
+ * ("gram", "gemm", "eigh", "eloc", "mc_sweep", "net_grad", "net_forward").
+ * jv_timer_read synchronises on the recorded events and returns the summed
+ * duration and the launch count. */
+int jv_timer_enable(int on);
+int jv_timer_reset(void);
+int jv_timer_read(const char* name, double* total_ms, int64_t* count);
 
 /* ------------------------------------------------------------------ networks */
 /* Validates a descriptor and computes the flat parameter layout (host only).
@@ -228,6 +236,16 @@ int jv_eigh(int64_t n, int32_t is_complex, double* A, double* w, void* stream);
 int jv_minsr_solve(const double* data, const double* e, int64_t n, int64_t p,
                    int32_t is_complex, double rcond, double diag_shift, double* update,
                    jv_solve_info* info, void* stream);
+
+/* Same update as jv_minsr_solve from the P x P Gram S = D^dagger D (+ shift I)
+ * instead of the N x N tangent kernel: the nonzero spectra of D D^dagger and
+ * D^dagger D coincide, D^dagger annihilates the null space of T, so
+ * -D^dagger pinv(T) e = -pinv(S) D^dagger e with the same relative cutoff.
+ * Cheaper whenever N > P.  No separate reference function (exact rewrite of
+ * minsr.py:53-80). */
+int jv_minsr_solve_dual(const double* data, const double* e, int64_t n, int64_t p,
+                        int32_t is_complex, double rcond, double diag_shift, double* update,
+                        jv_solve_info* info, void* stream);
 
 /* TDVP solve (tdvp.py:134-213).  data: centred O in jVMC's flat layout, complex
  * [n][pr]; e: centred E_loc, complex [n]; w: weights [n]; x = rhsPrefactor

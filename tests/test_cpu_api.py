@@ -138,3 +138,14 @@ def test_distribute_sampling_semantics():
     assert mpi.globNumSamples == 8192
     assert mpi.distribute_sampling(1000, 1, 300) == (4, 1200)
     assert mpi.distribute_sampling(1024) == (1024, 1024)
+
+
+def test_ctypes_signatures_match_header_arity(lib):
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    for name, params in re.findall(r"(?:int|const char\*)\s+(jv_\w+)\s*\(([^)]*)\)\s*;", text):
+        params = params.strip()
+        n = 0 if params in ("", "void") else params.count(",") + 1
+        if name == "jv_last_error":
+            continue
+        assert len(lib._SIGS[name]) == n, "%s: header has %d parameters, binding %d" % (name, n, len(lib._SIGS[name]))

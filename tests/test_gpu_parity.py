@@ -438,3 +438,31 @@ def test_gs_search_ed(solver):
             jv.util.ground_state_search(psi, H, eq, es, numSteps=200, stepSize=1e-2)
         E = jv.util.measure({"energy": H}, psi, es)["energy"]["mean"][0]
         assert abs((E - exE) / exE) < 1e-3
+
+
+def test_minsr_dual_equals_reference_form():
+    """The P x P dual solve gives the reference T-form update."""
+    psi, opsi, H, Ho, s, lp, p = _cnn_case((6, 6), (4, 4, 4), 512, 26)
+    El = jv.stats.SampledObs(H.get_O_loc(s, psi, lp, 0.0), p)
+    G = jv.stats.SampledObs(psi.gradients(s, layout=0), p)
+    a = to_np(jv.util.MinSR(None, pinvTol=1e-10, solver="T").solve(El, G, False))
+    b = to_np(jv.util.MinSR(None, pinvTol=1e-10, solver="dual").solve(El, G, False))
+    assert rel(b, a) < 1e-7
+    c = to_np(jv.util.MinSR(None, pinvTol=1e-10, diagonalShift=1e-3, solver="T").solve(El, G, False))
+    d = to_np(jv.util.MinSR(None, pinvTol=1e-10, diagonalShift=1e-3, solver="dual").solve(El, G, False))
+    assert rel(d, c) < 1e-9
+
+
+def test_minsr_dual_holomorphic():
+    rng = np.random.default_rng(27)
+    L = 8
+    onet = onets.CpxRBM((L,), numHidden=4, bias=True)
+    P = 0.1 * rng.normal(size=onet.num_real_params())
+    psi = make_psi(jv.nets.CpxRBM(numHidden=4, bias=True), (L,), P)
+    opsi = onets.NQS(onet, P)
+    H, Ho = tfim_pair((L,), -1.0, -0.7)
+    es, eso = jv.sampler.ExactSampler(psi, L), osamp.ExactSampler(opsi, L)
+    eq = jv.util.MinSR(es, pinvTol=1e-8, solver="dual")
+    upd = to_np(eq(psi.get_parameters(), 0.0, psi=psi, hamiltonian=H, intStep=0))
+    ref = osolv.MinSR(eso, pinvTol=1e-8)(opsi.get_parameters(), 0.0, psi=opsi, hamiltonian=Ho, intStep=0)
+    assert rel(upd, ref) < 1e-7

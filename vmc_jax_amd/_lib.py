@@ -48,7 +48,8 @@ class JvMcArgs(ctypes.Structure):
 
 
 class JvSolveInfo(ctypes.Structure):
-    _fields_ = [("residual", _dbl), ("pinv_cutoff", _dbl), ("iterations", _i32), ("rank", _i32)]
+    _fields_ = [("residual", _dbl), ("pinv_cutoff", _dbl), ("iterations", _i32), ("rank", _i32),
+                ("e_var", _dbl), ("tdvp_error", _dbl)]
 
 
 # name -> argtypes; every function returns int status
@@ -56,6 +57,9 @@ _SIGS = {
     "jv_version": [],
     "jv_set_device": [ctypes.c_int],
     "jv_shutdown": [],
+    "jv_timer_enable": [ctypes.c_int],
+    "jv_timer_reset": [],
+    "jv_timer_read": [ctypes.c_char_p, ctypes.POINTER(_dbl), ctypes.POINTER(_i64)],
     "jv_net_init": [ctypes.POINTER(JvNet)],
     "jv_net_forward": [ctypes.POINTER(JvNet), _vp, _vp, _i64, _vp, _vp],
     "jv_net_gradients": [ctypes.POINTER(JvNet), _vp, _vp, _i64, _i32, _vp, _vp, _vp],
@@ -72,8 +76,9 @@ _SIGS = {
     "jv_gemv": [_i32, _vp, _vp, _i64, _i64, _i32, _i32, _dbl, _vp, _vp],
     "jv_eigh": [_i64, _i32, _vp, _vp, _vp],
     "jv_minsr_solve": [_vp, _vp, _i64, _i64, _i32, _dbl, _dbl, _vp, ctypes.POINTER(JvSolveInfo), _vp],
+    "jv_minsr_solve_dual": [_vp, _vp, _i64, _i64, _i32, _dbl, _dbl, _vp, ctypes.POINTER(JvSolveInfo), _vp],
     "jv_tdvp_solve": [_vp, _vp, _vp, _i64, _i64, ctypes.POINTER(_dbl), _i32, _dbl, _i32, _dbl, _dbl, _dbl,
-                      _dbl, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(JvSolveInfo), _vp],
+                      _dbl, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(JvSolveInfo), _vp],
 }
 
 

@@ -28,6 +28,16 @@ int hip_fail(hipError_t e, const char* what);
 // Per-device scratch arena (grown on demand, released by jv_shutdown).
 void* workspace(size_t bytes, int slot);
 
+// Optional event timing of named kernels (jv_timer_enable / jv_timer_read).
+void* timer_begin(const char* name, hipStream_t st);
+void timer_end(void* tok, hipStream_t st);
+struct ScopedTimer {
+  void* tok;
+  hipStream_t st;
+  ScopedTimer(const char* name, hipStream_t s) : tok(timer_begin(name, s)), st(s) {}
+  ~ScopedTimer() { timer_end(tok, st); }
+};
+
 constexpr int WAVE = 64;
 
 // ------------------------------------------------------------------ complex

@@ -109,3 +109,75 @@ int jv_shutdown(void) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ kernel timers
+// Optional HIP-event timing of named kernels on the stream they run on.
+#include <map>
+
+namespace jv {
+namespace {
+struct TimerRec {
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+};
+std::mutex g_tmu;
+bool g_timing = false;
+std::map<std::string, TimerRec> g_timers;
+}  // namespace
+
+bool timing_enabled() { return g_timing; }
+
+void* timer_begin(const char* name, hipStream_t st) {
+  if (!g_timing) return nullptr;
+  hipEvent_t a, b;
+  if (hipEventCreate(&a) != hipSuccess) return nullptr;
+  if (hipEventCreate(&b) != hipSuccess) { (void)hipEventDestroy(a); return nullptr; }
+  (void)hipEventRecord(a, st);
+  std::lock_guard<std::mutex> lk(g_tmu);
+  auto& r = g_timers[name];
+  r.ev.push_back({a, b});
+  return (void*)b;
+}
+
+void timer_end(void* tok, hipStream_t st) {
+  if (tok) (void)hipEventRecord((hipEvent_t)tok, st);
+}
+
+}  // namespace jv
+
+extern "C" {
+
+int jv_timer_enable(int on) {
+  jv::g_timing = on != 0;
+  return JV_OK;
+}
+
+int jv_timer_reset(void) {
+  std::lock_guard<std::mutex> lk(jv::g_tmu);
+  for (auto& kv : jv::g_timers)
+    for (auto& p : kv.second.ev) {
+      (void)hipEventSynchronize(p.second);
+      (void)hipEventDestroy(p.first);
+      (void)hipEventDestroy(p.second);
+    }
+  jv::g_timers.clear();
+  return JV_OK;
+}
+
+int jv_timer_read(const char* name, double* total_ms, int64_t* count) {
+  if (!name || !total_ms || !count) return jv::fail(JV_E_ARG, "jv_timer_read: bad arguments");
+  std::lock_guard<std::mutex> lk(jv::g_tmu);
+  *total_ms = 0.0;
+  *count = 0;
+  auto it = jv::g_timers.find(name);
+  if (it == jv::g_timers.end()) return JV_OK;
+  for (auto& p : it->second.ev) {
+    JV_HIP(hipEventSynchronize(p.second));
+    float ms = 0.f;
+    JV_HIP(hipEventElapsedTime(&ms, p.first, p.second));
+    *total_ms += ms;
+    *count += 1;
+  }
+  return JV_OK;
+}
+
+}  // extern "C"

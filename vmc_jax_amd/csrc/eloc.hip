@@ -348,6 +348,7 @@ int jv_eloc(const jv_net* net, const double* params, const jv_optable* op, const
     rc = set_lds_limit((const void*)eloc_rbm, lds);
     if (rc) return rc;
     int64_t blocks = std::min<int64_t>((n + 3) / 4, 8192);
+    ScopedTimer tm("eloc", st);
     hipLaunchKernelGGL(eloc_rbm, dim3((unsigned)blocks), dim3(256), lds, st, rg, params, d, configs, logpsi_s,
                        n, out);
     JV_LAUNCH_CHECK("eloc_rbm");
@@ -361,6 +362,7 @@ int jv_eloc(const jv_net* net, const double* params, const jv_optable* op, const
     rc = set_lds_limit((const void*)eloc_cnn, lds);
     if (rc) return rc;
     int64_t blocks = std::min<int64_t>(n, 16384);
+    ScopedTimer tm("eloc", st);
     hipLaunchKernelGGL(eloc_cnn, dim3((unsigned)blocks), dim3(64 * waves), lds, st, cg, params, d, configs,
                        logpsi_s, n, out);
     JV_LAUNCH_CHECK("eloc_cnn");

@@ -211,6 +211,7 @@ int gemm(const GemmArgs& g, hipStream_t st) {
   const int64_t ntm = (g.M + BM - 1) / BM, ntn = (g.N + BN - 1) / BN;
   int64_t blocks = g.sym ? ntn * (ntn + 1) / 2 : ntm * ntn;
   if (g.sym && (g.M != g.N)) return fail(JV_E_ARG, "symmetric GEMM needs a square output");
+  ScopedTimer tm(g.sym ? "gram" : "gemm", st);
   if (g.cpx) hipLaunchKernelGGL(gemm_f64<true>, dim3((unsigned)blocks), dim3(256), 0, st, g);
   else hipLaunchKernelGGL(gemm_f64<false>, dim3((unsigned)blocks), dim3(256), 0, st, g);
   JV_LAUNCH_CHECK("gemm_f64");
@@ -307,6 +308,7 @@ int gemv(bool trans, const double* M, const double* x, int64_t rows, int64_t col
 // A^T = conj(A).  On return the row-major buffer holds z_k in row k, where
 // conj(A) z_k = lambda_k z_k, i.e. the eigenvectors of A are conj(row k).
 int eigh_raw(int64_t n, int cpx, double* A, double* w, hipStream_t st) {
+  ScopedTimer tm("eigh", st);
   rocblas_handle h = blas_handle(st);
   if (!h) return fail(JV_E_SOLVER, "rocblas_create_handle failed");
   double* E = (double*)workspace(sizeof(double) * (size_t)n + 64, 3);
@@ -787,6 +789,65 @@ int jv_tdvp_solve(const double* D, const double* e, const double* w, int64_t n, 
     info->rank = 0;
     info->e_var = hs[1];
     info->tdvp_error = hs[2];
+  }
+  return JV_OK;
+}
+
+}  // extern "C"
+
+// ====================================================================== MinSR, dual form
+namespace jv {
+__global__ void neg_copy(const double* __restrict__ in, int64_t len, double* __restrict__ out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < len) out[i] = -in[i];
+}
+}  // namespace jv
+
+extern "C" {
+
+int jv_minsr_solve_dual(const double* data, const double* e, int64_t n, int64_t p, int32_t is_complex,
+                        double rcond, double diag_shift, double* update, jv_solve_info* info, void* stream) {
+  if (!data || !e || !update || n < 1 || p < 1) return fail(JV_E_ARG, "jv_minsr_solve_dual: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  const int cpx = is_complex;
+  double* S = (double*)workspace(sizeof(double) * (size_t)p * p * (cpx ? 2 : 1), 6);
+  double* ev = (double*)workspace(sizeof(double) * (size_t)p, 7);
+  double* y = (double*)workspace(sizeof(double) * (size_t)p * 2, 8);
+  double* x = (double*)workspace(sizeof(double) * (size_t)p * 2, 9);
+  double* er = (double*)workspace(sizeof(double) * (size_t)n, 10);
+  int* rank = (int*)workspace(64, 11);
+  double* b = (double*)workspace(sizeof(double) * (size_t)p * 2, 15);
+  if (!S || !ev || !y || !x || !er || !rank || !b) return fail(JV_E_NOMEM, "jv_minsr_solve_dual: workspace allocation failed");
+  // S = D^dagger D (+ shift): same nonzero spectrum as T = D D^dagger (+ shift)
+  int rc = jv_gram_pp(data, n, p, cpx, S, stream);
+  if (rc) return rc;
+  if (!cpx && diag_shift != 0.0) {
+    hipLaunchKernelGGL(add_diag, dim3((unsigned)((p + 255) / 256)), dim3(256), 0, st, S, p, diag_shift, 0);
+    JV_LAUNCH_CHECK("add_diag");
+  }
+  const double* rhs = e;
+  if (!cpx) {
+    hipLaunchKernelGGL(real_part, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, e, n, er);
+    JV_LAUNCH_CHECK("real_part");
+    rhs = er;
+  }
+  if ((rc = gemv(true, data, rhs, n, p, cpx, 1, 1.0, b, st, 12))) return rc;  // b = D^dagger e
+  if ((rc = eigh_raw(p, cpx, S, ev, st))) return rc;
+  if ((rc = gemv(false, S, b, p, p, cpx, 0, 1.0, y, st, 12))) return rc;
+  hipLaunchKernelGGL(pinv_scale, dim3(1), dim3(1024), 0, st, ev, p, rcond, cpx, y, rank);
+  JV_LAUNCH_CHECK("pinv_scale");
+  if ((rc = gemv(true, S, y, p, p, cpx, 1, 1.0, x, st, 12))) return rc;
+  int64_t len = p * (cpx ? 2 : 1);
+  hipLaunchKernelGGL(neg_copy, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, st, x, len, update);
+  JV_LAUNCH_CHECK("neg_copy");
+  if (info) {
+    int hr = 0;
+    JV_HIP(hipMemcpyAsync(&hr, rank, sizeof(int), hipMemcpyDeviceToHost, st));
+    JV_HIP(hipStreamSynchronize(st));
+    info->rank = hr;
+    info->iterations = 0;
+    info->residual = 0.0;
+    info->pinv_cutoff = rcond;
   }
   return JV_OK;
 }

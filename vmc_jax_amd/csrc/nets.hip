@@ -288,6 +288,7 @@ int jv_net_gradients(const jv_net* net, const double* params, const int32_t* con
   CnnGeom cg;
   if (make_rbm_geom(net, rg)) {
     int64_t blocks = std::min<int64_t>((n + 3) / 4, 4096);
+    ScopedTimer tm(grad ? "net_grad" : "net_forward", st);
     hipLaunchKernelGGL(rbm_fwd_grad, dim3((unsigned)blocks), dim3(256), 0, st, rg, params, configs, n,
                        layout, grad, logpsi);
     JV_LAUNCH_CHECK("rbm_fwd_grad");
@@ -298,6 +299,7 @@ int jv_net_gradients(const jv_net* net, const double* params, const int32_t* con
     int rc = set_lds_limit((const void*)cnn_fwd_grad, lds);
     if (rc) return rc;
     int64_t blocks = std::min<int64_t>(n, 8192);
+    ScopedTimer tm(grad ? "net_grad" : "net_forward", st);
     hipLaunchKernelGGL(cnn_fwd_grad, dim3((unsigned)blocks), dim3(256), lds, st, cg, params, configs, n,
                        layout, grad, logpsi);
     JV_LAUNCH_CHECK("cnn_fwd_grad");

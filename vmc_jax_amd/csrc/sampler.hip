@@ -239,6 +239,7 @@ int jv_mc_sample(const jv_net* net, const double* params, const jv_mc_args* args
     size_t lds = nw * 8 * ((size_t)3 * rg.H + (rg.L + 1) / 2);
     int rc = set_lds_limit((const void*)mc_rbm, lds);
     if (rc) return rc;
+    ScopedTimer tm("mc_sweep", st);
     hipLaunchKernelGGL(mc_rbm, dim3((unsigned)((a.num_chains + nw - 1) / nw)), dim3(64 * nw), lds, st, rg,
                        params, a, walkers, configs, counters);
     JV_LAUNCH_CHECK("mc_rbm");
@@ -249,6 +250,7 @@ int jv_mc_sample(const jv_net* net, const double* params, const jv_mc_args* args
     size_t lds = lay.shared + nw * lay.per_chain;
     int rc = set_lds_limit((const void*)mc_cnn, lds);
     if (rc) return rc;
+    ScopedTimer tm("mc_sweep", st);
     hipLaunchKernelGGL(mc_cnn, dim3((unsigned)((a.num_chains + nw - 1) / nw)), dim3(64 * nw), lds, st, cg,
                        params, a, walkers, configs, counters);
     JV_LAUNCH_CHECK("mc_cnn");

@@ -16,7 +16,13 @@ from .tdvp import _Timer
 
 
 class MinSR:
-    def __init__(self, sampler, pinvTol=1e-14, diagonalShift=0., diagonalizeOnDevice=True):
+    def __init__(self, sampler, pinvTol=1e-14, diagonalShift=0., diagonalizeOnDevice=True, solver="T"):
+        """``solver``: "T" diagonalises the N x N tangent kernel as the reference
+        does; "dual" diagonalises the P x P Gram D^dagger D, which gives the same
+        update (identical nonzero spectrum); "auto" picks the smaller."""
+        if solver not in ("T", "dual", "auto"):
+            raise ValueError("solver must be 'T', 'dual' or 'auto'")
+        self.solver = solver
         self.sampler = sampler
         self.pinvTol = pinvTol
         self.diagonalShift = diagonalShift
@@ -44,7 +50,8 @@ class MinSR:
         cpx = torch.is_complex(D)
         upd = torch.empty(p, dtype=torch.complex128 if cpx else torch.float64, device=D.device)
         info = _lib.JvSolveInfo()
-        _lib.call("jv_minsr_solve", _lib.ptr(D), _lib.ptr(e), n, p, int(cpx), float(self.pinvTol),
+        dual = self.solver == "dual" or (self.solver == "auto" and n > p)
+        _lib.call("jv_minsr_solve_dual" if dual else "jv_minsr_solve", _lib.ptr(D), _lib.ptr(e), n, p, int(cpx), float(self.pinvTol),
                   float(self.diagonalShift), _lib.ptr(upd), ctypes.byref(info), _lib.stream_ptr())
         self.last_info = info
         if cpx:
