@@ -150,7 +150,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--mode", default="T", choices=["T", "dual"])
+    ap.add_argument("--mode", default="dual", choices=["T", "dual"])
+    ap.add_argument("--t-steps", type=int, default=3, help="steps of the reference T-form solve reported beside")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -158,6 +159,13 @@ def main():
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     elapsed, kt, energies, eq = run_gpu(args.steps, args.warmup, args.mode)
+    other = None
+    if args.t_steps > 0:
+        om = "T" if args.mode == "dual" else "dual"
+        e2, kt2, _, _ = run_gpu(args.t_steps, 1, om)
+        other = {"solver_form": om, "value": args.t_steps / e2, "unit": "steps/s",
+                 "ms_per_step": 1000.0 * e2 / args.t_steps,
+                 "kernel_ms_per_step": {k: v[0] / args.t_steps for k, v in kt2.items() if v[1] > 0}}
     ms_step = 1000.0 * elapsed / args.steps
     steps_s = args.steps / elapsed
     # dominant hand-written kernel
@@ -189,13 +197,17 @@ def main():
         "dtype": "f64",
         "data": "synthetic (He-normal CNN parameters, Metropolis samples of the network itself)",
         "config": {"workload": "TFIM 10x10 periodic, h=3; CNN 3x(8ch,3x3) log-cosh, P=1240; 512 chains, "
-                               "8192 samples/step; MinSR pinvTol=1e-10 (%s form); Euler dt=5e-3" % args.mode,
+                               "8192 samples/step; MinSR pinvTol=1e-10 (%s form); Euler dt=5e-3" % (
+            "dual form: eigendecomposition of the 1240x1240 D^T D, the same update as the 8192x8192 T solve"
+            if args.mode == "dual" else "T"),
                    "samples_per_step": SAMPLES, "solver_form": args.mode},
         "samples_per_s": steps_s * SAMPLES,
         "kernel_ms_per_step": {k: v[0] / args.steps for k, v in kt.items() if v[1] > 0},
         "energy_per_site_last": energies[-1] / 100.0 if energies else None,
         "roofline": roof,
     }
+    if other is not None:
+        out["other_solver_form"] = other
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline()

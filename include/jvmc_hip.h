@@ -75,6 +75,10 @@ typedef struct jv_optable {
   int32_t max_len;
   int32_t l_dim;             /* local dimension, must be 2 */
   int32_t num_diag;
+  int32_t max_sites_changed; /* upper bound on sites one string changes (factors with a
+                                non-identity map); 0 = unknown.  1 enables the single-flip
+                                window kernels. */
+  int32_t reserved;
   const int32_t* idx;        /* [num_ops][max_len] site of each factor */
   const int32_t* map;        /* [num_ops][max_len][l_dim] new local state */
   const double* matel;       /* [num_ops][max_len][l_dim] complex factor */

@@ -179,6 +179,25 @@ def test_eloc_cnn(shape):
     assert rel(e, eo) < 1e-11
 
 
+def test_eloc_window_kernel_equals_generic_kernel():
+    """Single-flip window kernel and the generic masked kernel on the same C3
+    inputs (max_sites_changed = 0 forces the generic path)."""
+    rng = np.random.default_rng(28)
+    shape = (10, 10)
+    net = jv.nets.CNN(F=(3, 3), channels=(8, 8, 8), strides=(1, 1))
+    psi = jv.NQS(net)
+    psi(torch.zeros((1, 1) + shape, dtype=torch.int32))
+    psi.set_parameters(net.he_normal(psi.desc, rng))
+    H, _ = tfim_pair(shape, -1.0, -3.0)
+    H2, _ = tfim_pair(shape, -1.0, -3.0)
+    H2.compile()
+    assert H.compile().max_sites_changed == 1
+    H2.max_sites_changed = 0
+    s = dev(rng.integers(0, 2, size=(1, 512) + shape).astype(np.int32))
+    lp = psi(s)
+    assert rel(to_np(H.get_O_loc(s, psi, lp)), to_np(H2.get_O_loc(s, psi, lp))) < 1e-13
+
+
 def test_eloc_fused_equals_materialised_c3():
     """Full C3 size (10x10, 8192 configurations): the fused kernel equals the
     materialised s' + forward path (base.py:166-192) on the device."""
@@ -222,11 +241,14 @@ def test_mc_rbm_bit_exact():
         assert float(ms.acceptance_ratio()[0]) == mo.acceptance_ratio()
 
 
-def test_mc_cnn_bit_exact():
+@pytest.mark.parametrize("ch", [(4, 4), (4, 4, 4)])
+def test_mc_cnn_bit_exact(ch):
+    """(4, 4): receptive-field windows fit the 6x6 lattice (window kernel);
+    (4, 4, 4): they wrap (generic masked kernel)."""
     rng = np.random.default_rng(19)
     shape = (6, 6)
-    onet = onets.CNN(shape, F=(3, 3), channels=(4, 4), strides=(1, 1))
-    net = jv.nets.CNN(F=(3, 3), channels=(4, 4), strides=(1, 1))
+    onet = onets.CNN(shape, F=(3, 3), channels=ch, strides=(1, 1))
+    net = jv.nets.CNN(F=(3, 3), channels=ch, strides=(1, 1))
     psi0 = jv.NQS(net)
     psi0(torch.zeros((1, 1) + shape, dtype=torch.int32))
     P = net.he_normal(psi0.desc, rng)
@@ -466,3 +488,23 @@ def test_minsr_dual_holomorphic():
     upd = to_np(eq(psi.get_parameters(), 0.0, psi=psi, hamiltonian=H, intStep=0))
     ref = osolv.MinSR(eso, pinvTol=1e-8)(opsi.get_parameters(), 0.0, psi=opsi, hamiltonian=Ho, intStep=0)
     assert rel(upd, ref) < 1e-7
+
+
+def test_checkpoint_resume_bit_exact(tmp_path):
+    """A run resumed from a checkpoint draws the same samples as the
+    uninterrupted run (walkers and Philox counters are saved)."""
+    rng = np.random.default_rng(29)
+    L = 10
+    P = 0.3 * rng.normal(size=onets.CpxRBM((L,), numHidden=6, bias=True).num_real_params())
+
+    def fresh():
+        psi = make_psi(jv.nets.CpxRBM(numHidden=6, bias=True), (L,), P)
+        return psi, jv.sampler.MCSampler(psi, (L,), 11, updateProposer=jv.sampler.propose_spin_flip, numChains=32)
+    psi, ms = fresh()
+    ms.sample(numSamples=256)
+    jv.util.checkpoint.save(str(tmp_path / "ck.npz"), psi, ms, time=0.5)
+    ref = to_np(ms.sample(numSamples=256)[0])
+    psi2, ms2 = fresh()
+    t = jv.util.checkpoint.load(str(tmp_path / "ck.npz"), psi2, ms2)
+    assert t == 0.5
+    np.testing.assert_array_equal(to_np(ms2.sample(numSamples=256)[0]), ref)

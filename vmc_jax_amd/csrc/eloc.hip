@@ -284,6 +284,173 @@ __global__ void __launch_bounds__(512) eloc_cnn(CnnGeom g, const double* __restr
   }
 }
 
+// ------------------------------------------------------------------ CNN O_loc, single-flip windows
+// One workgroup per sample.  After a position-major forward pass of s into LDS,
+// every string is classified (m, changed sites); strings that change one site
+// are processed in chunks of `fch` flips, layer by layer: all threads compute
+// window outputs of all flips of the chunk (one output position, all channels
+// per thread), so each layer costs one barrier for the whole chunk.  The last
+// layer's output changes are reduced per flip in a fixed order, and the
+// sample's O_loc is summed over strings in string order.
+struct EloWinLayout {
+  size_t bytes;
+  int fch;
+};
+
+__host__ __device__ inline size_t elo_win_bytes(const CnnGeom& g, int nops, int fch) {
+  size_t d = (size_t)g.n + g.atot + 8 + 4 * (size_t)nops;  // x0, A, red, m + contrib (complex)
+  for (int l = 0; l < g.nl - 1; ++l) d += (size_t)fch * win_ex(g, l) * win_ey(g, l) * g.C[l];
+  d += (size_t)fch * win_ex(g, g.nl - 1) * win_ey(g, g.nl - 1);  // last-layer partial sums
+  size_t i32 = (size_t)g.n + nops + fch;                          // cs, code, flip list
+  size_t sh = (size_t)g.n * g.FF;                                 // nbr
+  return 8 * d + 4 * i32 + 2 * sh + 64;
+}
+
+template <int CM>
+__global__ void __launch_bounds__(256) eloc_cnn_win(CnnGeom g, const double* __restrict__ Pg, OpDev op,
+                                                    const int32_t* __restrict__ cfg,
+                                                    const double* __restrict__ lps, int64_t n, int fch,
+                                                    double* __restrict__ out) {
+  extern __shared__ double lds[];
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int nops = op.num_ops;
+  double* x0 = lds;
+  double* A = x0 + g.n;
+  double* red = A + g.atot;
+  double* mstr = red + 8;            // complex [nops]
+  double* contrib = mstr + 2 * nops; // complex [nops]
+  double* W[JV_MAX_LAYERS];
+  double* wp = contrib + 2 * nops;
+  for (int l = 0; l < g.nl - 1; ++l) {
+    W[l] = wp;
+    wp += (size_t)fch * win_ex(g, l) * win_ey(g, l) * g.C[l];
+  }
+  double* part = wp;
+  wp += (size_t)fch * win_ex(g, g.nl - 1) * win_ey(g, g.nl - 1);
+  int32_t* cs = (int32_t*)wp;
+  int32_t* code = cs + g.n;     // per string: -1 none changed, -2 several, else 2*site + new value
+  int32_t* flips = code + nops; // string index of each queued flip
+  short* nbr = (short*)(flips + fch);
+  for (int t = tid; t < g.n * g.FF; t += nt) {
+    int p = t / g.FF, k = t % g.FF;
+    int i = p / g.Ly, j = p % g.Ly, ky = k / g.Fy, kx = k % g.Fy;
+    nbr[t] = (short)(((i + ky) % g.Lx) * g.Ly + (j + kx) % g.Ly);
+  }
+  __syncthreads();
+  auto sync = [] { __syncthreads(); };
+  const int L1 = g.nl - 1;
+  const int wlast = win_ex(g, L1) * win_ey(g, L1);
+  for (int64_t s = blockIdx.x; s < n; s += gridDim.x) {
+    const int32_t* c = cfg + s * g.n;
+    for (int i = tid; i < g.n; i += nt) {
+      cs[i] = c[i];
+      x0[i] = (double)(2 * c[i] - 1);
+    }
+    __syncthreads();
+    cnn_forward_pos<CM>(g, Pg, x0, A, nullptr, nbr, tid, nt, sync);
+    double pl = 0.0;
+    for (int t = tid; t < g.n * g.C[L1]; t += nt) pl += A[g.aoff[L1] + t];
+    const double lp = block_sum(pl, red) * g.inv_nrm;
+    const cd off = {lp - lps[2 * s], -lps[2 * s + 1]};
+    // classify strings
+    for (int o = tid; o < nops; o += nt) {
+      int sites[MAXLEN], vals[MAXLEN], nch;
+      cd m = apply_string(op, o, cs, sites, vals, nch);
+      mstr[2 * o] = m.re;
+      mstr[2 * o + 1] = m.im;
+      code[o] = nch == 0 ? -1 : (nch == 1 ? 2 * sites[0] + vals[0] : -2);
+      contrib[2 * o] = 0.0;
+      contrib[2 * o + 1] = 0.0;
+    }
+    __syncthreads();
+    if (tid == 0 && op.num_diag > 1) {  // diagonal merge (branch_free.py:483-485)
+      cd sum = {0.0, 0.0};
+      for (int d = 0; d < op.num_diag; ++d) sum = sum + cd{mstr[2 * op.diag[d]], mstr[2 * op.diag[d] + 1]};
+      for (int d = 1; d < op.num_diag; ++d) { mstr[2 * op.diag[d]] = 0.0; mstr[2 * op.diag[d] + 1] = 0.0; }
+      mstr[2 * op.diag[0]] = sum.re;
+      mstr[2 * op.diag[0] + 1] = sum.im;
+    }
+    __syncthreads();
+    // strings without a changed site contribute m * exp(off)
+    for (int o = tid; o < nops; o += nt) {
+      cd m = {mstr[2 * o], mstr[2 * o + 1]};
+      if (code[o] == -1 && nonzero(m)) {
+        cd r = m * cexp(off);
+        contrib[2 * o] = r.re;
+        contrib[2 * o + 1] = r.im;
+      }
+    }
+    // single flips, chunk by chunk (the queue is rebuilt in string order by thread 0)
+    int next = 0;
+    while (true) {
+      __syncthreads();
+      if (tid == 0) {
+        int nf = 0;
+        while (next < nops && nf < fch) {
+          cd m = {mstr[2 * next], mstr[2 * next + 1]};
+          if (code[next] >= 0 && nonzero(m)) flips[nf++] = next;
+          ++next;
+        }
+        red[0] = (double)nf;
+        red[1] = (double)next;
+      }
+      __syncthreads();
+      const int nf = (int)red[0];
+      next = (int)red[1];
+      if (nf == 0) break;
+      double z[CM];
+      for (int l = 0; l < g.nl; ++l) {
+        const int ex = win_ex(g, l), ey = win_ey(g, l), wsz = ex * ey, cout = g.C[l];
+        const int pwsz = win_ex(g, l - 1) * win_ey(g, l - 1);
+        const double* Xold = l == 0 ? x0 : A + g.aoff[l - 1];
+        for (int t = tid; t < nf * wsz; t += nt) {
+          const int f = t / wsz, j = t % wsz;
+          const int cd_ = code[flips[f]];
+          const int site = cd_ >> 1, nv = cd_ & 1;
+          const int si = site / g.Ly, sj = site % g.Ly;
+          const int a = j / ey, b = j % ey;
+          const double* Wprev = l == 0 ? nullptr : W[l - 1] + (size_t)f * pwsz * g.C[l - 1];
+          cnn_preact_win<CM>(g, Pg, l, Xold, Wprev, (double)(2 * nv - 1), si, sj, a, b, z);
+          if (l < L1) {
+            double* Wo = W[l] + ((size_t)f * wsz + j) * cout;
+#pragma unroll
+            for (int cc = 0; cc < CM; ++cc)
+              if (cc < cout) Wo[cc] = log_cosh(z[cc]);
+          } else {
+            const double* Ab = A + g.aoff[l] + win_pos(g, si, sj, a, b) * cout;
+            double acc = 0.0;
+#pragma unroll
+            for (int cc = 0; cc < CM; ++cc)
+              if (cc < cout) acc += log_cosh(z[cc]) - Ab[cc];
+            part[f * wlast + j] = acc;
+          }
+        }
+        __syncthreads();
+      }
+      for (int f = tid; f < nf; f += nt) {
+        double d = 0.0;
+        for (int j = 0; j < wlast; ++j) d += part[f * wlast + j];
+        const int o = flips[f];
+        cd r = cd{mstr[2 * o], mstr[2 * o + 1]} * cexp({d * g.inv_nrm + off.re, off.im});
+        contrib[2 * o] = r.re;
+        contrib[2 * o + 1] = r.im;
+      }
+    }
+    __syncthreads();
+    if (tid == 0) {
+      cd tot = {0.0, 0.0};
+      for (int o = 0; o < nops; ++o) {
+        tot = tot + cd{contrib[2 * o], contrib[2 * o + 1]};
+        // a string changing several sites violates the caller's max_sites_changed bound
+        if (code[o] == -2 && nonzero(cd{mstr[2 * o], mstr[2 * o + 1]})) tot = {NAN, NAN};
+      }
+      out[2 * s] = tot.re;
+      out[2 * s + 1] = tot.im;
+    }
+    __syncthreads();
+  }
+}
+
 OpDev make_opdev(const jv_optable* op, const double* pref, int L) {
   OpDev d;
   d.L = L;
@@ -296,6 +463,16 @@ OpDev make_opdev(const jv_optable* op, const double* pref, int L) {
   d.diag = op->diag;
   d.pref = pref;
   return d;
+}
+
+int cnn_cmax(const CnnGeom& g) {
+  int m = 0;
+  for (int l = 0; l < g.nl; ++l) m = std::max(m, g.C[l]);
+  if (m <= 4) return 4;
+  if (m <= 8) return 8;
+  if (m <= 16) return 16;
+  if (m <= 32) return 32;
+  return 0;
 }
 
 int check_op(const jv_optable* op, const double* pref) {
@@ -336,6 +513,7 @@ int jv_get_s_primes(const jv_optable* op, const double* prefactor, const int32_t
 int jv_eloc(const jv_net* net, const double* params, const jv_optable* op, const double* prefactor,
             const int32_t* configs, const double* logpsi_s, int64_t n, double* out, void* stream) {
   int rc = check_op(op, prefactor);
+  const bool op_single_site = op->max_sites_changed == 1;
   if (rc) return rc;
   if (!net || !params || !configs || !logpsi_s || !out || n < 0) return fail(JV_E_ARG, "jv_eloc: bad arguments");
   if (n == 0) return JV_OK;
@@ -352,6 +530,26 @@ int jv_eloc(const jv_net* net, const double* params, const jv_optable* op, const
     hipLaunchKernelGGL(eloc_rbm, dim3((unsigned)blocks), dim3(256), lds, st, rg, params, d, configs, logpsi_s,
                        n, out);
     JV_LAUNCH_CHECK("eloc_rbm");
+    return JV_OK;
+  }
+  if (make_cnn_geom(net, cg) && op_single_site && win_ok(cg) && cnn_cmax(cg) > 0) {
+    int fch = 32;
+    while (fch > 1 && elo_win_bytes(cg, op->num_ops, fch) > 78 * 1024) fch /= 2;
+    size_t lds = elo_win_bytes(cg, op->num_ops, fch);
+    int64_t blocks = std::min<int64_t>(n, 16384);
+    ScopedTimer tm("eloc", st);
+    switch (cnn_cmax(cg)) {
+#define JV_ELO(CMv)                                                                                     \
+  case CMv:                                                                                             \
+    rc = set_lds_limit((const void*)eloc_cnn_win<CMv>, lds);                                            \
+    if (rc) return rc;                                                                                  \
+    hipLaunchKernelGGL(eloc_cnn_win<CMv>, dim3((unsigned)blocks), dim3(256), lds, st, cg, params, d, configs, \
+                       logpsi_s, n, fch, out);                                                          \
+    break;
+      JV_ELO(4) JV_ELO(8) JV_ELO(16) JV_ELO(32)
+#undef JV_ELO
+    }
+    JV_LAUNCH_CHECK("eloc_cnn_win");
     return JV_OK;
   }
   if (make_cnn_geom(net, cg)) {

@@ -198,3 +198,116 @@ __device__ inline double cnn_local_delta(const CnnGeom& g, const double* P, cons
 }
 
 }  // namespace jv
+
+namespace jv {
+
+// ------------------------------------------------------------------ position-major CNN
+// One lane computes all output channels of one position.  The weight index
+// depends only on loop counters, so weight loads are wave-uniform (served from
+// the scalar cache); the accumulation order per channel equals cnn_preact's, so
+// both give bit-identical pre-activations.
+template <int CM>
+__device__ inline void cnn_preact_all(const CnnGeom& g, const double* __restrict__ Pg, int l, const double* X,
+                                      const short* nbr, int p, double* z) {
+  const int cin = g.Cin[l], cout = g.C[l];
+  const double* __restrict__ K = Pg + g.koff[l];
+  const int64_t bo = g.boff[l];
+#pragma unroll
+  for (int c = 0; c < CM; ++c) z[c] = (c < cout && bo >= 0) ? Pg[bo + c] : 0.0;
+  for (int k = 0; k < g.FF; ++k) {
+    const double* x = X + (int)nbr[p * g.FF + k] * cin;
+    const double* __restrict__ w = K + k * cin * cout;
+    for (int ci = 0; ci < cin; ++ci) {
+      const double xv = x[ci];
+#pragma unroll
+      for (int c = 0; c < CM; ++c)
+        if (c < cout) z[c] = fma(xv, w[ci * cout + c], z[c]);
+    }
+  }
+}
+
+// Forward pass, position-major.  A (and D = tanh(z) when non-null) are
+// [pos][channel] per layer as before.
+template <int CM, typename Sync>
+__device__ inline void cnn_forward_pos(const CnnGeom& g, const double* __restrict__ Pg, const double* x0, double* A,
+                                       double* D, const short* nbr, int tid, int nt, Sync sync) {
+  double z[CM];
+  for (int l = 0; l < g.nl; ++l) {
+    const double* X = l == 0 ? x0 : A + g.aoff[l - 1];
+    double* Al = A + g.aoff[l];
+    double* Dl = D ? D + g.aoff[l] : nullptr;
+    const int cout = g.C[l];
+    for (int p = tid; p < g.n; p += nt) {
+      cnn_preact_all<CM>(g, Pg, l, X, nbr, p, z);
+#pragma unroll
+      for (int c = 0; c < CM; ++c)
+        if (c < cout) {
+          Al[p * cout + c] = log_cosh(z[c]);
+          if (Dl) Dl[p * cout + c] = tanh(z[c]);
+        }
+    }
+    sync();
+  }
+}
+
+// Receptive-field windows of a single flipped site s = (si, sj).  The outputs
+// of layer l that depend on s are ((si - a) mod Lx, (sj - b) mod Ly) for
+// 0 <= a < ex(l), 0 <= b < ey(l), ex(l) = 1 + (l+1)(Fx-1); window index a*ey + b.
+// Valid while ex(nl-1) <= Lx and ey(nl-1) <= Ly (no wrap-around aliasing).
+__device__ __host__ inline int win_ex(const CnnGeom& g, int l) { return l < 0 ? 1 : 1 + (l + 1) * (g.Fx - 1); }
+__device__ __host__ inline int win_ey(const CnnGeom& g, int l) { return l < 0 ? 1 : 1 + (l + 1) * (g.Fy - 1); }
+__device__ __host__ inline bool win_ok(const CnnGeom& g) {
+  return win_ex(g, g.nl - 1) <= g.Lx && win_ey(g, g.nl - 1) <= g.Ly;
+}
+
+// Pre-activations of window element (a, b) of layer l after flipping site
+// (si, sj).  Inputs inside the layer-(l-1) window come from Wprev (window-major,
+// [a*ey + b][cin]); for l == 0 the only changed input is xflip at the site.
+template <int CM>
+__device__ inline void cnn_preact_win(const CnnGeom& g, const double* __restrict__ Pg, int l, const double* Xold,
+                                      const double* Wprev, double xflip, int si, int sj, int a, int b, double* z) {
+  const int cin = g.Cin[l], cout = g.C[l];
+  const double* __restrict__ K = Pg + g.koff[l];
+  const int64_t bo = g.boff[l];
+  const int pex = win_ex(g, l - 1), pey = win_ey(g, l - 1);
+#pragma unroll
+  for (int c = 0; c < CM; ++c) z[c] = (c < cout && bo >= 0) ? Pg[bo + c] : 0.0;
+  for (int ky = 0; ky < g.Fx; ++ky) {
+    const int aq = a - ky;
+    int qi = si - aq;
+    qi = qi < 0 ? qi + g.Lx : (qi >= g.Lx ? qi - g.Lx : qi);
+    for (int kx = 0; kx < g.Fy; ++kx) {
+      const int bq = b - kx;
+      int qj = sj - bq;
+      qj = qj < 0 ? qj + g.Ly : (qj >= g.Ly ? qj - g.Ly : qj);
+      const bool in = aq >= 0 && aq < pex && bq >= 0 && bq < pey;
+      const int k = ky * g.Fy + kx;
+      const double* __restrict__ w = K + k * cin * cout;
+      if (l == 0) {
+        const double xv = in ? xflip : Xold[qi * g.Ly + qj];
+#pragma unroll
+        for (int c = 0; c < CM; ++c)
+          if (c < cout) z[c] = fma(xv, w[c], z[c]);
+      } else {
+        const double* x = in ? Wprev + (aq * pey + bq) * cin : Xold + (qi * g.Ly + qj) * cin;
+        for (int ci = 0; ci < cin; ++ci) {
+          const double xv = x[ci];
+#pragma unroll
+          for (int c = 0; c < CM; ++c)
+            if (c < cout) z[c] = fma(xv, w[ci * cout + c], z[c]);
+        }
+      }
+    }
+  }
+}
+
+// Site index of window element (a, b) of a flip at (si, sj).
+__device__ inline int win_pos(const CnnGeom& g, int si, int sj, int a, int b) {
+  int pi = si - a;
+  pi = pi < 0 ? pi + g.Lx : pi;
+  int pj = sj - b;
+  pj = pj < 0 ? pj + g.Ly : pj;
+  return pi * g.Ly + pj;
+}
+
+}  // namespace jv

@@ -179,6 +179,113 @@ __global__ void __launch_bounds__(256) mc_cnn(CnnGeom g, const double* __restric
   }
 }
 
+// ------------------------------------------------------------------ CNN chains, single-flip windows
+// One wavefront per chain; lane j computes all channels of window position j of
+// each layer (closed-form receptive field of the flipped site), so a proposal
+// costs one pass over the 9 / 25 / 49 window positions of the 3x3 net.
+__host__ __device__ inline size_t mc_win_chain_doubles(const CnnGeom& g) {
+  size_t d = (size_t)g.n + g.atot + (g.n + 1) / 2;  // x0, A, walker (int32)
+  for (int l = 0; l < g.nl; ++l) d += (size_t)win_ex(g, l) * win_ey(g, l) * g.C[l];
+  return d;
+}
+__host__ __device__ inline size_t mc_win_shared_doubles(const CnnGeom& g) {
+  return ((size_t)g.n * g.FF + 3) / 4;  // nbr table (shorts)
+}
+
+template <int CM>
+__global__ void __launch_bounds__(256) mc_cnn_win(CnnGeom g, const double* __restrict__ Pg, jv_mc_args a,
+                                                  int32_t* __restrict__ walkers, int32_t* __restrict__ configs,
+                                                  int64_t* __restrict__ counters) {
+  extern __shared__ double lds[];
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int lane = tid % WAVE, w = tid / WAVE, nw = nt / WAVE;
+  short* nbr = (short*)lds;
+  for (int t = tid; t < g.n * g.FF; t += nt) {
+    int p = t / g.FF, k = t % g.FF;
+    int i = p / g.Ly, j = p % g.Ly, ky = k / g.Fy, kx = k % g.Fy;
+    nbr[t] = (short)(((i + ky) % g.Lx) * g.Ly + (j + kx) % g.Ly);
+  }
+  __syncthreads();
+  const int chain = blockIdx.x * nw + w;
+  if (chain >= a.num_chains) return;
+  double* x0 = lds + mc_win_shared_doubles(g) + (size_t)w * mc_win_chain_doubles(g);
+  double* A = x0 + g.n;
+  double* W[JV_MAX_LAYERS];
+  double* wp = A + g.atot;
+  for (int l = 0; l < g.nl; ++l) {
+    W[l] = wp;
+    wp += (size_t)win_ex(g, l) * win_ey(g, l) * g.C[l];
+  }
+  int32_t* s = (int32_t*)wp;
+  for (int i = lane; i < g.n; i += WAVE) {
+    s[i] = walkers[(int64_t)chain * g.n + i];
+    x0[i] = (double)(2 * s[i] - 1);
+  }
+  wave_sync();
+  cnn_forward_pos<CM>(g, Pg, x0, A, nullptr, nbr, lane, WAVE, [] { wave_sync(); });
+  const int L1 = g.nl - 1;
+  int64_t nacc = 0;
+  double z[CM];
+  const int total = a.therm_steps + a.samples_per_chain * a.sweep_steps;
+  for (int step = 0; step < total; ++step) {
+    int site;
+    double u;
+    mc_draw(a.seed, (uint32_t)step, (uint32_t)chain, a.call_index, a.stream, g.n, site, u);
+    const int si = site / g.Ly, sj = site % g.Ly;
+    const int nv = 1 - s[site];
+    const double xflip = (double)(2 * nv - 1);
+    double diff = 0.0;
+    for (int l = 0; l < g.nl; ++l) {
+      const int ey = win_ey(g, l), wsz = win_ex(g, l) * ey, cout = g.C[l];
+      const double* Xold = l == 0 ? x0 : A + g.aoff[l - 1];
+      for (int j = lane; j < wsz; j += WAVE) {
+        const int aa = j / ey, bb = j % ey;
+        cnn_preact_win<CM>(g, Pg, l, Xold, l == 0 ? nullptr : W[l - 1], xflip, si, sj, aa, bb, z);
+        double* Wo = W[l] + j * cout;
+        const double* Ab = A + g.aoff[l] + win_pos(g, si, sj, aa, bb) * cout;
+#pragma unroll
+        for (int c = 0; c < CM; ++c)
+          if (c < cout) {
+            const double v = log_cosh(z[c]);
+            Wo[c] = v;
+            if (l == L1) diff += v - Ab[c];
+          }
+      }
+      wave_sync();
+    }
+    const double dl = wave_sum(diff) * g.inv_nrm;
+    const bool acc = u < exp(a.mu * dl);  // random.bernoulli(key, P), sampler.py:336-340
+    if (acc) {
+      for (int l = 0; l < g.nl; ++l) {
+        const int ey = win_ey(g, l), wsz = win_ex(g, l) * ey, cout = g.C[l];
+        for (int t = lane; t < wsz * cout; t += WAVE) {
+          const int j = t / cout, c = t % cout;
+          A[g.aoff[l] + win_pos(g, si, sj, j / ey, j % ey) * cout + c] = W[l][t];
+        }
+      }
+      if (lane == 0) {
+        s[site] = nv;
+        x0[site] = xflip;
+      }
+      ++nacc;
+    }
+    wave_sync();
+    const int k = step + 1 - a.therm_steps;
+    if (k > 0 && k % a.sweep_steps == 0) {
+      const int64_t t = k / a.sweep_steps - 1;
+      int32_t* out = configs + (t * a.num_chains + chain) * g.n;
+      for (int i = lane; i < g.n; i += WAVE) out[i] = s[i];
+    }
+  }
+  for (int i = lane; i < g.n; i += WAVE) walkers[(int64_t)chain * g.n + i] = s[i];
+  if (lane == 0) {
+    atomicAdd((unsigned long long*)&counters[0], (unsigned long long)total);
+    atomicAdd((unsigned long long*)&counters[1], (unsigned long long)nacc);
+  }
+}
+
+int cnn_cmax(const CnnGeom& g);
+
 // ------------------------------------------------------------------ weights
 // p_i = exp(scale * Re(logpsi_i) - shift); one block, deterministic order.
 __global__ void __launch_bounds__(1024) weights_sum(const double* __restrict__ lp, int64_t n, double scale,
@@ -243,6 +350,24 @@ int jv_mc_sample(const jv_net* net, const double* params, const jv_mc_args* args
     hipLaunchKernelGGL(mc_rbm, dim3((unsigned)((a.num_chains + nw - 1) / nw)), dim3(64 * nw), lds, st, rg,
                        params, a, walkers, configs, counters);
     JV_LAUNCH_CHECK("mc_rbm");
+  } else if (make_cnn_geom(net, cg) && win_ok(cg) && cnn_cmax(cg) > 0) {
+    const int nw = a.num_chains <= 1024 ? 1 : 4;
+    size_t lds = 8 * (mc_win_shared_doubles(cg) + nw * mc_win_chain_doubles(cg));
+    const unsigned blocks = (unsigned)((a.num_chains + nw - 1) / nw);
+    ScopedTimer tm("mc_sweep", st);
+    int rc = JV_OK;
+    switch (cnn_cmax(cg)) {
+#define JV_MCW(CMv)                                                                                     \
+  case CMv:                                                                                             \
+    rc = set_lds_limit((const void*)mc_cnn_win<CMv>, lds);                                              \
+    if (rc) return rc;                                                                                  \
+    hipLaunchKernelGGL(mc_cnn_win<CMv>, dim3(blocks), dim3(64 * nw), lds, st, cg, params, a, walkers,       \
+                       configs, counters);                                                              \
+    break;
+      JV_MCW(4) JV_MCW(8) JV_MCW(16) JV_MCW(32)
+#undef JV_MCW
+    }
+    JV_LAUNCH_CHECK("mc_cnn_win");
   } else if (make_cnn_geom(net, cg)) {
     McCnnLayout lay = mc_cnn_layout(cg);
     int nw = 4;

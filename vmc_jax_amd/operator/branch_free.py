@@ -168,6 +168,11 @@ class BranchFreeOperator(Operator):
             idx.append(ri); mp.append(rm); me.append(re)
             if isDiag:
                 diag.append(o)
+        # upper bound on the sites one string changes: factors whose map is not
+        # the identity (the kernels pick single-flip code paths when it is 1)
+        ident = np.arange(self.lDim)
+        self.max_sites_changed = int(max([sum(int(not np.array_equal(np.asarray(f), ident)) for f in row)
+                                          for row in mp] + [0]))
         # host tables (numpy); uploaded to the device on first use
         self.idxH = np.array(idx, dtype=np.int32)
         self.mapH = np.array(mp, dtype=np.int32)
@@ -197,6 +202,7 @@ class BranchFreeOperator(Operator):
             t.max_len = self.maxOpStrLength
             t.l_dim = 2
             t.num_diag = len(self.diagH)
+            t.max_sites_changed = self.max_sites_changed
             t.idx = self.idxC.data_ptr()
             t.map = self.mapC.data_ptr()
             t.matel = self.matElsC.data_ptr()
