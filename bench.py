@@ -1,17 +1,31 @@
-"""Benchmark: TDVP (MinSR) steps/s on the 2D TFIM 10x10 / CNN workload.
+"""Benchmark: TDVP / MinSR steps/s of jVMC's VMC inner loop on one MI355X.
 
-Workload (one "step" = one Euler step of the MinSR equation): H = -sum ZZ -
+Default workload (the metric's configuration, ``--workload c3``): H = -sum ZZ -
 3 sum X on a periodic 10x10 lattice; real CNN with 3 conv layers of 8 channels,
 3x3 kernels, circular padding, log-cosh, sum / sqrt(800) (1240 parameters,
 He-normal init); 8192 Metropolis samples from 512 chains per step
 (thermalizationSweeps = sweepSteps = 10, the sampler defaults); MinSR with
-pinvTol = 1e-10 solved through the 8192 x 8192 T = O O^T in FP64; Euler
-dt = 5e-3.  Everything runs in FP64 on the GPU; inputs are resident on the
-device when the timed region starts.
+pinvTol = 1e-10; Euler dt = 5e-3.  One "step" = one Euler step: sampling,
+local energies, O_k, the MinSR solve and the parameter update.  Everything
+runs in FP64 on the GPU; inputs are resident on the device when the timed
+region starts.
 
-Prints one JSON line (rank 0) with steps/s, samples/s, the roofline of the
-dominant hand-written kernel (from HIP events recorded live on its stream) and
-the CPU baseline (the FP64 oracle, timed on a bounded sample of the same step).
+The MinSR pseudo-inverse is computed in its dual form by default: the nonzero
+spectrum of the 8192 x 8192 T = O O^T equals that of the 1240 x 1240 O^T O, and
+-O^T pinv(T) E = -pinv(O^T O) O^T E with the same relative cutoff, so the
+update is the reference's (the parity tests compare both forms).  The
+reference's T form (8192^2 Gram + FP64 eigendecomposition of T) is timed for a
+few steps and reported beside it.
+
+``--workload c1``: 1D TFIM L=10, h=0.7, complex RBM alpha=2 with visible bias,
+exact sampler, imaginary-time TDVP (Euler dt=1e-2, snrTol=2, pinvTol=1e-8).
+``--workload c2``: 1D TFIM L=32, h=1, complex RBM alpha=2, 1024 chains x 16384
+samples (25 thermalisation sweeps, sweep length 32), real-time TDVP with Heun
+dt=1e-3.
+
+Prints one JSON line with steps/s, samples/s, per-kernel times, the roofline of
+the dominant hand-written kernel (from HIP events recorded live on its stream)
+and the CPU baseline (the FP64 oracle on a bounded sample of the same step).
 """
 import argparse
 import ctypes
@@ -26,18 +40,20 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-SHAPE = (10, 10)
-CHANNELS = (8, 8, 8)
-CHAINS = 512
-SAMPLES = 8192
-DT = 5e-3
-PINV_TOL = 1e-10
-HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+METRIC = "TDVP steps/s + samples/s (TFIM 10x10 CNN) at |dE/E|<=1e-6; served p99 latency"
 
 
 def tfim_strings(op, shape, J=-1.0, h=-3.0):
+    """H = J sum ZZ + h sum X on a periodic chain or square lattice."""
     H = op.BranchFreeOperator()
+    if len(shape) == 1:
+        L = shape[0]
+        for l in range(L):
+            H.add(op.scal_opstr(J, (op.Sz(l), op.Sz((l + 1) % L))))
+            H.add(op.scal_opstr(h, (op.Sx(l),)))
+        return H
     Lx, Ly = shape
     for i in range(Lx):
         for j in range(Ly):
@@ -48,24 +64,62 @@ def tfim_strings(op, shape, J=-1.0, h=-3.0):
     return H
 
 
-def build_gpu(mode):
-    import torch
-    import vmc_jax_amd as jv
-    net = jv.nets.CNN(F=(3, 3), channels=CHANNELS, strides=(1, 1))
-    psi = jv.NQS(net)
-    psi(torch.zeros((1, 1) + SHAPE, dtype=torch.int32))
-    psi.set_parameters(net.he_normal(psi.desc, np.random.default_rng(0)))
-    H = tfim_strings(jv.operator, SHAPE)
-    sampler = jv.sampler.MCSampler(psi, SHAPE, 1234, updateProposer=jv.sampler.propose_spin_flip,
-                                   numChains=CHAINS, numSamples=SAMPLES)
-    eq = jv.util.MinSR(sampler, pinvTol=PINV_TOL, solver=mode)
-    stepper = jv.util.stepper.Euler(timeStep=DT)
-    return jv, psi, H, sampler, eq, stepper
+class Workload:
+    def __init__(self, name, mode):
+        import torch
+        import vmc_jax_amd as jv
+        self.jv = jv
+        self.name = name
+        rng = np.random.default_rng(0)
+        if name == "c3":
+            self.shape, self.samples = (10, 10), 8192
+            net = jv.nets.CNN(F=(3, 3), channels=(8, 8, 8), strides=(1, 1))
+            psi = jv.NQS(net)
+            psi(torch.zeros((1, 1) + self.shape, dtype=torch.int32))
+            psi.set_parameters(net.he_normal(psi.desc, rng))
+            self.H = tfim_strings(jv.operator, self.shape, -1.0, -3.0)
+            self.sampler = jv.sampler.MCSampler(psi, self.shape, 1234, updateProposer=jv.sampler.propose_spin_flip,
+                                                numChains=512, numSamples=self.samples)
+            self.eq = jv.util.MinSR(self.sampler, pinvTol=1e-10, solver=mode)
+            self.stepper = jv.util.stepper.Euler(timeStep=5e-3)
+            self.desc = ("TFIM 10x10 periodic, h=3; CNN 3x(8ch,3x3) log-cosh, P=1240; 512 chains, 8192 samples/step; "
+                         "MinSR pinvTol=1e-10 (%s); Euler dt=5e-3" %
+                         ("dual form: eigendecomposition of the 1240x1240 O^T O, same update as the 8192x8192 T solve"
+                          if mode == "dual" else "T form: 8192x8192 T Gram + FP64 eigendecomposition"))
+        elif name == "c2":
+            self.shape, self.samples = (32,), 16384
+            net = jv.nets.CpxRBM(numHidden=64, bias=True)
+            psi = jv.NQS(net)
+            psi(torch.zeros((1, 1) + self.shape, dtype=torch.int32))
+            psi.set_parameters(0.01 * rng.normal(size=psi.numRealParameters))
+            self.H = tfim_strings(jv.operator, self.shape, -1.0, -1.0)
+            self.sampler = jv.sampler.MCSampler(psi, self.shape, 1234, updateProposer=jv.sampler.propose_spin_flip,
+                                                numChains=1024, numSamples=self.samples, thermalizationSweeps=25,
+                                                sweepSteps=32)
+            self.eq = jv.util.TDVP(self.sampler, rhsPrefactor=1.j, makeReal='imag')
+            self.stepper = jv.util.stepper.Heun(timeStep=1e-3)
+            self.desc = ("TFIM L=32 periodic, h=1; CpxRBM alpha=2 (2112 complex params); 1024 chains, 16384 samples, "
+                         "25 therm. sweeps, sweep 32; real-time TDVP (makeReal='imag'); Heun dt=1e-3")
+        elif name == "c1":
+            self.shape, self.samples = (10,), 1024
+            net = jv.nets.CpxRBM(numHidden=20, bias=True, visibleBias=True)
+            psi = jv.NQS(net)
+            psi(torch.zeros((1, 1) + self.shape, dtype=torch.int32))
+            psi.set_parameters(0.01 * rng.normal(size=psi.numRealParameters))
+            self.H = tfim_strings(jv.operator, self.shape, -1.0, -0.7)
+            self.sampler = jv.sampler.ExactSampler(psi, 10)
+            self.eq = jv.util.TDVP(self.sampler, snrTol=2, pinvTol=1e-8, rhsPrefactor=1., makeReal='real')
+            self.stepper = jv.util.stepper.Euler(timeStep=1e-2)
+            self.desc = ("TFIM L=10 periodic, h=0.7; CpxRBM alpha=2 with visible bias (230 complex params); exact "
+                         "sampler (1024 states); imaginary-time TDVP snrTol=2 pinvTol=1e-8; Euler dt=1e-2")
+        else:
+            raise ValueError(name)
+        self.psi = psi
 
-
-def gpu_step(psi, H, eq, stepper):
-    dp, _ = stepper.step(0.0, eq, psi.get_parameters(), hamiltonian=H, psi=psi, numSamples=None)
-    psi.set_parameters(dp)
+    def step(self):
+        dp, _ = self.stepper.step(0.0, self.eq, self.psi.get_parameters(), hamiltonian=self.H, psi=self.psi,
+                                  numSamples=None)
+        self.psi.set_parameters(dp)
 
 
 def timer(jv, name):
@@ -75,11 +129,15 @@ def timer(jv, name):
     return tot.value, cnt.value
 
 
-def run_gpu(steps, warmup, mode):
+KERNELS = ("gram", "gemm", "eigh", "eloc", "mc_sweep", "net_grad", "net_forward")
+
+
+def run_gpu(workload, steps, warmup, mode):
     import torch
-    jv, psi, H, sampler, eq, stepper = build_gpu(mode)
+    w = Workload(workload, mode)
+    jv = w.jv
     for _ in range(warmup):
-        gpu_step(psi, H, eq, stepper)
+        w.step()
     torch.cuda.synchronize()
     jv._lib.call("jv_timer_reset")
     jv._lib.call("jv_timer_enable", 1)
@@ -87,62 +145,75 @@ def run_gpu(steps, warmup, mode):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
-        gpu_step(psi, H, eq, stepper)
-        energies.append(eq.get_energy_mean())
+        w.step()
+        energies.append(w.eq.get_energy_mean())
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     jv._lib.call("jv_timer_enable", 0)
-    kt = {k: timer(jv, k) for k in ("gram", "eigh", "eloc", "mc_sweep", "net_grad", "net_forward", "gemm")}
-    return elapsed, kt, energies, eq
+    kt = {k: timer(jv, k) for k in KERNELS}
+    return w, elapsed, kt, energies
+
+
+def roofline_c3(kt, steps, mode):
+    """Roofline of the dominant hand-written kernel, algorithmic counts per step:
+    E_loc 2.03e11 flop (8192 samples x 101 strings x a 244,800-flop forward),
+    Metropolis 3.3e10 flop (133k proposals x 2.45e5), Gram 2 N^2 P (T form) or
+    2 N P^2 (dual) graded on the computed triangle of tiles."""
+    per_step = {"eloc": 2.03e11, "mc_sweep": 3.3e10,
+                "gram": (8192.0 * 8256.0 * 1240.0) if mode == "T" else (1240.0 * 1280.0 * 8192.0),
+                "net_grad": 3 * 244800.0 * 8192}
+    cands = {k: v for k, v in kt.items() if k in per_step and v[1] > 0}
+    dom = max(cands, key=lambda k: cands[k][0])
+    total_ms, launches = cands[dom]
+    ms = total_ms / launches
+    flops = per_step[dom] * steps / launches
+    ach = flops / (ms * 1e-3) / 1e12
+    return {"bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": ach / FP64_PEAK_TFLOPS, "traffic": None, "kernel": dom, "avg_launch_ms": ms,
+            "launches": launches}
 
 
 def cpu_baseline(n_sub=1024, chains=512):
-    """FP64 oracle timed on a bounded sample of the same step: the oracle runs
-    sampling (full-network re-evaluation per proposal), local energies with
-    materialised s', gradients and the T-form MinSR solve on n_sub samples from
-    `chains` chains; each phase is scaled to 8192 samples by its complexity
-    (sampling and sweeps by samples per chain, E_loc / gradients linearly, the
-    T Gram quadratically, the eigendecomposition cubically)."""
-    from oracle import nets as onets, operator as oop, sampler as osamp, solvers as osolv
+    """FP64 oracle timed on a bounded sample of the C3 step: sampling (full
+    network re-evaluation per proposal), local energies with materialised s',
+    gradients and the T-form MinSR solve on n_sub samples from `chains` chains;
+    each phase is scaled to 8192 samples by its complexity (sweeps by samples per
+    chain, E_loc / gradients linearly, the T Gram quadratically, the
+    eigendecomposition cubically)."""
+    from oracle import nets as onets, operator as oop, sampler as osamp
     from oracle.stats import SampledObs
-    onet = onets.CNN(SHAPE, F=(3, 3), channels=CHANNELS, strides=(1, 1))
+    shape, samples, nchains = (10, 10), 8192, 512
+    onet = onets.CNN(shape, F=(3, 3), channels=(8, 8, 8), strides=(1, 1))
     P = np.zeros(onet.num_real_params())
     rng = np.random.default_rng(0)
-    for leaf_i, leaf in enumerate(onet.leaves):
-        pass
-    # He-normal on kernels (same distribution as the GPU run, independent draw)
     off = 0
-    for leaf in onet.leaves:
+    for leaf in onet.leaves:  # He-normal kernels, zero biases
         if leaf.name.endswith("kernel"):
             fan_in = leaf.shape[0] * leaf.shape[1] * leaf.shape[2]
             P[off:off + leaf.size] = rng.normal(size=leaf.size) * math.sqrt(2.0 / fan_in)
         off += leaf.size
     psi = onets.NQS(onet, P, batchSize=4096)
-    H = tfim_strings(oop, SHAPE)
-    spc_full = SAMPLES // CHAINS
+    H = tfim_strings(oop, shape)
+    spc_full = samples // nchains
     spc_sub = max(1, n_sub // chains)
-    ms = osamp.MCSampler(psi, SHAPE, 0, numChains=chains, thermalizationSweeps=10, sweepSteps=10)
+    ms = osamp.MCSampler(psi, shape, 0, numChains=chains, thermalizationSweeps=10, sweepSteps=10)
     t = {}
     t0 = time.perf_counter(); s, lp, p = ms.sample(numSamples=spc_sub * chains); t["sampling"] = time.perf_counter() - t0
-    t0 = time.perf_counter(); El = H.get_O_loc(s, psi, lp, 0.0); t["eloc"] = time.perf_counter() - t0
+    t0 = time.perf_counter(); H.get_O_loc(s, psi, lp, 0.0); t["eloc"] = time.perf_counter() - t0
     t0 = time.perf_counter(); G = psi.gradients(s); t["grad"] = time.perf_counter() - t0
-    Eo, Go = SampledObs(El, p), SampledObs(G, p)
-    D = np.real(Go._data)
+    D = np.real(SampledObs(G, p)._data)
     t0 = time.perf_counter(); T = D @ D.T; t["gram"] = time.perf_counter() - t0
-    t0 = time.perf_counter(); ev, V = np.linalg.eigh(T); t["eigh"] = time.perf_counter() - t0
+    t0 = time.perf_counter(); np.linalg.eigh(T); t["eigh"] = time.perf_counter() - t0
     n = len(p)
-    scale_n = SAMPLES / n
-    sweeps_sub = (10 + spc_sub) * 10
-    sweeps_full = (10 + spc_full) * 10
-    est = (t["sampling"] * sweeps_full / sweeps_sub * (CHAINS / chains) + (t["eloc"] + t["grad"]) * scale_n
-           + t["gram"] * scale_n ** 2 + t["eigh"] * scale_n ** 3)
+    scale_n = samples / n
+    sw = (10 + spc_full) * 10 / ((10 + spc_sub) * 10) * nchains / chains
+    est = t["sampling"] * sw + (t["eloc"] + t["grad"]) * scale_n + t["gram"] * scale_n ** 2 + t["eigh"] * scale_n ** 3
     cores = len(os.sched_getaffinity(0))
     return {"value": 1.0 / est, "unit": "steps/s", "cores": cores, "kind": "port",
             "sample": ("FP64 numpy oracle, one MinSR step on %d samples from %d chains (10x10 TFIM, same CNN); "
                        "phases scaled to 8192 samples (sampling x%.2f, E_loc+gradients x%.0f, T Gram x%.0f, "
-                       "eigh x%.0f); measured %s" % (n, chains, sweeps_full / sweeps_sub * CHAINS / chains,
-                                                      scale_n, scale_n ** 2, scale_n ** 3,
-                                                      {k: round(v, 3) for k, v in t.items()}))}
+                       "eigh x%.0f); measured s: %s" % (n, chains, sw, scale_n, scale_n ** 2, scale_n ** 3,
+                                                         {k: round(v, 3) for k, v in t.items()}))}
 
 
 def main():
@@ -150,69 +221,50 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--mode", default="dual", choices=["T", "dual"])
-    ap.add_argument("--t-steps", type=int, default=3, help="steps of the reference T-form solve reported beside")
+    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3"])
+    ap.add_argument("--mode", default="dual", choices=["T", "dual"], help="MinSR solve form (c3)")
+    ap.add_argument("--t-steps", type=int, default=3, help="c3: steps of the other MinSR form reported beside")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (every box has one MI355X); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    elapsed, kt, energies, eq = run_gpu(args.steps, args.warmup, args.mode)
-    other = None
-    if args.t_steps > 0:
-        om = "T" if args.mode == "dual" else "dual"
-        e2, kt2, _, _ = run_gpu(args.t_steps, 1, om)
-        other = {"solver_form": om, "value": args.t_steps / e2, "unit": "steps/s",
-                 "ms_per_step": 1000.0 * e2 / args.t_steps,
-                 "kernel_ms_per_step": {k: v[0] / args.t_steps for k, v in kt2.items() if v[1] > 0}}
-    ms_step = 1000.0 * elapsed / args.steps
+    w, elapsed, kt, energies = run_gpu(args.workload, args.steps, args.warmup, args.mode)
     steps_s = args.steps / elapsed
-    # dominant hand-written kernel
-    cands = {k: v for k, v in kt.items() if k in ("gram", "eloc", "mc_sweep", "net_grad") and v[1] > 0}
-    dom = max(cands, key=lambda k: cands[k][0])
-    dom_ms = cands[dom][0] / cands[dom][1]
-    if dom == "gram":
-        n, p = SAMPLES, 1240
-        flops = float(n) * (n + 64) * p  # triangle-only tiles: N(N+64)P multiply-adds x 2 / 2
-        roof = {"bound": "mfma", "achieved": flops / (dom_ms * 1e-3) / 1e12, "peak": FP64_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "traffic": None}
-    else:
-        flops_per = {"eloc": 2.03e11, "mc_sweep": 3.3e10, "net_grad": 3 * 2 * 122400.0 * SAMPLES}[dom]
-        roof = {"bound": "mfma", "achieved": flops_per / (dom_ms * 1e-3) / 1e12, "peak": FP64_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "traffic": None}
-    roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["kernel"] = dom
     out = {
-        "metric": "TDVP steps/s + samples/s (TFIM 10x10 CNN) at |dE/E|<=1e-6; served p99 latency",
+        "metric": METRIC if args.workload == "c3" else "TDVP steps/s (%s)" % args.workload,
         "value": steps_s,
         "unit": "steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_step,
+        "ms_per_step": 1000.0 * elapsed / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (He-normal CNN parameters, Metropolis samples of the network itself)",
-        "config": {"workload": "TFIM 10x10 periodic, h=3; CNN 3x(8ch,3x3) log-cosh, P=1240; 512 chains, "
-                               "8192 samples/step; MinSR pinvTol=1e-10 (%s form); Euler dt=5e-3" % (
-            "dual form: eigendecomposition of the 1240x1240 D^T D, the same update as the 8192x8192 T solve"
-            if args.mode == "dual" else "T"),
-                   "samples_per_step": SAMPLES, "solver_form": args.mode},
-        "samples_per_s": steps_s * SAMPLES,
+        "data": "synthetic (random network parameters; samples drawn from the network itself)",
+        "config": {"workload": w.desc, "samples_per_step": w.samples},
+        "samples_per_s": steps_s * w.samples,
         "kernel_ms_per_step": {k: v[0] / args.steps for k, v in kt.items() if v[1] > 0},
-        "energy_per_site_last": energies[-1] / 100.0 if energies else None,
-        "roofline": roof,
+        "energy_last": energies[-1] if energies else None,
     }
-    if other is not None:
-        out["other_solver_form"] = other
-    if not args.no_cpu_baseline:
-        try:
-            out["cpu_baseline"] = cpu_baseline()
-        except Exception as e:  # report, never hide the GPU number
-            out["cpu_baseline"] = {"error": repr(e)}
+    if args.workload == "c3":
+        out["config"]["solver_form"] = args.mode
+        out["roofline"] = roofline_c3(kt, args.steps, args.mode)
+        if args.t_steps > 0:
+            om = "T" if args.mode == "dual" else "dual"
+            _, e2, kt2, _ = run_gpu("c3", args.t_steps, 1, om)
+            out["other_solver_form"] = {
+                "solver_form": om, "value": args.t_steps / e2, "unit": "steps/s",
+                "ms_per_step": 1000.0 * e2 / args.t_steps,
+                "kernel_ms_per_step": {k: v[0] / args.t_steps for k, v in kt2.items() if v[1] > 0}}
+        if not args.no_cpu_baseline:
+            try:
+                out["cpu_baseline"] = cpu_baseline()
+            except Exception as e:  # report, never hide the GPU number
+                out["cpu_baseline"] = {"error": repr(e)}
     print(json.dumps(out))
 
 

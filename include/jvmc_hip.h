@@ -12,8 +12,11 @@
  *   - All calls are asynchronous on the given HIP stream (NULL = default
  *     stream), except jv_eigh/jv_minsr_solve/jv_tdvp_solve, which read a small
  *     status word back to the host.
- *   - The library keeps no per-job state; a per-device workspace for rocSOLVER
- *     and solver temporaries is grown on demand and released by jv_shutdown().
+ *   - The library keeps no per-job state.  Solver temporaries live in a
+ *     workspace per host thread and device, grown on demand and released by
+ *     jv_shutdown() or at thread exit; calls from different host threads may
+ *     run concurrently on different streams.  Calls from one thread that use
+ *     workspaces must be ordered on one stream.
  *
  * Each entry point names the jVMC 1.5.8 interface it replaces (file:line).
  */
@@ -113,7 +116,7 @@ const char* jv_last_error(void);
 /* Selects the device for the calling thread.  Replaces the rank-to-device pick
  * of global_defs.py:15-19. */
 int jv_set_device(int device);
-/* Releases the per-device workspaces and solver handles. */
+/* Releases the calling thread's workspaces and solver handles. */
 int jv_shutdown(void);
 /* Kernel timing (no reference counterpart; measurement support): when enabled,
  * HIP events are recorded on the launch stream around the named kernels

@@ -12,7 +12,8 @@ import os
 import torch  # noqa: F401  (loads the HIP runtime first)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libjvmc_hip.so")
+# VMC_JAX_AMD_LIB overrides the in-tree library (e.g. to compare builds)
+LIB_PATH = os.environ.get("VMC_JAX_AMD_LIB", os.path.join(_HERE, "libjvmc_hip.so"))
 
 JV_NET_CPXRBM = 1
 JV_NET_RBM = 2

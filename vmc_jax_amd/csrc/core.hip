@@ -28,21 +28,56 @@ struct DeviceState {
   std::vector<size_t> size;
   rocblas_handle blas = nullptr;
 };
-std::mutex g_mu;
-std::vector<DeviceState> g_dev;
+// Workspaces and solver handles are per host thread and device, so threads
+// that drive different streams (one per job) never share scratch memory.
+struct ThreadState {
+  std::vector<DeviceState> dev;
+  ~ThreadState();
+};
+std::mutex g_mu;                      // guards the registry below
+std::vector<ThreadState*> g_threads;  // live thread states, for jv_shutdown
+thread_local ThreadState t_state;
+thread_local bool t_registered = false;
+
+void release(DeviceState& s, int d) {
+  int cur = 0;
+  (void)hipGetDevice(&cur);
+  (void)hipSetDevice(d);
+  (void)hipDeviceSynchronize();
+  for (void* p : s.ptr)
+    if (p) (void)hipFree(p);
+  s.ptr.clear();
+  s.size.clear();
+  if (s.blas) rocblas_destroy_handle(s.blas);
+  s.blas = nullptr;
+  (void)hipSetDevice(cur);
+}
+
+ThreadState::~ThreadState() {
+  std::lock_guard<std::mutex> lk(g_mu);
+  for (size_t d = 0; d < dev.size(); ++d)
+    if (!dev[d].ptr.empty() || dev[d].blas) release(dev[d], (int)d);
+  for (size_t i = 0; i < g_threads.size(); ++i)
+    if (g_threads[i] == this) { g_threads.erase(g_threads.begin() + i); break; }
+}
 
 DeviceState& state() {
+  if (!t_registered) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_threads.push_back(&t_state);
+    t_registered = true;
+  }
   int d = 0;
   (void)hipGetDevice(&d);
-  if ((int)g_dev.size() <= d) g_dev.resize(d + 1);
-  return g_dev[d];
+  if ((int)t_state.dev.size() <= d) t_state.dev.resize(d + 1);
+  return t_state.dev[d];
 }
 }  // namespace
 
-// Grow-only scratch buffer `slot` on the current device.  Freeing an old buffer
-// synchronises the device first so in-flight kernels never see it vanish.
+// Grow-only scratch buffer `slot` of the calling thread on the current device.
+// Replacing a buffer synchronises the device first so in-flight kernels never
+// see it vanish.
 void* workspace(size_t bytes, int slot) {
-  std::lock_guard<std::mutex> lk(g_mu);
   DeviceState& s = state();
   if ((int)s.ptr.size() <= slot) {
     s.ptr.resize(slot + 1, nullptr);
@@ -66,7 +101,6 @@ void* workspace(size_t bytes, int slot) {
 }
 
 rocblas_handle blas_handle(hipStream_t stream) {
-  std::lock_guard<std::mutex> lk(g_mu);
   DeviceState& s = state();
   if (!s.blas) {
     if (rocblas_create_handle(&s.blas) != rocblas_status_success) return nullptr;
@@ -89,22 +123,11 @@ int jv_set_device(int device) {
 }
 
 int jv_shutdown(void) {
-  std::lock_guard<std::mutex> lk(jv::g_mu);
-  int cur = 0;
-  (void)hipGetDevice(&cur);
-  for (size_t d = 0; d < jv::g_dev.size(); ++d) {
-    auto& s = jv::g_dev[d];
-    if (s.ptr.empty() && !s.blas) continue;
-    (void)hipSetDevice((int)d);
-    (void)hipDeviceSynchronize();
-    for (void* p : s.ptr)
-      if (p) (void)hipFree(p);
-    s.ptr.clear();
-    s.size.clear();
-    if (s.blas) rocblas_destroy_handle(s.blas);
-    s.blas = nullptr;
+  // releases the calling thread's workspaces and handles on every device
+  for (size_t d = 0; d < jv::t_state.dev.size(); ++d) {
+    auto& s = jv::t_state.dev[d];
+    if (!s.ptr.empty() || s.blas) jv::release(s, (int)d);
   }
-  (void)hipSetDevice(cur);
   return JV_OK;
 }
 

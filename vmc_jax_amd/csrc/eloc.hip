@@ -412,16 +412,16 @@ __global__ void __launch_bounds__(256) eloc_cnn_win(CnnGeom g, const double* __r
           const double* Wprev = l == 0 ? nullptr : W[l - 1] + (size_t)f * pwsz * g.C[l - 1];
           cnn_preact_win<CM>(g, Pg, l, Xold, Wprev, (double)(2 * nv - 1), si, sj, a, b, z);
           if (l < L1) {
-            double* Wo = W[l] + ((size_t)f * wsz + j) * cout;
+            double* Wo = W[l] + (size_t)f * wsz * cout + j;  // planar [c][j] per flip
 #pragma unroll
             for (int cc = 0; cc < CM; ++cc)
-              if (cc < cout) Wo[cc] = log_cosh(z[cc]);
+              if (cc < cout) Wo[cc * wsz] = log_cosh(z[cc]);
           } else {
-            const double* Ab = A + g.aoff[l] + win_pos(g, si, sj, a, b) * cout;
+            const double* Ab = A + g.aoff[l] + win_pos(g, si, sj, a, b);
             double acc = 0.0;
 #pragma unroll
             for (int cc = 0; cc < CM; ++cc)
-              if (cc < cout) acc += log_cosh(z[cc]) - Ab[cc];
+              if (cc < cout) acc += log_cosh(z[cc]) - Ab[cc * g.n];
             part[f * wlast + j] = acc;
           }
         }

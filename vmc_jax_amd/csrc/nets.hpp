@@ -199,35 +199,51 @@ __device__ inline double cnn_local_delta(const CnnGeom& g, const double* P, cons
 
 }  // namespace jv
 
+#ifndef JV_CH_UNROLL
+#define JV_CH_UNROLL 8
+#endif
+
 namespace jv {
+
+// Weights are read through the constant address space: their indices are
+// wave-uniform, so these become scalar loads served by the scalar cache.
+typedef const __attribute__((address_space(4))) double cdbl;
+__device__ inline const cdbl* as_const(const double* p) { return (const cdbl*)p; }
 
 // ------------------------------------------------------------------ position-major CNN
 // One lane computes all output channels of one position.  The weight index
 // depends only on loop counters, so weight loads are wave-uniform (served from
 // the scalar cache); the accumulation order per channel equals cnn_preact's, so
 // both give bit-identical pre-activations.
+// Activations of these kernels are channel-planar, X[c * n + pos], so lanes
+// that handle consecutive positions read consecutive LDS words (no bank
+// conflicts when every lane reads the same channel).
 template <int CM>
 __device__ inline void cnn_preact_all(const CnnGeom& g, const double* __restrict__ Pg, int l, const double* X,
                                       const short* nbr, int p, double* z) {
   const int cin = g.Cin[l], cout = g.C[l];
-  const double* __restrict__ K = Pg + g.koff[l];
+  const cdbl* K = as_const(Pg + g.koff[l]);
   const int64_t bo = g.boff[l];
+  const cdbl* B = as_const(Pg + (bo >= 0 ? bo : 0));
 #pragma unroll
-  for (int c = 0; c < CM; ++c) z[c] = (c < cout && bo >= 0) ? Pg[bo + c] : 0.0;
+  for (int c = 0; c < CM; ++c) z[c] = (c < cout && bo >= 0) ? B[c] : 0.0;
   for (int k = 0; k < g.FF; ++k) {
-    const double* x = X + (int)nbr[p * g.FF + k] * cin;
-    const double* __restrict__ w = K + k * cin * cout;
-    for (int ci = 0; ci < cin; ++ci) {
-      const double xv = x[ci];
+    const int q = nbr[p * g.FF + k];
+    const cdbl* w = K + k * cin * cout;
+#pragma unroll JV_CH_UNROLL
+    for (int ci = 0; ci < CM; ++ci) {
+      if (ci < cin) {
+        const double xv = X[ci * g.n + q];
 #pragma unroll
-      for (int c = 0; c < CM; ++c)
-        if (c < cout) z[c] = fma(xv, w[ci * cout + c], z[c]);
+        for (int c = 0; c < CM; ++c)
+          if (c < cout) z[c] = fma(xv, w[ci * cout + c], z[c]);
+      }
     }
   }
 }
 
-// Forward pass, position-major.  A (and D = tanh(z) when non-null) are
-// [pos][channel] per layer as before.
+// Forward pass, position-parallel, channel-planar activations: A + aoff[l]
+// holds layer l as [c][pos]; D (tanh z) likewise when non-null.
 template <int CM, typename Sync>
 __device__ inline void cnn_forward_pos(const CnnGeom& g, const double* __restrict__ Pg, const double* x0, double* A,
                                        double* D, const short* nbr, int tid, int nt, Sync sync) {
@@ -242,8 +258,8 @@ __device__ inline void cnn_forward_pos(const CnnGeom& g, const double* __restric
 #pragma unroll
       for (int c = 0; c < CM; ++c)
         if (c < cout) {
-          Al[p * cout + c] = log_cosh(z[c]);
-          if (Dl) Dl[p * cout + c] = tanh(z[c]);
+          Al[c * g.n + p] = log_cosh(z[c]);
+          if (Dl) Dl[c * g.n + p] = tanh(z[c]);
         }
     }
     sync();
@@ -257,44 +273,58 @@ __device__ inline void cnn_forward_pos(const CnnGeom& g, const double* __restric
 __device__ __host__ inline int win_ex(const CnnGeom& g, int l) { return l < 0 ? 1 : 1 + (l + 1) * (g.Fx - 1); }
 __device__ __host__ inline int win_ey(const CnnGeom& g, int l) { return l < 0 ? 1 : 1 + (l + 1) * (g.Fy - 1); }
 __device__ __host__ inline bool win_ok(const CnnGeom& g) {
-  return win_ex(g, g.nl - 1) <= g.Lx && win_ey(g, g.nl - 1) <= g.Ly;
+  // the window kernels unroll filters up to 3x3
+  return g.Fx <= 3 && g.Fy <= 3 && win_ex(g, g.nl - 1) <= g.Lx && win_ey(g, g.nl - 1) <= g.Ly;
 }
 
 // Pre-activations of window element (a, b) of layer l after flipping site
-// (si, sj).  Inputs inside the layer-(l-1) window come from Wprev (window-major,
-// [a*ey + b][cin]); for l == 0 the only changed input is xflip at the site.
+// (si, sj).  Inputs inside the layer-(l-1) window come from Wprev (planar,
+// [ci][a*ey + b]); for l == 0 the only changed input is xflip at the site.
 template <int CM>
 __device__ inline void cnn_preact_win(const CnnGeom& g, const double* __restrict__ Pg, int l, const double* Xold,
                                       const double* Wprev, double xflip, int si, int sj, int a, int b, double* z) {
   const int cin = g.Cin[l], cout = g.C[l];
-  const double* __restrict__ K = Pg + g.koff[l];
+  const cdbl* K = as_const(Pg + g.koff[l]);
   const int64_t bo = g.boff[l];
+  const cdbl* B = as_const(Pg + (bo >= 0 ? bo : 0));
   const int pex = win_ex(g, l - 1), pey = win_ey(g, l - 1);
 #pragma unroll
-  for (int c = 0; c < CM; ++c) z[c] = (c < cout && bo >= 0) ? Pg[bo + c] : 0.0;
-  for (int ky = 0; ky < g.Fx; ++ky) {
+  for (int c = 0; c < CM; ++c) z[c] = (c < cout && bo >= 0) ? B[c] : 0.0;
+#ifndef JV_FILTER_UNROLL
+#define JV_FILTER_UNROLL 1
+#endif
+#pragma unroll JV_FILTER_UNROLL
+  for (int ky = 0; ky < 3; ++ky) {
+    if (ky >= g.Fx) break;
     const int aq = a - ky;
     int qi = si - aq;
     qi = qi < 0 ? qi + g.Lx : (qi >= g.Lx ? qi - g.Lx : qi);
-    for (int kx = 0; kx < g.Fy; ++kx) {
+#pragma unroll JV_FILTER_UNROLL
+    for (int kx = 0; kx < 3; ++kx) {
+      if (kx >= g.Fy) break;
       const int bq = b - kx;
       int qj = sj - bq;
       qj = qj < 0 ? qj + g.Ly : (qj >= g.Ly ? qj - g.Ly : qj);
       const bool in = aq >= 0 && aq < pex && bq >= 0 && bq < pey;
       const int k = ky * g.Fy + kx;
-      const double* __restrict__ w = K + k * cin * cout;
+      const cdbl* w = K + k * cin * cout;
       if (l == 0) {
         const double xv = in ? xflip : Xold[qi * g.Ly + qj];
 #pragma unroll
         for (int c = 0; c < CM; ++c)
           if (c < cout) z[c] = fma(xv, w[c], z[c]);
       } else {
-        const double* x = in ? Wprev + (aq * pey + bq) * cin : Xold + (qi * g.Ly + qj) * cin;
-        for (int ci = 0; ci < cin; ++ci) {
-          const double xv = x[ci];
+        // planar: base [ci][pos] (stride n), window [ci][widx] (stride pex*pey)
+        const double* x = in ? Wprev + (aq * pey + bq) : Xold + (qi * g.Ly + qj);
+        const int xs = in ? pex * pey : g.n;
+#pragma unroll JV_CH_UNROLL
+        for (int ci = 0; ci < CM; ++ci) {
+          if (ci < cin) {
+            const double xv = x[ci * xs];
 #pragma unroll
-          for (int c = 0; c < CM; ++c)
-            if (c < cout) z[c] = fma(xv, w[ci * cout + c], z[c]);
+            for (int c = 0; c < CM; ++c)
+              if (c < cout) z[c] = fma(xv, w[ci * cout + c], z[c]);
+          }
         }
       }
     }

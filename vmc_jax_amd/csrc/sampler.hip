@@ -241,14 +241,14 @@ __global__ void __launch_bounds__(256) mc_cnn_win(CnnGeom g, const double* __res
       for (int j = lane; j < wsz; j += WAVE) {
         const int aa = j / ey, bb = j % ey;
         cnn_preact_win<CM>(g, Pg, l, Xold, l == 0 ? nullptr : W[l - 1], xflip, si, sj, aa, bb, z);
-        double* Wo = W[l] + j * cout;
-        const double* Ab = A + g.aoff[l] + win_pos(g, si, sj, aa, bb) * cout;
+        double* Wo = W[l] + j;  // planar [c][j]
+        const double* Ab = A + g.aoff[l] + win_pos(g, si, sj, aa, bb);
 #pragma unroll
         for (int c = 0; c < CM; ++c)
           if (c < cout) {
             const double v = log_cosh(z[c]);
-            Wo[c] = v;
-            if (l == L1) diff += v - Ab[c];
+            Wo[c * wsz] = v;
+            if (l == L1) diff += v - Ab[c * g.n];
           }
       }
       wave_sync();
@@ -259,8 +259,8 @@ __global__ void __launch_bounds__(256) mc_cnn_win(CnnGeom g, const double* __res
       for (int l = 0; l < g.nl; ++l) {
         const int ey = win_ey(g, l), wsz = win_ex(g, l) * ey, cout = g.C[l];
         for (int t = lane; t < wsz * cout; t += WAVE) {
-          const int j = t / cout, c = t % cout;
-          A[g.aoff[l] + win_pos(g, si, sj, j / ey, j % ey) * cout + c] = W[l][t];
+          const int c = t / wsz, j = t % wsz;
+          A[g.aoff[l] + c * g.n + win_pos(g, si, sj, j / ey, j % ey)] = W[l][t];
         }
       }
       if (lane == 0) {

@@ -1,0 +1,203 @@
+"""Continuous-batching request loop for many independent VMC jobs on one GPU.
+
+A job is one variational state with its Hamiltonian, sampler, TDVP / MinSR
+equation and stepper, plus a step budget.  The server
+
+* admits queued jobs whenever a slot is free and the job's HBM footprint fits
+  the remaining budget (jobs that can never fit are refused),
+* runs every admitted job on its own HIP stream (default priority) from its own
+  host thread, so the kernels of different jobs overlap on the device and a
+  job's latency-bound phases (the FP64 eigendecomposition, host reads of the
+  solver status) hide behind other jobs' work,
+* keeps each job's parameters, walkers and Philox counters on the device
+  between steps,
+* records the wall-clock latency of every step and reports p50 / p99 per-step
+  latency with the aggregate steps/s,
+* isolates failures: a job whose step raises (or produces a non-finite energy)
+  is marked failed and its slot and memory are released; other jobs continue.
+
+Results do not depend on the mix: every job draws from its own Philox stream,
+all reductions have a fixed order, and the library keeps solver workspaces per
+host thread, so a job's trajectory is bit-identical alone and in the mix.
+
+jVMC has no serving layer; the per-step work is the reference's
+``stepper.step(t, tdvp, params, hamiltonian=..., psi=...)`` (util.py:188-212).
+"""
+import collections
+import heapq
+import math
+import threading
+import time
+
+import numpy as np
+import torch
+
+
+class Job:
+    """One VMC job.  ``make`` builds (psi, hamiltonian, equation, stepper) on
+    the device when the job is admitted; ``footprint`` is its HBM estimate in
+    bytes."""
+
+    def __init__(self, job_id, make, num_steps, footprint, arrival=0.0, meta=None):
+        self.job_id = job_id
+        self.make = make
+        self.num_steps = int(num_steps)
+        self.footprint = int(footprint)
+        self.arrival = float(arrival)
+        self.meta = meta or {}
+        self.state = "queued"
+        self.error = None
+        self.steps_done = 0
+        self.latencies = []
+        self.energies = []
+        self.t = 0.0
+        self.admitted_at = None
+        self.finished_at = None
+        self.params = None
+
+    def run(self, stream, stop_event):
+        """Body of the job's worker thread."""
+        try:
+            ctx = torch.cuda.stream(stream) if stream is not None else _NullCtx()
+            with ctx:
+                psi, H, eq, stepper = self.make()
+                while self.steps_done < self.num_steps and not stop_event.is_set():
+                    t0 = time.perf_counter()
+                    dp, dt = stepper.step(self.t, eq, psi.get_parameters(), hamiltonian=H, psi=psi,
+                                          numSamples=None)
+                    psi.set_parameters(dp)
+                    e = eq.get_energy_mean()
+                    if stream is not None:
+                        stream.synchronize()
+                    self.latencies.append(time.perf_counter() - t0)
+                    if not math.isfinite(e):
+                        raise FloatingPointError("non-finite energy at step %d" % self.steps_done)
+                    self.energies.append(e)
+                    self.t += dt
+                    self.steps_done += 1
+                p = psi.get_parameters()
+                self.params = p.cpu() if torch.is_tensor(p) else p
+            self.state = "done"
+        except Exception as exc:  # isolate the failure to this job
+            self.state = "failed"
+            self.error = repr(exc)
+        finally:
+            self.finished_at = time.perf_counter()
+
+
+class _NullCtx:
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        return False
+
+
+class Server:
+    """Admission and scheduling of ``Job`` objects on one GPU.
+    ``stream_factory`` makes the per-slot streams (``torch.cuda.Stream`` by
+    default; host-only tests pass ``lambda: None``)."""
+
+    def __init__(self, max_slots=8, hbm_budget=None, stream_factory=None):
+        self.max_slots = int(max_slots)
+        self.stream_factory = stream_factory or torch.cuda.Stream
+        if hbm_budget is None:
+            free, total = torch.cuda.mem_get_info()
+            hbm_budget = int(0.8 * free)
+        self.hbm_budget = int(hbm_budget)
+        self.queue = []          # heap of (arrival, seq, job)
+        self._seq = 0
+        self.active = {}         # job_id -> (job, thread, stream)
+        self.finished = []
+        self.refused = []
+        self.used = 0
+        self._stop = threading.Event()
+        self._streams = collections.deque()
+
+    def submit(self, job):
+        if job.footprint > self.hbm_budget:
+            job.state = "refused"
+            job.error = "footprint %d B exceeds the HBM budget %d B" % (job.footprint, self.hbm_budget)
+            self.refused.append(job)
+            return False
+        heapq.heappush(self.queue, (job.arrival, self._seq, job))
+        self._seq += 1
+        return True
+
+    def _stream(self):
+        return self._streams.popleft() if self._streams else self.stream_factory()
+
+    def _admit(self, now):
+        admitted = []
+        while self.queue and len(self.active) < self.max_slots:
+            arrival, _, job = self.queue[0]
+            if arrival > now or job.footprint > self.hbm_budget - self.used:
+                break  # FIFO by arrival: the head waits for memory
+            heapq.heappop(self.queue)
+            st = self._stream()
+            job.state = "running"
+            job.admitted_at = time.perf_counter()
+            th = threading.Thread(target=job.run, args=(st, self._stop), daemon=True)
+            self.active[job.job_id] = (job, th, st)
+            self.used += job.footprint
+            th.start()
+            admitted.append(job)
+        return admitted
+
+    def _reap(self):
+        for jid in list(self.active):
+            job, th, st = self.active[jid]
+            if not th.is_alive():
+                th.join()
+                self.used -= job.footprint
+                self._streams.append(st)
+                del self.active[jid]
+                self.finished.append(job)
+
+    def run(self, poll=0.002, time_scale=1.0):
+        """Runs until every submitted job has finished.  Arrival times are in
+        seconds after the start of ``run`` (scaled by ``time_scale``)."""
+        t_start = time.perf_counter()
+        while self.queue or self.active:
+            now = (time.perf_counter() - t_start) / time_scale
+            self._reap()
+            self._admit(now)
+            time.sleep(poll)
+        self.wall = time.perf_counter() - t_start
+        return self.report()
+
+    def report(self):
+        lat = np.array([l for j in self.finished for l in j.latencies]) if self.finished else np.zeros(0)
+        steps = int(sum(j.steps_done for j in self.finished))
+        return {
+            "jobs_done": sum(j.state == "done" for j in self.finished),
+            "jobs_failed": sum(j.state == "failed" for j in self.finished),
+            "jobs_refused": len(self.refused),
+            "steps": steps,
+            "wall_s": getattr(self, "wall", None),
+            "steps_per_s": steps / self.wall if getattr(self, "wall", 0) else None,
+            "p50_step_ms": float(np.percentile(lat, 50) * 1e3) if lat.size else None,
+            "p99_step_ms": float(np.percentile(lat, 99) * 1e3) if lat.size else None,
+        }
+
+    def stop(self):
+        self._stop.set()
+
+
+def footprint_bytes(num_samples, num_real_params, num_sites, complex_params, solver, chains):
+    """HBM estimate of one job: O (compact and jVMC layouts for TDVP), centred
+    copies, the solver matrix and eigenvectors, SNR buffers, walkers and
+    samples."""
+    n, pr = int(num_samples), int(num_real_params)
+    pc = pr // 2 if complex_params else pr
+    b = 0
+    b += n * num_sites * 4 * 2 + chains * num_sites * 4   # configs, s' scratch, walkers
+    b += n * 16 * 6                                       # log psi, E_loc, weights, centred copies
+    if solver == "tdvp":
+        b += n * pr * 16 * 4                              # O, centred O, SNR data and projection
+        b += pr * pr * 16 * 3                             # S0, S, eigenvectors
+    else:
+        m = min(n, pc)
+        b += n * pc * (16 if complex_params else 8) * 2   # O and centred O
+        b += m * m * (16 if complex_params else 8) * 2    # Gram and eigenvectors
+    return int(1.25 * b) + (64 << 20)
