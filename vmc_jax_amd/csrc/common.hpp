@@ -98,9 +98,20 @@ __device__ inline double re_log_cosh(cd x) {
   double t = fma(2.0 * e2, cos(2.0 * b), e2 * e2);
   return a + 0.5 * log1p(t) - LN2;
 }
-__device__ inline double log_cosh(double x) {
-  double a = fabs(x);
-  return a + log1p(exp(-2.0 * a)) - LN2;
+// Real log cosh with the same formula, |x| + log1p(t) - log 2, t = exp(-2|x|)
+// in (0, 1].  log1p(t) = 2 atanh(s), s = t / (2 + t) in [0, 1/3], summed as
+// 2 s sum_{k<=17} s^(2k) / (2k+1) (truncation < (1/9)^18): about 60 FP64
+// operations instead of the ~190 of the library's double-double log1p, with
+// ~1 ulp absolute error relative to |x|.
+__host__ __device__ inline double log_cosh(double x) {
+  const double a = fabs(x);
+  const double t = exp(-2.0 * a);
+  const double s = t / (2.0 + t);
+  const double z = s * s;
+  double p = 1.0 / 35.0;
+#pragma unroll
+  for (int k = 16; k >= 0; --k) p = fma(p, z, 1.0 / (double)(2 * k + 1));
+  return a + 2.0 * s * p - LN2;
 }
 // tanh = d log_cosh / dx, via exp(-2|Re x|) so it never overflows.
 __device__ inline cd ctanh(cd x) {

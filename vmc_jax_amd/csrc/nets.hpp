@@ -199,10 +199,6 @@ __device__ inline double cnn_local_delta(const CnnGeom& g, const double* P, cons
 
 }  // namespace jv
 
-#ifndef JV_CH_UNROLL
-#define JV_CH_UNROLL 8
-#endif
-
 namespace jv {
 
 // Weights are read through the constant address space: their indices are
@@ -230,7 +226,7 @@ __device__ inline void cnn_preact_all(const CnnGeom& g, const double* __restrict
   for (int k = 0; k < g.FF; ++k) {
     const int q = nbr[p * g.FF + k];
     const cdbl* w = K + k * cin * cout;
-#pragma unroll JV_CH_UNROLL
+#pragma unroll
     for (int ci = 0; ci < CM; ++ci) {
       if (ci < cin) {
         const double xv = X[ci * g.n + q];
@@ -290,18 +286,11 @@ __device__ inline void cnn_preact_win(const CnnGeom& g, const double* __restrict
   const int pex = win_ex(g, l - 1), pey = win_ey(g, l - 1);
 #pragma unroll
   for (int c = 0; c < CM; ++c) z[c] = (c < cout && bo >= 0) ? B[c] : 0.0;
-#ifndef JV_FILTER_UNROLL
-#define JV_FILTER_UNROLL 1
-#endif
-#pragma unroll JV_FILTER_UNROLL
-  for (int ky = 0; ky < 3; ++ky) {
-    if (ky >= g.Fx) break;
+  for (int ky = 0; ky < g.Fx; ++ky) {
     const int aq = a - ky;
     int qi = si - aq;
     qi = qi < 0 ? qi + g.Lx : (qi >= g.Lx ? qi - g.Lx : qi);
-#pragma unroll JV_FILTER_UNROLL
-    for (int kx = 0; kx < 3; ++kx) {
-      if (kx >= g.Fy) break;
+    for (int kx = 0; kx < g.Fy; ++kx) {
       const int bq = b - kx;
       int qj = sj - bq;
       qj = qj < 0 ? qj + g.Ly : (qj >= g.Ly ? qj - g.Ly : qj);
@@ -317,7 +306,7 @@ __device__ inline void cnn_preact_win(const CnnGeom& g, const double* __restrict
         // planar: base [ci][pos] (stride n), window [ci][widx] (stride pex*pey)
         const double* x = in ? Wprev + (aq * pey + bq) : Xold + (qi * g.Ly + qj);
         const int xs = in ? pex * pey : g.n;
-#pragma unroll JV_CH_UNROLL
+#pragma unroll
         for (int ci = 0; ci < CM; ++ci) {
           if (ci < cin) {
             const double xv = x[ci * xs];

@@ -82,6 +82,13 @@ class TDVP:
     def get_S(self):
         return self.S
 
+    def _num_samples(self):
+        """mpi.globNumSamples as the SNR uses it (tdvp.py:181).  The sampler's own
+        copy is read so that jobs running concurrently in one process do not
+        see each other's sample counts."""
+        n = getattr(self.sampler, "globNumSamples", None)
+        return mpi.globNumSamples if n is None else n
+
     def solve(self, Eloc, gradients):
         """tdvp.py:183-213 on centred data (SampledObs of E_loc and O)."""
         D = gradients._data
@@ -101,7 +108,7 @@ class TDVP:
         info = _lib.JvSolveInfo()
         _lib.call("jv_tdvp_solve", _lib.ptr(D), _lib.ptr(e), _lib.ptr(w), n, pr, x, int(self._imag),
                   float(self.diagonalShift), int(not isinstance(self.sampler, ExactSampler)),
-                  float(mpi.globNumSamples), float(self.snrTol), float(self.pinvTol), float(self.pinvCutoff),
+                  float(self._num_samples()), float(self.snrTol), float(self.pinvTol), float(self.pinvCutoff),
                   _lib.ptr(update), _lib.ptr(S), _lib.ptr(F), _lib.ptr(ev), _lib.ptr(snr), ctypes.byref(info),
                   _lib.stream_ptr())
         self.S, self.F, self.ev, self.snr = S, F, ev, snr
