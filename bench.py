@@ -173,6 +173,78 @@ def roofline_c3(kt, steps, mode):
             "launches": launches}
 
 
+def serve_mix(num_jobs=16, slots=8, seed=7, rate=8.0):
+    """Served workload, scaled down from the 1000-request mix: Poisson arrivals
+    (rate jobs/s); 70% 1D jobs (TFIM or XXZ with Delta ~ U[0, 2], L ~ U{16, 20,
+    24}, complex RBM alpha=2; half ground-state MinSR, half real-time TDVP with
+    Heun) and 30% 2D TFIM jobs (side ~ U{6..10}, h ~ U[2, 4], the 3x(8ch,3x3)
+    CNN, MinSR); 3-8 steps and 2048-4096 samples per job.  Jobs are admitted
+    by HBM footprint into `slots` concurrent slots, each on its own stream."""
+    import torch
+    import vmc_jax_amd as jv
+    from vmc_jax_amd import serving
+    rng = np.random.default_rng(seed)
+    t, jobs = 0.0, []
+    for i in range(num_jobs):
+        t += rng.exponential(1.0 / rate)
+        steps = int(rng.integers(3, 9))
+        ns = int(rng.choice([2048, 4096]))
+        jseed = int(rng.integers(1 << 30))
+        if rng.uniform() < 0.7:
+            L = int(rng.choice([16, 20, 24]))
+            xxz = rng.uniform() < 0.5
+            delta = float(rng.uniform(0, 2))
+            ground = rng.uniform() < 0.5
+
+            def make(L=L, xxz=xxz, delta=delta, ground=ground, ns=ns, jseed=jseed):
+                psi = jv.NQS(jv.nets.CpxRBM(numHidden=2 * L, bias=True), seed=jseed)
+                psi(torch.zeros((1, 1, L), dtype=torch.int32))
+                op = jv.operator
+                H = op.BranchFreeOperator()
+                for l in range(L):
+                    r = (l + 1) % L
+                    if xxz:
+                        H.add(op.scal_opstr(1.0, (op.Sx(l), op.Sx(r))))
+                        H.add(op.scal_opstr(1.0, (op.Sy(l), op.Sy(r))))
+                        H.add(op.scal_opstr(delta, (op.Sz(l), op.Sz(r))))
+                    else:
+                        H.add(op.scal_opstr(-1.0, (op.Sz(l), op.Sz(r))))
+                        H.add(op.scal_opstr(-1.0, (op.Sx(l),)))
+                ms = jv.sampler.MCSampler(psi, (L,), jseed, updateProposer=jv.sampler.propose_spin_flip,
+                                          numChains=256, numSamples=ns)
+                if ground:
+                    return psi, H, jv.util.MinSR(ms, pinvTol=1e-8, solver="auto"), jv.util.stepper.Euler(1e-2)
+                return psi, H, jv.util.TDVP(ms, rhsPrefactor=1.j, makeReal='imag'), jv.util.stepper.Heun(1e-3)
+            pr = 2 * (2 * L * L + 2 * L)
+            fp = serving.footprint_bytes(ns, pr, L, True, "minsr" if ground else "tdvp", 256)
+            kind = "1d-%s-%s-L%d" % ("xxz" if xxz else "tfim", "gs" if ground else "rt", L)
+        else:
+            side = int(rng.integers(6, 11))
+            h = float(rng.uniform(2, 4))
+
+            def make(side=side, h=h, ns=ns, jseed=jseed):
+                net = jv.nets.CNN(F=(3, 3), channels=(8, 8, 8), strides=(1, 1))
+                psi = jv.NQS(net, seed=jseed)
+                psi(torch.zeros((1, 1, side, side), dtype=torch.int32))
+                psi.set_parameters(net.he_normal(psi.desc, np.random.default_rng(jseed)))
+                H = tfim_strings(jv.operator, (side, side), -1.0, -h)
+                ms = jv.sampler.MCSampler(psi, (side, side), jseed, updateProposer=jv.sampler.propose_spin_flip,
+                                          numChains=256, numSamples=ns)
+                return psi, H, jv.util.MinSR(ms, pinvTol=1e-10, solver="auto"), jv.util.stepper.Euler(5e-3)
+            fp = serving.footprint_bytes(ns, 1240, side * side, False, "minsr", 256)
+            kind = "2d-cnn-%d" % side
+        jobs.append(serving.Job(i, make, steps, fp, arrival=t, meta={"kind": kind}))
+    srv = serving.Server(max_slots=slots)
+    for j in jobs:
+        srv.submit(j)
+    rep = srv.run()
+    rep["jobs"] = num_jobs
+    rep["slots"] = slots
+    rep["failures"] = [j.error for j in srv.finished if j.state == "failed"]
+    rep["workload"] = serve_mix.__doc__.split("\n")[0].strip()
+    return rep
+
+
 def cpu_baseline(n_sub=1024, chains=512):
     """FP64 oracle timed on a bounded sample of the C3 step: sampling (full
     network re-evaluation per proposal), local energies with materialised s',
@@ -225,6 +297,7 @@ def main():
     ap.add_argument("--mode", default="dual", choices=["T", "dual"], help="MinSR solve form (c3)")
     ap.add_argument("--t-steps", type=int, default=3, help="c3: steps of the other MinSR form reported beside")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serve-jobs", type=int, default=16, help="c3: jobs of the served mix (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (every box has one MI355X); "
@@ -260,6 +333,11 @@ def main():
                 "solver_form": om, "value": args.t_steps / e2, "unit": "steps/s",
                 "ms_per_step": 1000.0 * e2 / args.t_steps,
                 "kernel_ms_per_step": {k: v[0] / args.t_steps for k, v in kt2.items() if v[1] > 0}}
+        if args.serve_jobs > 0:
+            try:
+                out["served"] = serve_mix(args.serve_jobs)
+            except Exception as e:
+                out["served"] = {"error": repr(e)}
         if not args.no_cpu_baseline:
             try:
                 out["cpu_baseline"] = cpu_baseline()
