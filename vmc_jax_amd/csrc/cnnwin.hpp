@@ -1,0 +1,172 @@
+// Receptive-field window machinery of the CNN local-energy and Metropolis
+// kernels (operators / proposals that change one site per string or step).
+//
+// Flipping site s = (si, sj) changes only the outputs of layer l inside the
+// window ((si - a) mod Lx, (sj - b) mod Ly), 0 <= a < ex(l), 0 <= b < ey(l),
+// ex(l) = 1 + (l+1)(Fx-1) (window index a*ey + b).  The kernels recompute those
+// outputs from the stored activations of the unflipped configuration and sum
+// the change of the last layer: the same log psi(s') as a full forward pass of
+// the reference network (cnn.py:160-171) with a fraction of the work.
+//
+// Weights are read from a packed copy with every layer padded to CM channels
+// (zeros): a padded output channel has pre-activation +0 in every
+// configuration, so its change is exactly zero, and a padded input channel
+// multiplies a zero weight, so the accumulation of the real channels is the
+// same sequence of fused multiply-adds as the unpadded network's.
+#pragma once
+#include "nets.hpp"
+
+namespace jv {
+
+// LDS view of a generic pointer that points into the workgroup's LDS, so the
+// loads below compile to ds_read instead of flat loads.
+typedef const __attribute__((address_space(3))) double ldbl;
+__device__ inline ldbl* as_lds(const double* p) { return (ldbl*)p; }
+
+// Layer l of the packed weights: kernel ((k*G + gi)*cinp + ci)*CG + cg for
+// output channel c = gi*CG + cg (CG = CM/G, cinp = 1 for the first layer and
+// CM after), then the bias [c].  G groups the output channels so that a wave
+// owning group gi reads one contiguous block per kernel offset.
+struct CnnPack {
+  int cm, groups;
+  int64_t koff[JV_MAX_LAYERS];
+  int64_t boff[JV_MAX_LAYERS];
+  int64_t total;
+};
+
+inline CnnPack make_cnn_pack(const CnnGeom& g, int cm, int groups) {
+  CnnPack p;
+  p.cm = cm;
+  p.groups = groups;
+  int64_t off = 0;
+  for (int l = 0; l < g.nl; ++l) {
+    const int cinp = l == 0 ? 1 : cm;
+    p.koff[l] = off;
+    off += (int64_t)g.FF * cinp * cm;
+    p.boff[l] = off;
+    off += cm;
+  }
+  p.total = off;
+  return p;
+}
+
+// Writes the packed weights of params (jVMC flat layout) to out[pk.total].
+__global__ void cnn_pack(CnnGeom g, CnnPack pk, const double* __restrict__ Pg, double* __restrict__ out);
+
+// Launches cnn_pack into workspace slot `slot`; returns the packed buffer or
+// nullptr (error set).
+double* pack_cnn_weights(const CnnGeom& g, const CnnPack& pk, const double* params, hipStream_t st, int slot);
+
+// Largest channel count rounded up to the kernels' instantiations (0: none).
+int cnn_cmax(const CnnGeom& g);
+
+// Activations of one configuration with the packed weights (G channel
+// groups), channel-planar:
+// layer l occupies A + l*lstride as CM planes of cstride doubles (positions
+// [0, n)).  All CM planes are written (padded channels hold log_cosh(0));
+// callers sum only the real channels of the last layer.
+template <int CM, int G, typename Sync>
+__device__ inline void win_forward(const CnnGeom& g, const cdbl* __restrict__ Wp, const CnnPack& pk,
+                                   const double* x0, double* A, int lstride, int cstride, const short* nbr,
+                                   int tid, int nt, Sync sync) {
+  for (int l = 0; l < g.nl; ++l) {
+    const cdbl* K = Wp + pk.koff[l];
+    const cdbl* B = Wp + pk.boff[l];
+    ldbl* X = as_lds(l == 0 ? x0 : A + (l - 1) * lstride);
+    ldbl* X0 = as_lds(x0);
+    double* Al = A + l * lstride;
+    for (int p = tid; p < g.n; p += nt) {
+      double z[CM];
+#pragma unroll
+      for (int c = 0; c < CM; ++c) z[c] = B[c];
+      for (int k = 0; k < g.FF; ++k) {
+        const int q = nbr[p * g.FF + k];
+        constexpr int CG = CM / G;
+        if (l == 0) {
+          const double xv = X0[q];
+          const cdbl* w = K + k * CM;
+#pragma unroll
+          for (int c = 0; c < CM; ++c) z[c] = fma(xv, w[(c / CG) * CG + c % CG], z[c]);
+        } else {
+          const cdbl* w = K + k * CM * CM;
+#pragma unroll
+          for (int ci = 0; ci < CM; ++ci) {
+            const double xv = X[ci * cstride + q];
+#pragma unroll
+            for (int c = 0; c < CM; ++c) z[c] = fma(xv, w[((c / CG) * CM + ci) * CG + c % CG], z[c]);
+          }
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < CM; ++c) Al[c * cstride + p] = log_cosh(z[c]);
+    }
+    sync();
+  }
+}
+
+// Position index of window element (a, b) of a flip at (si, sj).
+__device__ inline int win_site(const CnnGeom& g, int si, int sj, int a, int b) {
+  int pi = si - a;
+  pi = pi < 0 ? pi + g.Lx : pi;
+  int pj = sj - b;
+  pj = pj < 0 ? pj + g.Ly : pj;
+  return pi * g.Ly + pj;
+}
+
+// Pre-activations (all CM channels, packed weights with G = 1) of window
+// element (a, b) of layer l >= 1 after flipping (si, sj).  Inputs inside the
+// layer-(l-1) window come from Xw ([ci][pwsz] planar), the others from the
+// stored activations Xb ([ci][n] planar).
+template <int CM>
+__device__ inline void win_preact(const CnnGeom& g, const cdbl* __restrict__ K, const cdbl* __restrict__ B,
+                                  const double* Xb, const double* Xw, int pex, int pey, int si, int sj, int a,
+                                  int b, double* z) {
+  const int pwsz = pex * pey;
+#pragma unroll
+  for (int c = 0; c < CM; ++c) z[c] = B[c];
+  for (int ky = 0; ky < g.Fx; ++ky) {
+    const int aq = a - ky;
+    int qi = si - aq;
+    qi = qi < 0 ? qi + g.Lx : (qi >= g.Lx ? qi - g.Lx : qi);
+    const bool rin = aq >= 0 && aq < pex;
+    for (int kx = 0; kx < g.Fy; ++kx) {
+      const int bq = b - kx;
+      int qj = sj - bq;
+      qj = qj < 0 ? qj + g.Ly : (qj >= g.Ly ? qj - g.Ly : qj);
+      const bool in = rin && bq >= 0 && bq < pey;
+      ldbl* x = as_lds(in ? Xw + (aq * pey + bq) : Xb + (qi * g.Ly + qj));
+      const int xs = in ? pwsz : g.n;
+      const cdbl* w = K + (ky * g.Fy + kx) * CM * CM;
+#pragma unroll
+      for (int ci = 0; ci < CM; ++ci) {
+        const double xv = x[ci * xs];
+#pragma unroll
+        for (int c = 0; c < CM; ++c) z[c] = fma(xv, w[ci * CM + c], z[c]);
+      }
+    }
+  }
+}
+
+// Layer-0 version: one input channel, the flipped site reads xflip.
+template <int CM>
+__device__ inline void win_preact0(const CnnGeom& g, const cdbl* __restrict__ K, const cdbl* __restrict__ B,
+                                   const double* x0, double xflip, int site, int si, int sj, int a, int b,
+                                   double* z) {
+#pragma unroll
+  for (int c = 0; c < CM; ++c) z[c] = B[c];
+  for (int ky = 0; ky < g.Fx; ++ky) {
+    int qi = si - a + ky;
+    qi = qi < 0 ? qi + g.Lx : (qi >= g.Lx ? qi - g.Lx : qi);
+    for (int kx = 0; kx < g.Fy; ++kx) {
+      int qj = sj - b + kx;
+      qj = qj < 0 ? qj + g.Ly : (qj >= g.Ly ? qj - g.Ly : qj);
+      const int q = qi * g.Ly + qj;
+      const double xv = q == site ? xflip : as_lds(x0)[q];
+      const cdbl* w = K + (ky * g.Fy + kx) * CM;
+#pragma unroll
+      for (int c = 0; c < CM; ++c) z[c] = fma(xv, w[c], z[c]);
+    }
+  }
+}
+
+}  // namespace jv

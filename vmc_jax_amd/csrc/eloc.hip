@@ -3,7 +3,7 @@
 // incremental (RBM) or receptive-field-limited (CNN) re-evaluation.
 #include <algorithm>
 
-#include "nets.hpp"
+#include "cnnwin.hpp"
 
 namespace jv {
 
@@ -284,62 +284,135 @@ __global__ void __launch_bounds__(512) eloc_cnn(CnnGeom g, const double* __restr
   }
 }
 
-// ------------------------------------------------------------------ CNN O_loc, single-flip windows
-// One workgroup per sample.  After a position-major forward pass of s into LDS,
-// every string is classified (m, changed sites); strings that change one site
-// are processed in chunks of `fch` flips, layer by layer: all threads compute
-// window outputs of all flips of the chunk (one output position, all channels
-// per thread), so each layer costs one barrier for the whole chunk.  The last
-// layer's output changes are reduced per flip in a fixed order, and the
-// sample's O_loc is summed over strings in string order.
-struct EloWinLayout {
-  size_t bytes;
-  int fch;
-};
+// Applies string o to s (branch_free.py:456-478, factors in stored order) in
+// registers and classifies it: code = -1 when no site changes, -2 when several
+// change, else 2*site + new value.
+__device__ inline cd classify_string(const OpDev& op, int o, const int32_t* s, int& code) {
+  cd m = {op.pref[2 * o], op.pref[2 * o + 1]};
+  int ts[MAXLEN], tv[MAXLEN];
+  int nt = 0;
+#pragma unroll
+  for (int k = 0; k < MAXLEN; ++k) {
+    ts[k] = -1;
+    tv[k] = 0;
+  }
+#pragma unroll
+  for (int k = 0; k < MAXLEN; ++k) {
+    if (k < op.max_len) {
+      const int base = o * op.max_len + k;
+      // indices are clamped like JAX gathers clamp (site to [0, L), state to [0, 1])
+      const int i = min(max(op.idx[base], 0), op.L - 1);
+      int slot = -1;
+#pragma unroll
+      for (int t = 0; t < MAXLEN; ++t)
+        if (t < nt && ts[t] == i) slot = t;
+      int cur = s[i];
+#pragma unroll
+      for (int t = 0; t < MAXLEN; ++t)
+        if (t == slot) cur = tv[t];
+      cur = min(max(cur, 0), 1);
+      const cd f = {op.matel[2 * (2 * base + cur)], op.matel[2 * (2 * base + cur) + 1]};
+      m = m * f;
+      const int nv = min(max(op.map[2 * base + cur], 0), 1);
+#pragma unroll
+      for (int t = 0; t < MAXLEN; ++t) {
+        if (slot >= 0 && t == slot) tv[t] = nv;
+        if (slot < 0 && t == nt) {
+          ts[t] = i;
+          tv[t] = nv;
+        }
+      }
+      if (slot < 0) ++nt;
+    }
+  }
+  int nch = 0, first = -1;
+#pragma unroll
+  for (int t = 0; t < MAXLEN; ++t)
+    if (t < nt && tv[t] != s[ts[t]]) {
+      if (nch == 0) first = 2 * ts[t] + tv[t];
+      ++nch;
+    }
+  code = nch == 0 ? -1 : (nch == 1 ? first : -2);
+  return m;
+}
 
-__host__ __device__ inline size_t elo_win_bytes(const CnnGeom& g, int nops, int fch) {
-  size_t d = (size_t)g.n + g.atot + 8 + 4 * (size_t)nops;  // x0, A, red, m + contrib (complex)
-  for (int l = 0; l < g.nl - 1; ++l) d += (size_t)fch * win_ex(g, l) * win_ey(g, l) * g.C[l];
-  d += (size_t)fch * win_ex(g, g.nl - 1) * win_ey(g, g.nl - 1);  // last-layer partial sums
-  size_t i32 = (size_t)g.n + nops + fch;                          // cs, code, flip list
-  size_t sh = (size_t)g.n * g.FF;                                 // nbr
-  return 8 * d + 4 * i32 + 2 * sh + 64;
+// ------------------------------------------------------------------ CNN O_loc, single-flip windows
+// One workgroup per sample (grid-stride).  A forward pass of s with the packed
+// weights leaves every layer's activations in LDS; every string is classified
+// in registers (m, changed sites); strings changing one site are compacted in
+// string order and processed in chunks of `fch` flips, layer by layer: each
+// thread computes all CM channels of one window element of one flip, so a
+// layer costs one barrier per chunk.  Each flip's last-layer change is reduced
+// by one wave; the sample's O_loc is a block reduction over strings.
+__host__ __device__ inline size_t elo_win_bytes(const CnnGeom& g, int cm, int nops, int fch) {
+  size_t d = (size_t)g.n + (size_t)g.nl * cm * g.n + 4 * (size_t)nops + 16;
+  for (int l = 0; l < g.nl - 1; ++l) d += (size_t)fch * cm * win_ex(g, l) * win_ey(g, l);
+  d += (size_t)fch * win_ex(g, g.nl - 1) * win_ey(g, g.nl - 1);
+  const size_t i32 = (size_t)g.n + 2 * (size_t)nops + 16;
+  const size_t sh = (size_t)g.n * g.FF;
+  return 8 * d + 4 * i32 + 2 * sh;
+}
+
+// Flips per chunk: the fewest estimated VALU cycles per flip among the sizes
+// whose LDS fits two workgroups per CU (at least one flip).
+inline int elo_win_chunk(const CnnGeom& g, int cm, int nops, int threads) {
+  int best = 1;
+  double best_cost = 1e300;
+  for (int f = 1; f <= 64; ++f) {
+    if (f > 1 && elo_win_bytes(g, cm, nops, f) > 80 * 1024) break;
+    double cost = 0.0;
+    for (int l = 0; l < g.nl; ++l) {
+      const int wsz = win_ex(g, l) * win_ey(g, l);
+      const double item = (double)g.FF * (l == 0 ? 1 : cm) * cm + 45.0 * cm;
+      cost += (double)((f * wsz + threads - 1) / threads) * item;
+    }
+    cost /= f;
+    if (cost < best_cost - 1e-9) {
+      best_cost = cost;
+      best = f;
+    }
+  }
+  return best;
 }
 
 template <int CM>
-__global__ void __launch_bounds__(256) eloc_cnn_win(CnnGeom g, const double* __restrict__ Pg, OpDev op,
+__global__ void __launch_bounds__(256) eloc_cnn_win(CnnGeom g, const double* __restrict__ Wpk, CnnPack pk, OpDev op,
                                                     const int32_t* __restrict__ cfg,
                                                     const double* __restrict__ lps, int64_t n, int fch,
                                                     double* __restrict__ out) {
   extern __shared__ double lds[];
   const int tid = threadIdx.x, nt = blockDim.x;
-  const int nops = op.num_ops;
+  const int lane = tid % WAVE, wv = tid / WAVE, nw = nt / WAVE;
+  const int nops = op.num_ops, L1 = g.nl - 1;
+  const int lsz = CM * g.n;  // doubles per stored layer
+  const int wlast = win_ex(g, L1) * win_ey(g, L1);
   double* x0 = lds;
-  double* A = x0 + g.n;
-  double* red = A + g.atot;
-  double* mstr = red + 8;            // complex [nops]
-  double* contrib = mstr + 2 * nops; // complex [nops]
+  double* A = x0 + g.n;  // layer l as [c][n] at A + l*lsz
+  double* mre = A + (size_t)g.nl * lsz;
+  double* mim = mre + nops;
+  double* cre = mim + nops;
+  double* cim = cre + nops;
+  double* red = cim + nops;
   double* W[JV_MAX_LAYERS];
-  double* wp = contrib + 2 * nops;
-  for (int l = 0; l < g.nl - 1; ++l) {
-    W[l] = wp;
-    wp += (size_t)fch * win_ex(g, l) * win_ey(g, l) * g.C[l];
+  double* wp = red + 16;
+  for (int l = 0; l < L1; ++l) {
+    W[l] = wp;  // per flip [c][wsz]
+    wp += (size_t)fch * CM * win_ex(g, l) * win_ey(g, l);
   }
-  double* part = wp;
-  wp += (size_t)fch * win_ex(g, g.nl - 1) * win_ey(g, g.nl - 1);
+  double* part = wp;  // per flip [wlast]
+  wp += (size_t)fch * wlast;
   int32_t* cs = (int32_t*)wp;
-  int32_t* code = cs + g.n;     // per string: -1 none changed, -2 several, else 2*site + new value
-  int32_t* flips = code + nops; // string index of each queued flip
-  short* nbr = (short*)(flips + fch);
+  int32_t* code = cs + g.n;
+  int32_t* flips = code + nops;
+  int32_t* wcnt = flips + nops;
+  short* nbr = (short*)(wcnt + 16);
+  const cdbl* Wp = as_const(Wpk);
   for (int t = tid; t < g.n * g.FF; t += nt) {
-    int p = t / g.FF, k = t % g.FF;
-    int i = p / g.Ly, j = p % g.Ly, ky = k / g.Fy, kx = k % g.Fy;
+    const int p = t / g.FF, k = t % g.FF;
+    const int i = p / g.Ly, j = p % g.Ly, ky = k / g.Fy, kx = k % g.Fy;
     nbr[t] = (short)(((i + ky) % g.Lx) * g.Ly + (j + kx) % g.Ly);
   }
-  __syncthreads();
   auto sync = [] { __syncthreads(); };
-  const int L1 = g.nl - 1;
-  const int wlast = win_ex(g, L1) * win_ey(g, L1);
   for (int64_t s = blockIdx.x; s < n; s += gridDim.x) {
     const int32_t* c = cfg + s * g.n;
     for (int i = tid; i < g.n; i += nt) {
@@ -347,107 +420,126 @@ __global__ void __launch_bounds__(256) eloc_cnn_win(CnnGeom g, const double* __r
       x0[i] = (double)(2 * c[i] - 1);
     }
     __syncthreads();
-    cnn_forward_pos<CM>(g, Pg, x0, A, nullptr, nbr, tid, nt, sync);
+    win_forward<CM, 1>(g, Wp, pk, x0, A, lsz, g.n, nbr, tid, nt, sync);
     double pl = 0.0;
-    for (int t = tid; t < g.n * g.C[L1]; t += nt) pl += A[g.aoff[L1] + t];
+    for (int t = tid; t < g.C[L1] * g.n; t += nt) pl += A[(size_t)L1 * lsz + t];
     const double lp = block_sum(pl, red) * g.inv_nrm;
-    const cd off = {lp - lps[2 * s], -lps[2 * s + 1]};
-    // classify strings
+    const double off_re = lp - lps[2 * s], off_im = -lps[2 * s + 1];
+    const double ph_c = cos(off_im), ph_s = sin(off_im);
     for (int o = tid; o < nops; o += nt) {
-      int sites[MAXLEN], vals[MAXLEN], nch;
-      cd m = apply_string(op, o, cs, sites, vals, nch);
-      mstr[2 * o] = m.re;
-      mstr[2 * o + 1] = m.im;
-      code[o] = nch == 0 ? -1 : (nch == 1 ? 2 * sites[0] + vals[0] : -2);
-      contrib[2 * o] = 0.0;
-      contrib[2 * o + 1] = 0.0;
+      int cd_;
+      const cd m = classify_string(op, o, cs, cd_);
+      mre[o] = m.re;
+      mim[o] = m.im;
+      code[o] = cd_;
+      cre[o] = 0.0;
+      cim[o] = 0.0;
     }
     __syncthreads();
-    if (tid == 0 && op.num_diag > 1) {  // diagonal merge (branch_free.py:483-485)
-      cd sum = {0.0, 0.0};
-      for (int d = 0; d < op.num_diag; ++d) sum = sum + cd{mstr[2 * op.diag[d]], mstr[2 * op.diag[d] + 1]};
-      for (int d = 1; d < op.num_diag; ++d) { mstr[2 * op.diag[d]] = 0.0; mstr[2 * op.diag[d] + 1] = 0.0; }
-      mstr[2 * op.diag[0]] = sum.re;
-      mstr[2 * op.diag[0] + 1] = sum.im;
-    }
-    __syncthreads();
-    // strings without a changed site contribute m * exp(off)
-    for (int o = tid; o < nops; o += nt) {
-      cd m = {mstr[2 * o], mstr[2 * o + 1]};
-      if (code[o] == -1 && nonzero(m)) {
-        cd r = m * cexp(off);
-        contrib[2 * o] = r.re;
-        contrib[2 * o + 1] = r.im;
+    if (op.num_diag > 1) {  // diagonal merge (branch_free.py:483-485)
+      double dr = 0.0, di = 0.0;
+      for (int d = tid; d < op.num_diag; d += nt) {
+        dr += mre[op.diag[d]];
+        di += mim[op.diag[d]];
       }
-    }
-    // single flips, chunk by chunk (the queue is rebuilt in string order by thread 0)
-    int next = 0;
-    while (true) {
+      dr = block_sum(dr, red);
+      di = block_sum(di, red);
+      for (int d = tid; d < op.num_diag; d += nt) {
+        mre[op.diag[d]] = d == 0 ? dr : 0.0;
+        mim[op.diag[d]] = d == 0 ? di : 0.0;
+      }
       __syncthreads();
-      if (tid == 0) {
-        int nf = 0;
-        while (next < nops && nf < fch) {
-          cd m = {mstr[2 * next], mstr[2 * next + 1]};
-          if (code[next] >= 0 && nonzero(m)) flips[nf++] = next;
-          ++next;
+    }
+    // strings changing no site contribute m exp(off); single flips are listed in string order
+    int nflip = 0;
+    for (int base = 0; base < nops; base += nt) {
+      const int o = base + tid;
+      bool flag = false;
+      if (o < nops) {
+        const double ar = mre[o], ai = mim[o];
+        const bool nz = nonzero(cd{ar, ai});
+        if (code[o] == -1 && nz) {
+          const double e = exp(off_re);
+          cre[o] = ar * (e * ph_c) - ai * (e * ph_s);
+          cim[o] = ar * (e * ph_s) + ai * (e * ph_c);
         }
-        red[0] = (double)nf;
-        red[1] = (double)next;
+        flag = code[o] >= 0 && nz;
       }
+      const uint64_t mask = __ballot(flag);
+      const int pre = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0));
+      if (lane == 0) wcnt[wv] = __popcll(mask);
       __syncthreads();
-      const int nf = (int)red[0];
-      next = (int)red[1];
-      if (nf == 0) break;
-      double z[CM];
+      int before = 0, tot = 0;
+      for (int i = 0; i < nw; ++i) {
+        before += i < wv ? wcnt[i] : 0;
+        tot += wcnt[i];
+      }
+      if (flag) flips[nflip + before + pre] = o;
+      nflip += tot;
+      __syncthreads();
+    }
+    for (int f0 = 0; f0 < nflip; f0 += fch) {
+      const int nf = min(fch, nflip - f0);
       for (int l = 0; l < g.nl; ++l) {
-        const int ex = win_ex(g, l), ey = win_ey(g, l), wsz = ex * ey, cout = g.C[l];
-        const int pwsz = win_ex(g, l - 1) * win_ey(g, l - 1);
-        const double* Xold = l == 0 ? x0 : A + g.aoff[l - 1];
+        const int ey = win_ey(g, l), wsz = win_ex(g, l) * ey;
+        const int pex = win_ex(g, l - 1), pey = win_ey(g, l - 1);
+        const cdbl* K = Wp + pk.koff[l];
+        const cdbl* B = Wp + pk.boff[l];
         for (int t = tid; t < nf * wsz; t += nt) {
-          const int f = t / wsz, j = t % wsz;
-          const int cd_ = code[flips[f]];
-          const int site = cd_ >> 1, nv = cd_ & 1;
-          const int si = site / g.Ly, sj = site % g.Ly;
-          const int a = j / ey, b = j % ey;
-          const double* Wprev = l == 0 ? nullptr : W[l - 1] + (size_t)f * pwsz * g.C[l - 1];
-          cnn_preact_win<CM>(g, Pg, l, Xold, Wprev, (double)(2 * nv - 1), si, sj, a, b, z);
+          const int f = t / wsz, j = t - f * wsz;
+          const int cv = code[flips[f0 + f]];
+          const int site = cv >> 1;
+          const int si = site / g.Ly, sj = site - si * g.Ly;
+          const int a = j / ey, b = j - a * ey;
+          double z[CM];
+          if (l == 0)
+            win_preact0<CM>(g, K, B, x0, (double)(2 * (cv & 1) - 1), site, si, sj, a, b, z);
+          else
+            win_preact<CM>(g, K, B, A + (size_t)(l - 1) * lsz, W[l - 1] + (size_t)f * CM * pex * pey, pex, pey,
+                           si, sj, a, b, z);
           if (l < L1) {
-            double* Wo = W[l] + (size_t)f * wsz * cout + j;  // planar [c][j] per flip
+            __attribute__((address_space(3))) double* Wo =
+                (__attribute__((address_space(3))) double*)(W[l] + (size_t)f * CM * wsz + j);
 #pragma unroll
-            for (int cc = 0; cc < CM; ++cc)
-              if (cc < cout) Wo[cc * wsz] = log_cosh(z[cc]);
+            for (int cc = 0; cc < CM; ++cc) Wo[cc * wsz] = log_cosh(z[cc]);
           } else {
-            const double* Ab = A + g.aoff[l] + win_pos(g, si, sj, a, b);
+            ldbl* Ab = as_lds(A + (size_t)L1 * lsz + win_site(g, si, sj, a, b));
             double acc = 0.0;
 #pragma unroll
-            for (int cc = 0; cc < CM; ++cc)
-              if (cc < cout) acc += log_cosh(z[cc]) - Ab[cc * g.n];
+            for (int cc = 0; cc < CM; ++cc) acc += log_cosh(z[cc]) - Ab[cc * g.n];
             part[f * wlast + j] = acc;
           }
         }
         __syncthreads();
       }
-      for (int f = tid; f < nf; f += nt) {
+      for (int f = wv; f < nf; f += nw) {
         double d = 0.0;
-        for (int j = 0; j < wlast; ++j) d += part[f * wlast + j];
-        const int o = flips[f];
-        cd r = cd{mstr[2 * o], mstr[2 * o + 1]} * cexp({d * g.inv_nrm + off.re, off.im});
-        contrib[2 * o] = r.re;
-        contrib[2 * o + 1] = r.im;
+        for (int j = lane; j < wlast; j += WAVE) d += part[f * wlast + j];
+        d = wave_sum(d);
+        if (lane == 0) {
+          const int o = flips[f0 + f];
+          const double e = exp(d * g.inv_nrm + off_re);
+          const double ar = mre[o], ai = mim[o];
+          cre[o] = ar * (e * ph_c) - ai * (e * ph_s);
+          cim[o] = ar * (e * ph_s) + ai * (e * ph_c);
+        }
       }
     }
     __syncthreads();
+    double tr = 0.0, ti = 0.0, bad = 0.0;
+    for (int o = tid; o < nops; o += nt) {
+      tr += cre[o];
+      ti += cim[o];
+      // a string changing several sites violates the caller's max_sites_changed bound
+      if (code[o] == -2 && nonzero(cd{mre[o], mim[o]})) bad = 1.0;
+    }
+    tr = block_sum(tr, red);
+    ti = block_sum(ti, red);
+    bad = block_sum(bad, red);
     if (tid == 0) {
-      cd tot = {0.0, 0.0};
-      for (int o = 0; o < nops; ++o) {
-        tot = tot + cd{contrib[2 * o], contrib[2 * o + 1]};
-        // a string changing several sites violates the caller's max_sites_changed bound
-        if (code[o] == -2 && nonzero(cd{mstr[2 * o], mstr[2 * o + 1]})) tot = {NAN, NAN};
-      }
-      out[2 * s] = tot.re;
-      out[2 * s + 1] = tot.im;
+      out[2 * s] = bad > 0.0 ? NAN : tr;
+      out[2 * s + 1] = bad > 0.0 ? NAN : ti;
     }
-    __syncthreads();
   }
 }
 
@@ -463,16 +555,6 @@ OpDev make_opdev(const jv_optable* op, const double* pref, int L) {
   d.diag = op->diag;
   d.pref = pref;
   return d;
-}
-
-int cnn_cmax(const CnnGeom& g) {
-  int m = 0;
-  for (int l = 0; l < g.nl; ++l) m = std::max(m, g.C[l]);
-  if (m <= 4) return 4;
-  if (m <= 8) return 8;
-  if (m <= 16) return 16;
-  if (m <= 32) return 32;
-  return 0;
 }
 
 int check_op(const jv_optable* op, const double* pref) {
@@ -533,24 +615,29 @@ int jv_eloc(const jv_net* net, const double* params, const jv_optable* op, const
     return JV_OK;
   }
   if (make_cnn_geom(net, cg) && op_single_site && win_ok(cg) && cnn_cmax(cg) > 0) {
-    int fch = 32;
-    while (fch > 1 && elo_win_bytes(cg, op->num_ops, fch) > 78 * 1024) fch /= 2;
-    size_t lds = elo_win_bytes(cg, op->num_ops, fch);
-    int64_t blocks = std::min<int64_t>(n, 16384);
-    ScopedTimer tm("eloc", st);
-    switch (cnn_cmax(cg)) {
+    const int cm = cnn_cmax(cg);
+    const int fch = elo_win_chunk(cg, cm, op->num_ops, 256);
+    const size_t lds = elo_win_bytes(cg, cm, op->num_ops, fch);
+    if (lds <= 160 * 1024) {
+      const CnnPack pk = make_cnn_pack(cg, cm, 1);
+      ScopedTimer tm("eloc", st);
+      const double* wpk = pack_cnn_weights(cg, pk, params, st, 40);
+      if (!wpk) return JV_E_NOMEM;
+      const int64_t blocks = std::min<int64_t>(n, 1 << 20);
+      switch (cm) {
 #define JV_ELO(CMv)                                                                                     \
   case CMv:                                                                                             \
     rc = set_lds_limit((const void*)eloc_cnn_win<CMv>, lds);                                            \
     if (rc) return rc;                                                                                  \
-    hipLaunchKernelGGL(eloc_cnn_win<CMv>, dim3((unsigned)blocks), dim3(256), lds, st, cg, params, d, configs, \
+    hipLaunchKernelGGL(eloc_cnn_win<CMv>, dim3((unsigned)blocks), dim3(256), lds, st, cg, wpk, pk, d, configs, \
                        logpsi_s, n, fch, out);                                                          \
     break;
-      JV_ELO(4) JV_ELO(8) JV_ELO(16) JV_ELO(32)
+        JV_ELO(4) JV_ELO(8) JV_ELO(16) JV_ELO(32)
 #undef JV_ELO
+      }
+      JV_LAUNCH_CHECK("eloc_cnn_win");
+      return JV_OK;
     }
-    JV_LAUNCH_CHECK("eloc_cnn_win");
-    return JV_OK;
   }
   if (make_cnn_geom(net, cg)) {
     EllocCnnLayout lay = eloc_cnn_layout(cg, false);

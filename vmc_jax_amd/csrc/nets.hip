@@ -3,7 +3,7 @@
 #include <string>
 #include <vector>
 
-#include "nets.hpp"
+#include "cnnwin.hpp"
 
 namespace jv {
 
@@ -209,6 +209,57 @@ __global__ void __launch_bounds__(256) cnn_fwd_grad(CnnGeom g, const double* __r
       __syncthreads();
     }
   }
+}
+
+// ------------------------------------------------------------------ packed CNN weights
+__global__ void cnn_pack(CnnGeom g, CnnPack pk, const double* __restrict__ Pg, double* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= pk.total) return;
+  int l = 0;
+  while (l + 1 < g.nl && t >= pk.koff[l + 1]) ++l;
+  const int cm = pk.cm, cg_n = cm / pk.groups;
+  const int cinp = l == 0 ? 1 : cm;
+  double v = 0.0;
+  if (t >= pk.boff[l]) {
+    const int c = (int)(t - pk.boff[l]);
+    if (c < g.C[l] && g.boff[l] >= 0) v = Pg[g.boff[l] + c];
+  } else {
+    int64_t r = t - pk.koff[l];
+    const int cg = (int)(r % cg_n);
+    r /= cg_n;
+    const int ci = (int)(r % cinp);
+    r /= cinp;
+    const int gi = (int)(r % pk.groups);
+    const int k = (int)(r / pk.groups);
+    const int c = gi * cg_n + cg;
+    if (ci < g.Cin[l] && c < g.C[l]) v = Pg[g.koff[l] + ((int64_t)k * g.Cin[l] + ci) * g.C[l] + c];
+  }
+  out[t] = v;
+}
+
+double* pack_cnn_weights(const CnnGeom& g, const CnnPack& pk, const double* params, hipStream_t st, int slot) {
+  double* out = (double*)workspace(sizeof(double) * (size_t)pk.total, slot);
+  if (!out) {
+    fail(JV_E_NOMEM, "packed CNN weights: workspace allocation failed");
+    return nullptr;
+  }
+  hipLaunchKernelGGL(cnn_pack, dim3((unsigned)((pk.total + 255) / 256)), dim3(256), 0, st, g, pk, params, out);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    hip_fail(e, "cnn_pack launch");
+    return nullptr;
+  }
+  return out;
+}
+
+int cnn_cmax(const CnnGeom& g) {
+  int m = 0;
+  for (int l = 0; l < g.nl; ++l) m = std::max(m, g.C[l]);
+  if (m <= 4) return 4;
+  if (m <= 8) return 8;
+  if (m <= 16) return 16;
+  if (m <= 32) return 32;
+  return 0;
 }
 
 }  // namespace jv

@@ -2,7 +2,7 @@
 // sample weights of the Metropolis and exact samplers.
 #include <algorithm>
 
-#include "nets.hpp"
+#include "cnnwin.hpp"
 
 namespace jv {
 
@@ -180,111 +180,218 @@ __global__ void __launch_bounds__(256) mc_cnn(CnnGeom g, const double* __restric
 }
 
 // ------------------------------------------------------------------ CNN chains, single-flip windows
-// One wavefront per chain; lane j computes all channels of window position j of
-// each layer (closed-form receptive field of the flipped site), so a proposal
-// costs one pass over the 9 / 25 / 49 window positions of the 3x3 net.
-__host__ __device__ inline size_t mc_win_chain_doubles(const CnnGeom& g) {
-  size_t d = (size_t)g.n + g.atot + (g.n + 1) / 2;  // x0, A, walker (int32)
-  for (int l = 0; l < g.nl; ++l) d += (size_t)win_ex(g, l) * win_ey(g, l) * g.C[l];
-  return d;
+// One workgroup of G waves per chain.  Wave w owns output channels
+// [w*CG, (w+1)*CG) of every layer (CG = CM/G), lane j the window element j, so
+// the weights a wave reads are wave-uniform (scalar loads).  Activations live in
+// per-layer buffers U + l*CM*CS: CM planes of CS doubles, [0, n) the chain's
+// current activations and [n, n + wsz_l) the proposed window.  Before each
+// proposal the workgroup tabulates, for every window element and kernel offset,
+// where its input lies (a stored position, or n + window index), so the
+// convolution reads inputs through one table lookup and immediate offsets.
+// Per proposal: one barrier per layer, then the accept / commit / next-table
+// phase.  The acceptance draw of step t is a pure function of (seed, t, chain,
+// call, stream); each wave computes 64 draws at a time, one per lane.
+__host__ __device__ inline int mc_win_tab_len(const CnnGeom& g) {
+  int t = 0;
+  for (int l = 0; l < g.nl; ++l) t += win_ex(g, l) * win_ey(g, l) * g.FF;
+  return t;
 }
-__host__ __device__ inline size_t mc_win_shared_doubles(const CnnGeom& g) {
-  return ((size_t)g.n * g.FF + 3) / 4;  // nbr table (shorts)
+__host__ __device__ inline int mc_win_pos_len(const CnnGeom& g) {
+  int t = 0;
+  for (int l = 0; l < g.nl; ++l) t += win_ex(g, l) * win_ey(g, l);
+  return t;
+}
+__host__ __device__ inline size_t mc_win_bytes(const CnnGeom& g, int cm, int cs) {
+  const size_t d = (size_t)g.nl * cm * cs + g.n + 16;                                // U, x0, red
+  const size_t i32 = (size_t)g.n + 2 * (size_t)mc_win_tab_len(g) + 2 * (size_t)mc_win_pos_len(g);
+  return 8 * d + 4 * i32 + 2 * (size_t)g.n * g.FF;
 }
 
-template <int CM>
-__global__ void __launch_bounds__(256) mc_cnn_win(CnnGeom g, const double* __restrict__ Pg, jv_mc_args a,
-                                                  int32_t* __restrict__ walkers, int32_t* __restrict__ configs,
-                                                  int64_t* __restrict__ counters) {
+// Input-location table of the proposal at `site` (all threads of the block).
+__device__ inline void mc_win_tables(const CnnGeom& g, int site, int* tab, int* pos, int tid, int nt) {
+  const int si = site / g.Ly, sj = site - (site / g.Ly) * g.Ly;
+  int toff = 0, poff = 0;
+  for (int l = 0; l < g.nl; ++l) {
+    const int ey = win_ey(g, l), wsz = win_ex(g, l) * ey;
+    const int pex = win_ex(g, l - 1), pey = win_ey(g, l - 1);
+    for (int t = tid; t < wsz * g.FF; t += nt) {
+      const int j = t / g.FF, k = t - j * g.FF;
+      const int a = j / ey, b = j - a * ey;
+      const int ky = k / g.Fy, kx = k - ky * g.Fy;
+      const int aq = a - ky, bq = b - kx;
+      int qi = si - aq;
+      qi = qi < 0 ? qi + g.Lx : (qi >= g.Lx ? qi - g.Lx : qi);
+      int qj = sj - bq;
+      qj = qj < 0 ? qj + g.Ly : (qj >= g.Ly ? qj - g.Ly : qj);
+      const bool in = l > 0 && aq >= 0 && aq < pex && bq >= 0 && bq < pey;
+      tab[toff + t] = in ? g.n + aq * pey + bq : qi * g.Ly + qj;
+    }
+    for (int j = tid; j < wsz; j += nt) pos[poff + j] = win_site(g, si, sj, j / ey, j - (j / ey) * ey);
+    toff += wsz * g.FF;
+    poff += wsz;
+  }
+}
+
+// Plane stride of the window buffers: the smallest instantiated size holding
+// the n positions and the largest window (0 when none does).
+inline int mc_win_cs(const CnnGeom& g) {
+  int need = 0;
+  for (int l = 0; l < g.nl; ++l) need = std::max(need, g.n + win_ex(g, l) * win_ey(g, l));
+  for (int cs : {128, 256, 512, 1024})
+    if (need <= cs) return cs;
+  return 0;
+}
+
+template <int CM, int G, int CS>
+__global__ void __launch_bounds__(64 * G) mc_cnn_win(CnnGeom g, const double* __restrict__ Wpk, CnnPack pk,
+                                                     jv_mc_args a, int32_t* __restrict__ walkers,
+                                                     int32_t* __restrict__ configs,
+                                                     int64_t* __restrict__ counters) {
+  constexpr int CG = CM / G;
   extern __shared__ double lds[];
-  const int tid = threadIdx.x, nt = blockDim.x;
-  const int lane = tid % WAVE, w = tid / WAVE, nw = nt / WAVE;
-  short* nbr = (short*)lds;
+  const int tid = threadIdx.x, nt = blockDim.x, lane = tid % WAVE;
+  const int w = __builtin_amdgcn_readfirstlane(tid / WAVE);
+  const int chain = blockIdx.x;
+  const int L1 = g.nl - 1;
+  const int tlen = mc_win_tab_len(g), plen = mc_win_pos_len(g);
+  double* U = lds;
+  double* x0 = U + (size_t)g.nl * CM * CS;
+  double* red = x0 + g.n;
+  int32_t* s = (int32_t*)(red + 16);
+  int32_t* tabs = s + g.n;       // [2][tlen]
+  int32_t* poss = tabs + 2 * tlen;  // [2][plen]
+  short* nbr = (short*)(poss + 2 * plen);
+  const cdbl* Wp = as_const(Wpk);
   for (int t = tid; t < g.n * g.FF; t += nt) {
-    int p = t / g.FF, k = t % g.FF;
-    int i = p / g.Ly, j = p % g.Ly, ky = k / g.Fy, kx = k % g.Fy;
+    const int p = t / g.FF, k = t % g.FF;
+    const int i = p / g.Ly, j = p % g.Ly, ky = k / g.Fy, kx = k % g.Fy;
     nbr[t] = (short)(((i + ky) % g.Lx) * g.Ly + (j + kx) % g.Ly);
   }
-  __syncthreads();
-  const int chain = blockIdx.x * nw + w;
-  if (chain >= a.num_chains) return;
-  double* x0 = lds + mc_win_shared_doubles(g) + (size_t)w * mc_win_chain_doubles(g);
-  double* A = x0 + g.n;
-  double* W[JV_MAX_LAYERS];
-  double* wp = A + g.atot;
-  for (int l = 0; l < g.nl; ++l) {
-    W[l] = wp;
-    wp += (size_t)win_ex(g, l) * win_ey(g, l) * g.C[l];
-  }
-  int32_t* s = (int32_t*)wp;
-  for (int i = lane; i < g.n; i += WAVE) {
+  for (int i = tid; i < g.n; i += nt) {
     s[i] = walkers[(int64_t)chain * g.n + i];
     x0[i] = (double)(2 * s[i] - 1);
   }
-  wave_sync();
-  cnn_forward_pos<CM>(g, Pg, x0, A, nullptr, nbr, lane, WAVE, [] { wave_sync(); });
-  const int L1 = g.nl - 1;
-  int64_t nacc = 0;
-  double z[CM];
+  __syncthreads();
+  win_forward<CM, G>(g, Wp, pk, x0, U, CM * CS, CS, nbr, tid, nt, [] { __syncthreads(); });
   const int total = a.therm_steps + a.samples_per_chain * a.sweep_steps;
+  // per-lane batch of 64 draws
+  int dsite = 0;
+  double du = 0.0;
+  auto batch = [&](int base) {
+    if (base + lane < total)
+      mc_draw(a.seed, (uint32_t)(base + lane), (uint32_t)chain, a.call_index, a.stream, g.n, dsite, du);
+  };
+  batch(0);
+  int site = __builtin_amdgcn_readlane(dsite, 0);
+  if (total > 0) mc_win_tables(g, site, tabs, poss, tid, nt);
+  __syncthreads();
+  int64_t nacc = 0;
   for (int step = 0; step < total; ++step) {
-    int site;
-    double u;
-    mc_draw(a.seed, (uint32_t)step, (uint32_t)chain, a.call_index, a.stream, g.n, site, u);
-    const int si = site / g.Ly, sj = site % g.Ly;
-    const int nv = 1 - s[site];
-    const double xflip = (double)(2 * nv - 1);
+    const int cur = step & 1;
+    const int* tab = tabs + cur * tlen;
+    const int* pos = poss + cur * plen;
+    const int r = step & 63;
+    const int ulo = __builtin_amdgcn_readlane((int)__double2loint(du), r);
+    const int uhi = __builtin_amdgcn_readlane((int)__double2hiint(du), r);
+    const double u = __hiloint2double(uhi, ulo);
     double diff = 0.0;
+    int toff = 0, poff = 0;
     for (int l = 0; l < g.nl; ++l) {
-      const int ey = win_ey(g, l), wsz = win_ex(g, l) * ey, cout = g.C[l];
-      const double* Xold = l == 0 ? x0 : A + g.aoff[l - 1];
+      const int wsz = win_ex(g, l) * win_ey(g, l);
+      const cdbl* K = Wp + pk.koff[l];
+      const cdbl* B = Wp + pk.boff[l] + w * CG;
+      const double* X = l == 0 ? x0 : U + (size_t)(l - 1) * CM * CS;
+      double* Ul = U + (size_t)l * CM * CS + (size_t)w * CG * CS;
       for (int j = lane; j < wsz; j += WAVE) {
-        const int aa = j / ey, bb = j % ey;
-        cnn_preact_win<CM>(g, Pg, l, Xold, l == 0 ? nullptr : W[l - 1], xflip, si, sj, aa, bb, z);
-        double* Wo = W[l] + j;  // planar [c][j]
-        const double* Ab = A + g.aoff[l] + win_pos(g, si, sj, aa, bb);
+        double z[CG];
 #pragma unroll
-        for (int c = 0; c < CM; ++c)
-          if (c < cout) {
-            const double v = log_cosh(z[c]);
-            Wo[c * wsz] = v;
-            if (l == L1) diff += v - Ab[c * g.n];
+        for (int c = 0; c < CG; ++c) z[c] = B[c];
+        const int* tj = tab + toff + j * g.FF;
+        if (l == 0) {
+          for (int k = 0; k < g.FF; ++k) {
+            const int q = tj[k];
+            const double xv = q == site ? -x0[q] : x0[q];
+            const cdbl* wk = K + (k * G + w) * CG;
+#pragma unroll
+            for (int c = 0; c < CG; ++c) z[c] = fma(xv, wk[c], z[c]);
           }
-      }
-      wave_sync();
-    }
-    const double dl = wave_sum(diff) * g.inv_nrm;
-    const bool acc = u < exp(a.mu * dl);  // random.bernoulli(key, P), sampler.py:336-340
-    if (acc) {
-      for (int l = 0; l < g.nl; ++l) {
-        const int ey = win_ey(g, l), wsz = win_ex(g, l) * ey, cout = g.C[l];
-        for (int t = lane; t < wsz * cout; t += WAVE) {
-          const int c = t / wsz, j = t % wsz;
-          A[g.aoff[l] + c * g.n + win_pos(g, si, sj, j / ey, j % ey)] = W[l][t];
+        } else {
+          for (int k = 0; k < g.FF; ++k) {
+            const double* xq = X + tj[k];
+            const cdbl* wk = K + (size_t)(k * G + w) * CM * CG;
+#pragma unroll
+            for (int ci = 0; ci < CM; ++ci) {
+              const double xv = xq[ci * CS];
+#pragma unroll
+              for (int c = 0; c < CG; ++c) z[c] = fma(xv, wk[ci * CG + c], z[c]);
+            }
+          }
+        }
+        if (l < L1) {
+#pragma unroll
+          for (int c = 0; c < CG; ++c) Ul[c * CS + g.n + j] = log_cosh(z[c]);
+        } else {
+          const int p = pos[poff + j];
+#pragma unroll
+          for (int c = 0; c < CG; ++c) {
+            const double v = log_cosh(z[c]);
+            Ul[c * CS + g.n + j] = v;
+            diff += v - Ul[c * CS + p];
+          }
         }
       }
-      if (lane == 0) {
-        s[site] = nv;
-        x0[site] = xflip;
+      toff += wsz * g.FF;
+      poff += wsz;
+      if (l < L1) __syncthreads();
+    }
+    diff = wave_sum(diff);
+    if (lane == 0) red[w] = diff;
+    __syncthreads();
+    double dsum = 0.0;
+#pragma unroll
+    for (int i = 0; i < G; ++i) dsum += red[i];
+    const double dl = dsum * g.inv_nrm;
+    const bool acc = u < exp(a.mu * dl);  // random.bernoulli(key, P), sampler.py:336-340
+    if (acc) {
+      int t = tid;
+      int po = 0;
+      for (int l = 0; l < g.nl; ++l) {
+        const int wsz = win_ex(g, l) * win_ey(g, l);
+        for (; t < CM * wsz; t += nt) {
+          const int c = t / wsz, j = t - c * wsz;
+          double* Uc = U + ((size_t)l * CM + c) * CS;
+          Uc[pos[po + j]] = Uc[g.n + j];
+        }
+        t -= CM * wsz;
+        po += wsz;
+      }
+      if (tid == 0) {
+        s[site] = 1 - s[site];
+        x0[site] = -x0[site];
       }
       ++nacc;
     }
-    wave_sync();
+    if (step + 1 < total) {
+      if (((step + 1) & 63) == 0) batch(step + 1);
+      site = __builtin_amdgcn_readlane(dsite, (step + 1) & 63);
+      mc_win_tables(g, site, tabs + (cur ^ 1) * tlen, poss + (cur ^ 1) * plen, tid, nt);
+    }
+    __syncthreads();
     const int k = step + 1 - a.therm_steps;
     if (k > 0 && k % a.sweep_steps == 0) {
       const int64_t t = k / a.sweep_steps - 1;
-      int32_t* out = configs + (t * a.num_chains + chain) * g.n;
-      for (int i = lane; i < g.n; i += WAVE) out[i] = s[i];
+      int32_t* outc = configs + (t * a.num_chains + chain) * g.n;
+      for (int i = tid; i < g.n; i += nt) outc[i] = s[i];
     }
   }
-  for (int i = lane; i < g.n; i += WAVE) walkers[(int64_t)chain * g.n + i] = s[i];
-  if (lane == 0) {
+  for (int i = tid; i < g.n; i += nt) walkers[(int64_t)chain * g.n + i] = s[i];
+  if (tid == 0) {
     atomicAdd((unsigned long long*)&counters[0], (unsigned long long)total);
     atomicAdd((unsigned long long*)&counters[1], (unsigned long long)nacc);
   }
 }
 
-int cnn_cmax(const CnnGeom& g);
+
 
 // ------------------------------------------------------------------ weights
 // p_i = exp(scale * Re(logpsi_i) - shift); one block, deterministic order.
@@ -350,23 +457,30 @@ int jv_mc_sample(const jv_net* net, const double* params, const jv_mc_args* args
     hipLaunchKernelGGL(mc_rbm, dim3((unsigned)((a.num_chains + nw - 1) / nw)), dim3(64 * nw), lds, st, rg,
                        params, a, walkers, configs, counters);
     JV_LAUNCH_CHECK("mc_rbm");
-  } else if (make_cnn_geom(net, cg) && win_ok(cg) && cnn_cmax(cg) > 0) {
-    const int nw = a.num_chains <= 1024 ? 1 : 4;
-    size_t lds = 8 * (mc_win_shared_doubles(cg) + nw * mc_win_chain_doubles(cg));
-    const unsigned blocks = (unsigned)((a.num_chains + nw - 1) / nw);
+  } else if (make_cnn_geom(net, cg) && win_ok(cg) && cnn_cmax(cg) > 0 && mc_win_cs(cg) > 0 &&
+             mc_win_bytes(cg, cnn_cmax(cg), mc_win_cs(cg)) <= 160 * 1024) {
+    const int cm = cnn_cmax(cg), cs = mc_win_cs(cg);
+    const int groups = cm >= 8 ? 4 : 2;
+    const size_t lds = mc_win_bytes(cg, cm, cs);
+    const CnnPack pk = make_cnn_pack(cg, cm, groups);
     ScopedTimer tm("mc_sweep", st);
+    const double* wpk = pack_cnn_weights(cg, pk, params, st, 41);
+    if (!wpk) return JV_E_NOMEM;
     int rc = JV_OK;
-    switch (cnn_cmax(cg)) {
-#define JV_MCW(CMv)                                                                                     \
-  case CMv:                                                                                             \
-    rc = set_lds_limit((const void*)mc_cnn_win<CMv>, lds);                                              \
+    const void* fn = nullptr;
+#define JV_MCW(CMv, Gv, CSv)                                                                            \
+  if (cm == CMv && cs == CSv) {                                                                         \
+    fn = (const void*)mc_cnn_win<CMv, Gv, CSv>;                                                         \
+    rc = set_lds_limit(fn, lds);                                                                        \
     if (rc) return rc;                                                                                  \
-    hipLaunchKernelGGL(mc_cnn_win<CMv>, dim3(blocks), dim3(64 * nw), lds, st, cg, params, a, walkers,       \
-                       configs, counters);                                                              \
-    break;
-      JV_MCW(4) JV_MCW(8) JV_MCW(16) JV_MCW(32)
+    hipLaunchKernelGGL((mc_cnn_win<CMv, Gv, CSv>), dim3((unsigned)a.num_chains), dim3(64 * Gv), lds, st, cg, \
+                       wpk, pk, a, walkers, configs, counters);                                         \
+  }
+#define JV_MCW_CS(CMv, Gv) JV_MCW(CMv, Gv, 128) JV_MCW(CMv, Gv, 256) JV_MCW(CMv, Gv, 512) JV_MCW(CMv, Gv, 1024)
+    JV_MCW_CS(4, 2) JV_MCW_CS(8, 4) JV_MCW_CS(16, 4) JV_MCW_CS(32, 4)
+#undef JV_MCW_CS
 #undef JV_MCW
-    }
+    if (!fn) return fail(JV_E_ARG, "jv_mc_sample: no window kernel for this channel count");
     JV_LAUNCH_CHECK("mc_cnn_win");
   } else if (make_cnn_geom(net, cg)) {
     McCnnLayout lay = mc_cnn_layout(cg);
