@@ -154,6 +154,13 @@ def run_gpu(workload, steps, warmup, mode):
     return w, elapsed, kt, energies
 
 
+# HBM bytes per launch (read + written) of the hand-written kernels on this
+# workload, from rocprofv3 FETCH_SIZE / WRITE_SIZE passes scaled by a copy
+# kernel of known size (x2.000 read, x1.000 written); the summary with the
+# commands is profiles/eloc_mc_gram_tfim10x10_cnn_hbm_traffic.md.
+MEASURED_TRAFFIC = {"eloc": 6.204e6 + 0.262e6, "mc_sweep": 0.763e6 + 3.653e6, "gram": 651.451e6 + 12.313e6}
+
+
 def roofline_c3(kt, steps, mode):
     """Roofline of the dominant hand-written kernel, algorithmic counts per step:
     E_loc 2.03e11 flop (8192 samples x 101 strings x a 244,800-flop forward),
@@ -169,7 +176,8 @@ def roofline_c3(kt, steps, mode):
     flops = per_step[dom] * steps / launches
     ach = flops / (ms * 1e-3) / 1e12
     return {"bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": ach / FP64_PEAK_TFLOPS, "traffic": None, "kernel": dom, "avg_launch_ms": ms,
+            "frac": ach / FP64_PEAK_TFLOPS, "traffic": MEASURED_TRAFFIC.get(dom) if mode == "dual" else None,
+            "kernel": dom, "avg_launch_ms": ms,
             "launches": launches}
 
 
@@ -298,6 +306,9 @@ def main():
     ap.add_argument("--t-steps", type=int, default=3, help="c3: steps of the other MinSR form reported beside")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--serve-jobs", type=int, default=16, help="c3: jobs of the served mix (0 = skip)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="end with an elementwise 512 MiB -> 512 MiB FP64 device copy (a kernel of known "
+                         "HBM traffic that scales the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (every box has one MI355X); "
@@ -344,6 +355,15 @@ def main():
             except Exception as e:  # report, never hide the GPU number
                 out["cpu_baseline"] = {"error": repr(e)}
     print(json.dumps(out))
+    if args.calibrate_copy:
+        import torch
+        a = torch.ones(64 << 20, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        torch.mul(a, 1.0, out=b)
+        torch.cuda.synchronize()
+        print("calibration copy: %d bytes read, %d bytes written" % (a.numel() * 8, b.numel() * 8),
+              file=sys.stderr)
 
 
 if __name__ == "__main__":
