@@ -475,6 +475,21 @@ def test_minsr_dual_equals_reference_form():
     assert rel(d, c) < 1e-9
 
 
+@pytest.mark.parametrize("shape,ch,N,seed", [((10, 10), (8, 8, 8), 2048, 31), ((6, 6), (4, 4, 4), 128, 32)])
+def test_minsr_dual_tridiagonal(shape, ch, N, seed):
+    """The dual solve through the Householder tridiagonalisation (P >= 256) equals
+    the reference T form: full rank (the 10x10 workload's 1240 parameters) and
+    rank deficient (N < P, most eigenvalues of O^T O at round-off level)."""
+    psi, opsi, H, Ho, s, lp, p = _cnn_case(shape, ch, N, seed)
+    El = jv.stats.SampledObs(H.get_O_loc(s, psi, lp, 0.0), p)
+    G = jv.stats.SampledObs(psi.gradients(s, layout=0), p)
+    assert psi.desc.num_real_params >= 256
+    for tol, shift in ((1e-10, 0.0), (1e-8, 1e-3)):
+        a = to_np(jv.util.MinSR(None, pinvTol=tol, diagonalShift=shift, solver="T").solve(El, G, False))
+        b = to_np(jv.util.MinSR(None, pinvTol=tol, diagonalShift=shift, solver="dual").solve(El, G, False))
+        assert rel(b, a) < 1e-7, (shape, N, tol, shift, rel(b, a))
+
+
 def test_minsr_dual_holomorphic():
     rng = np.random.default_rng(27)
     L = 8

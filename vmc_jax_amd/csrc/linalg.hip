@@ -10,6 +10,12 @@
 
 namespace jv {
 
+// tridiag.hip
+bool trd_supported(int64_t n);
+size_t trd_work_doubles(int n);
+int trd_eig_solve(double* S, int n, double* ev, double* c, double* Zbuf, double* work, hipStream_t st);
+int trd_apply(const double* V, const double* tau, int n, double* x, hipStream_t st);
+
 rocblas_handle blas_handle(hipStream_t stream);
 
 typedef double v4d __attribute__((ext_vector_type(4)));
@@ -90,6 +96,8 @@ struct GemmArgs {
   int cpx;
   int sym;
   double alpha;
+  int64_t kchunk;  // split-K: workgroup row blockIdx.y covers k in [y*kchunk, (y+1)*kchunk)
+  double* part;    // split-K partial tiles [gridDim.y][M][N] (null: write C directly)
 };
 
 constexpr int BM = 64, BN = 64, BK = 16;
@@ -131,7 +139,9 @@ __global__ void __launch_bounds__(256) gemm_f64(GemmArgs g) {
     }
   const double sa_im = g.conjA ? -1.0 : 1.0, sb_im = g.conjB ? -1.0 : 1.0;
   const bool a_k_fast = g.sak == 1, b_k_fast = g.sbk == 1;
-  for (int64_t k0 = 0; k0 < g.K; k0 += BK) {
+  const int64_t kbeg = g.part ? (int64_t)blockIdx.y * g.kchunk : 0;
+  const int64_t kend = g.part ? std::min<int64_t>(g.K, kbeg + g.kchunk) : g.K;
+  for (int64_t k0 = kbeg; k0 < kend; k0 += BK) {
     // global -> LDS (zero padded), each thread moves 4 elements of each tile
     for (int r = 0; r < (BM * BK) / 256; ++r) {
       int e = tid + 256 * r;
@@ -139,7 +149,7 @@ __global__ void __launch_bounds__(256) gemm_f64(GemmArgs g) {
       int kk = a_k_fast ? e % BK : e / BM;
       int64_t gm = m0 + mm, gk = k0 + kk;
       double vr = 0.0, vi = 0.0;
-      if (gm < g.M && gk < g.K) {
+      if (gm < g.M && gk < kend) {
         int64_t off = gm * g.sam + gk * g.sak;
         if (CPX) { vr = g.A[2 * off]; vi = sa_im * g.A[2 * off + 1]; }
         else vr = g.A[off];
@@ -150,7 +160,7 @@ __global__ void __launch_bounds__(256) gemm_f64(GemmArgs g) {
       int kb = b_k_fast ? e % BK : e / BN;
       int64_t gn = n0 + nn, gkb = k0 + kb;
       vr = 0.0; vi = 0.0;
-      if (gn < g.N && gkb < g.K) {
+      if (gn < g.N && gkb < kend) {
         int64_t off = gkb * g.sbk + gn * g.sbn;
         if (CPX) { vr = g.B[2 * off]; vi = sb_im * g.B[2 * off + 1]; }
         else vr = g.B[off];
@@ -183,6 +193,23 @@ __global__ void __launch_bounds__(256) gemm_f64(GemmArgs g) {
     __syncthreads();
   }
   // epilogue
+  if (g.part) {  // split-K partial tile, reduced and mirrored by gemm_reduce
+    double* P = g.part + (size_t)blockIdx.y * g.M * g.N * (CPX ? 2 : 1);
+    for (int a = 0; a < 2; ++a)
+      for (int b = 0; b < 2; ++b)
+        for (int j = 0; j < 4; ++j) {
+          int64_t i = m0 + wm + 16 * a + mf_crow(lane, j);
+          int64_t c = n0 + wn + 16 * b + mf_ccol(lane, j);
+          if (i >= g.M || c >= g.N) continue;
+          if (CPX) {
+            P[2 * (i * g.N + c)] = accr[a][b][j];
+            P[2 * (i * g.N + c) + 1] = acci[a][b][j];
+          } else {
+            P[i * g.N + c] = accr[a][b][j];
+          }
+        }
+    return;
+  }
   for (int a = 0; a < 2; ++a)
     for (int b = 0; b < 2; ++b)
       for (int j = 0; j < 4; ++j) {
@@ -206,15 +233,68 @@ __global__ void __launch_bounds__(256) gemm_f64(GemmArgs g) {
       }
 }
 
-int gemm(const GemmArgs& g, hipStream_t st) {
+// Sums the split-K partials in split order into C (alpha scaled), mirroring
+// Hermitian outputs exactly like the direct epilogue.
+template <bool CPX>
+__global__ void gemm_reduce(GemmArgs g, int splits) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.M * g.N) return;
+  const int64_t i = t / g.N, c = t % g.N;
+  if (g.sym && (i / BM > c / BN || (i / BM == c / BN && i > c))) return;
+  const size_t stride = (size_t)g.M * g.N * (CPX ? 2 : 1);
+  double vr = 0.0, vi = 0.0;
+  for (int s = 0; s < splits; ++s) {
+    if (CPX) {
+      vr += g.part[s * stride + 2 * t];
+      vi += g.part[s * stride + 2 * t + 1];
+    } else {
+      vr += g.part[s * stride + t];
+    }
+  }
+  vr *= g.alpha;
+  vi *= g.alpha;
+  if (CPX) {
+    g.C[2 * (i * g.ldc + c)] = vr;
+    g.C[2 * (i * g.ldc + c) + 1] = vi;
+    if (g.sym && i != c) {
+      g.C[2 * (c * g.ldc + i)] = vr;
+      g.C[2 * (c * g.ldc + i) + 1] = -vi;
+    }
+  } else {
+    g.C[i * g.ldc + c] = vr;
+    if (g.sym && i != c) g.C[c * g.ldc + i] = vr;
+  }
+}
+
+int gemm(const GemmArgs& g0, hipStream_t st) {
+  GemmArgs g = g0;
+  g.part = nullptr;
+  g.kchunk = 0;
   if (g.M <= 0 || g.N <= 0) return JV_OK;
   const int64_t ntm = (g.M + BM - 1) / BM, ntn = (g.N + BN - 1) / BN;
   int64_t blocks = g.sym ? ntn * (ntn + 1) / 2 : ntm * ntn;
   if (g.sym && (g.M != g.N)) return fail(JV_E_ARG, "symmetric GEMM needs a square output");
+  // split K when the tiles alone cannot fill the chip (~4 workgroups per CU)
+  int splits = 1;
+  while (blocks * splits < 1024 && g.K / (splits * 2) >= 4 * BK) splits *= 2;
   ScopedTimer tm(g.sym ? "gram" : "gemm", st);
-  if (g.cpx) hipLaunchKernelGGL(gemm_f64<true>, dim3((unsigned)blocks), dim3(256), 0, st, g);
-  else hipLaunchKernelGGL(gemm_f64<false>, dim3((unsigned)blocks), dim3(256), 0, st, g);
+  if (splits > 1) {
+    const size_t bytes = sizeof(double) * (size_t)splits * g.M * g.N * (g.cpx ? 2 : 1);
+    g.part = (double*)workspace(bytes, 18);
+    if (!g.part) return fail(JV_E_NOMEM, "gemm: split-K workspace allocation failed");
+    g.kchunk = ((g.K + splits - 1) / splits + BK - 1) / BK * BK;
+    splits = (int)((g.K + g.kchunk - 1) / g.kchunk);
+  }
+  const dim3 grid((unsigned)blocks, (unsigned)splits);
+  if (g.cpx) hipLaunchKernelGGL(gemm_f64<true>, grid, dim3(256), 0, st, g);
+  else hipLaunchKernelGGL(gemm_f64<false>, grid, dim3(256), 0, st, g);
   JV_LAUNCH_CHECK("gemm_f64");
+  if (splits > 1) {
+    const unsigned rb = (unsigned)((g.M * g.N + 255) / 256);
+    if (g.cpx) hipLaunchKernelGGL(gemm_reduce<true>, dim3(rb), dim3(256), 0, st, g, splits);
+    else hipLaunchKernelGGL(gemm_reduce<false>, dim3(rb), dim3(256), 0, st, g, splits);
+    JV_LAUNCH_CHECK("gemm_reduce");
+  }
   return JV_OK;
 }
 
@@ -832,11 +912,29 @@ int jv_minsr_solve_dual(const double* data, const double* e, int64_t n, int64_t 
     rhs = er;
   }
   if ((rc = gemv(true, data, rhs, n, p, cpx, 1, 1.0, b, st, 12))) return rc;  // b = D^dagger e
-  if ((rc = eigh_raw(p, cpx, S, ev, st))) return rc;
-  if ((rc = gemv(false, S, b, p, p, cpx, 0, 1.0, y, st, 12))) return rc;
-  hipLaunchKernelGGL(pinv_scale, dim3(1), dim3(1024), 0, st, ev, p, rcond, cpx, y, rank);
-  JV_LAUNCH_CHECK("pinv_scale");
-  if ((rc = gemv(true, S, y, p, p, cpx, 1, 1.0, x, st, 12))) return rc;
+  if (!cpx && trd_supported(p)) {
+    // S = Q Z diag(ev) Z^T Q^T: Householder tridiagonalisation (Q^T b formed on
+    // the way), divide and conquer on T, then x = Q Z pinv(ev) Z^T Q^T b
+    double* Z = (double*)workspace(sizeof(double) * (size_t)p * p, 16);
+    double* tw = (double*)workspace(sizeof(double) * trd_work_doubles((int)p), 17);
+    if (!Z || !tw) return fail(JV_E_NOMEM, "jv_minsr_solve_dual: workspace allocation failed");
+    JV_HIP(hipMemcpyAsync(x, b, sizeof(double) * (size_t)p, hipMemcpyDeviceToDevice, st));
+    {
+      ScopedTimer tm("eigh", st);
+      if ((rc = trd_eig_solve(S, (int)p, ev, x, Z, tw, st))) return rc;
+    }
+    if ((rc = gemv(false, Z, x, p, p, 0, 0, 1.0, y, st, 12))) return rc;
+    hipLaunchKernelGGL(pinv_scale, dim3(1), dim3(1024), 0, st, ev, p, rcond, 0, y, rank);
+    JV_LAUNCH_CHECK("pinv_scale");
+    if ((rc = gemv(true, Z, y, p, p, 0, 1, 1.0, x, st, 12))) return rc;
+    if ((rc = trd_apply(S, tw + p, (int)p, x, st))) return rc;
+  } else {
+    if ((rc = eigh_raw(p, cpx, S, ev, st))) return rc;
+    if ((rc = gemv(false, S, b, p, p, cpx, 0, 1.0, y, st, 12))) return rc;
+    hipLaunchKernelGGL(pinv_scale, dim3(1), dim3(1024), 0, st, ev, p, rcond, cpx, y, rank);
+    JV_LAUNCH_CHECK("pinv_scale");
+    if ((rc = gemv(true, S, y, p, p, cpx, 1, 1.0, x, st, 12))) return rc;
+  }
   int64_t len = p * (cpx ? 2 : 1);
   hipLaunchKernelGGL(neg_copy, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, st, x, len, update);
   JV_LAUNCH_CHECK("neg_copy");

@@ -1,0 +1,370 @@
+// Householder tridiagonalisation of a real symmetric matrix, one launch per
+// column, and the application of its orthogonal factor to vectors: the
+// eigendecomposition behind the pseudo-inverse of the MinSR Gram matrix
+// (minsr.py:67-78, jnp.linalg.pinv(hermitian=True) = eigh + relative cutoff).
+//
+// A = Q T Q^T, Q = H_0 H_1 ... H_{n-2}, H_k = I - tau_k v_k v_k^T with
+// v_k[k+1] = 1 and v_k zero above k+1 (LAPACK dsytrd 'L' conventions: the same
+// reflector and sign rules as dlarfg, beta = -sign(alpha) ||x||).
+//
+// Step k needs y = A^(k-1) v_k (the trailing matrix after k-1 updates) from
+// every column, so each step is one launch (the launch boundary is the grid
+// synchronisation).  Workgroup g owns columns j = g (mod G); the rank-2
+// update of step k-1 is deferred into step k's pass over the owned columns,
+// which also forms y_j = column_j . v_k and
+//   u_j = A^(k-1)[j][k+1] - tau_k y_j.
+// Column k+1 after update k is then x = u + gamma v_k with the scalar
+// gamma = 2 alpha' - tau_k y_{k+1}, alpha' = tau_k^2 / 2 (y . v_k), so every
+// workgroup rebuilds x, v_{k+1} and w_k = tau_k y - alpha' v_k from the
+// published y, u and the per-workgroup partial dots alone: no workgroup ever
+// waits for another inside a launch.  One extra workgroup applies H_k to a
+// right-hand side as the reflectors appear (c = Q^T b).
+#include <hip/hip_runtime.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+
+#include "common.hpp"
+
+namespace jv {
+
+rocblas_handle blas_handle(hipStream_t stream);
+
+namespace {
+
+constexpr int TRD_T = 512;
+constexpr int TRD_W = TRD_T / WAVE;
+constexpr int TRD_R = 8;  // n-vector elements per thread (n <= TRD_T * TRD_R)
+constexpr int TRD_CP = 16;  // column rows per lane per load batch
+
+struct TrdArgs {
+  double* M;    // n x n symmetric (row k is overwritten by reflector k once step k has run)
+  double* d;    // diagonal of T [n]
+  double* e;    // off-diagonal of T [n-1]
+  double* tau;  // [n]
+  double* y;    // [2][n]
+  double* u;    // [2][n]
+  double* pd;   // [2][G]
+  double* c;    // right-hand side receiving Q^T (may be null)
+  double* v0;   // reflector 0 until step 1 copies it into row 0 of M
+  int n, G;
+};
+
+// Block sum in a fixed order (identical in every workgroup for identical
+// inputs); red holds TRD_W doubles.
+__device__ inline double trd_block_sum(double v, double* red) {
+  v = wave_sum(v);
+  const int w = threadIdx.x / WAVE, l = threadIdx.x % WAVE;
+  __syncthreads();
+  if (l == 0) red[w] = v;
+  __syncthreads();
+  double s = 0.0;
+#pragma unroll
+  for (int i = 0; i < TRD_W; ++i) s += red[i];
+  return s;
+}
+
+// Step k of the reduction (k = 0 .. n-1; step n-1 only finishes d[n-1]).
+// Global reads are batched per thread (all loads of a batch are issued before
+// any is used): each step is latency bound, not bandwidth bound.
+__global__ void __launch_bounds__(TRD_T) trd_step(TrdArgs a, int k) {
+  extern __shared__ double sh[];
+  const int n = a.n, G = a.G, tid = threadIdx.x, g = blockIdx.x;
+  const bool qwg = g == G;
+  // first owned column >= k+1
+  int j0 = g + ((k + 1 - g + G - 1) / G) * G;
+  if (g >= k + 1) j0 = g;
+  const bool owns = !qwg && j0 < n;
+  if (!owns && g != 0 && !qwg) {
+    if (threadIdx.x == 0) a.pd[(k & 1) * G + g] = 0.0;
+    return;
+  }
+  if (qwg && (a.c == nullptr || k > n - 2)) return;
+  const int w = tid / WAVE, lane = tid % WAVE;
+  const int jw = j0 + w * G;
+  double* vk = sh;          // v_k
+  double* vp = sh + n;      // v_{k-1}
+  double* wp = sh + 2 * n;  // w_{k-1}
+  double* x = sh + 3 * n;   // column k after update k-1
+  double* red = sh + 4 * n; // 2 * TRD_W
+  // ---- prologue: rebuild column k, its reflector and w_{k-1}
+  if (k == 0) {
+    double xr[TRD_R];
+#pragma unroll
+    for (int r = 0; r < TRD_R; ++r) {
+      const int i = tid + r * TRD_T;
+      xr[r] = i < n ? a.M[i] : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < TRD_R; ++r) {
+      const int i = tid + r * TRD_T;
+      if (i < n) {
+        x[i] = xr[r];
+        vp[i] = 0.0;
+        wp[i] = 0.0;
+      }
+    }
+  } else {
+    const int pb = (k - 1) & 1;
+    const double* yv = a.y + (size_t)pb * n;
+    const double* uv = a.u + (size_t)pb * n;
+    const double* vrow = k == 1 ? a.v0 : a.M + (size_t)(k - 1) * n;
+    double yr[TRD_R], ur[TRD_R], vr[TRD_R];
+#pragma unroll
+    for (int r = 0; r < TRD_R; ++r) {
+      const int i = tid + r * TRD_T;
+      const bool in = i >= k && i < n;
+      yr[r] = in ? yv[i] : 0.0;
+      ur[r] = in ? uv[i] : 0.0;
+      vr[r] = in ? vrow[i] : 0.0;
+    }
+    double pdv = 0.0;
+    for (int q = tid; q < G; q += TRD_T) pdv += a.pd[pb * G + q];
+    const double tp = a.tau[k - 1];
+    const double yk = yv[k];
+    const double sdot = trd_block_sum(pdv, red);
+    const double ap = 0.5 * tp * tp * sdot;
+    const double gam = 2.0 * ap - tp * yk;
+#pragma unroll
+    for (int r = 0; r < TRD_R; ++r) {
+      const int i = tid + r * TRD_T;
+      if (i < n) {
+        vp[i] = vr[r];
+        wp[i] = tp * yr[r] - ap * vr[r];
+        x[i] = ur[r] + gam * vr[r];
+      }
+    }
+  }
+  __syncthreads();
+  double tk = 0.0;
+  if (k <= n - 2) {
+    double part = 0.0;
+#pragma unroll
+    for (int r = 0; r < TRD_R; ++r) {
+      const int i = tid + r * TRD_T;
+      if (i >= k + 2 && i < n) part += x[i] * x[i];
+    }
+    const double sigma = trd_block_sum(part, red + TRD_W);
+    const double alpha = x[k + 1];
+    double beta = alpha, scale = 0.0;
+    if (sigma != 0.0) {
+      beta = -copysign(sqrt(alpha * alpha + sigma), alpha);
+      tk = (beta - alpha) / beta;
+      scale = 1.0 / (alpha - beta);
+    }
+    for (int i = tid; i < n; i += TRD_T) vk[i] = i < k + 1 ? 0.0 : (i == k + 1 ? 1.0 : x[i] * scale);
+    if (g == 0 && tid == 0) {
+      a.e[k] = beta;
+      a.tau[k] = tk;
+    }
+  } else {
+    for (int i = tid; i < n; i += TRD_T) vk[i] = 0.0;
+  }
+  if (g == 0 && tid == 0) a.d[k] = x[k];
+  __syncthreads();
+  if (k > n - 2) return;
+  if (g == 0) {
+    // reflector k into row k of M: no step >= k reads column k, except that
+    // step 0's prologue reads column 0, so reflector 0 waits in v0 until step 1
+    double* row = k == 0 ? a.v0 : a.M + (size_t)k * n;
+    for (int i = tid; i < n; i += TRD_T) row[i] = vk[i];
+    if (k == 1)
+      for (int i = tid; i < n; i += TRD_T) a.M[i] = a.v0[i];
+  }
+  if (qwg) {  // c <- H_k c
+    if (tk != 0.0) {
+      double cr[TRD_R];
+      double part = 0.0;
+#pragma unroll
+      for (int r = 0; r < TRD_R; ++r) {
+        const int i = tid + r * TRD_T;
+        cr[r] = (i > k && i < n) ? a.c[i] : 0.0;
+      }
+#pragma unroll
+      for (int r = 0; r < TRD_R; ++r) {
+        const int i = tid + r * TRD_T;
+        if (i > k && i < n) part += vk[i] * cr[r];
+      }
+      const double dot = trd_block_sum(part, red) * tk;
+#pragma unroll
+      for (int r = 0; r < TRD_R; ++r) {
+        const int i = tid + r * TRD_T;
+        if (i > k && i < n) a.c[i] = cr[r] - dot * vk[i];
+      }
+    }
+    return;
+  }
+  if (!owns) {
+    if (tid == 0) a.pd[(k & 1) * G + g] = 0.0;
+    return;
+  }
+  // ---- owned columns j >= k+1: apply update k-1 to rows >= k+1, y_j, u_j
+  const int cb = k & 1;
+  double ppart = 0.0;
+  for (int j = jw; j < n; j += TRD_W * G) {
+    double* col = a.M + (size_t)j * n;
+    const double vj = vp[j], wj = wp[j];
+    double acc = 0.0, cj = 0.0;
+    for (int base = k + 1; base < n; base += TRD_CP * WAVE) {
+      double cv[TRD_CP];
+#pragma unroll
+      for (int t = 0; t < TRD_CP; ++t) {
+        const int i = base + t * WAVE + lane;
+        cv[t] = i < n ? col[i] : 0.0;
+      }
+#pragma unroll
+      for (int t = 0; t < TRD_CP; ++t) {
+        const int i = base + t * WAVE + lane;
+        if (i < n) {
+          double v = cv[t];
+          if (k >= 1) v -= vp[i] * wj + wp[i] * vj;
+          col[i] = v;
+          acc += v * vk[i];
+          if (i == k + 1) cj = v;
+        }
+      }
+    }
+    const double yj = wave_sum(acc);
+    if (lane == 0) {
+      a.y[(size_t)cb * n + j] = yj;
+      a.u[(size_t)cb * n + j] = cj - tk * yj;
+      ppart += yj * vk[j];
+    }
+  }
+  __syncthreads();
+  if (lane == 0) red[w] = ppart;
+  __syncthreads();
+  if (tid == 0) {
+    double s = 0.0;
+    for (int i = 0; i < TRD_W; ++i) s += red[i];
+    a.pd[cb * G + g] = s;
+  }
+}
+
+// x <- Q x = H_0 (H_1 (... H_{n-2} x)); reflector k in row k of V (zeros at
+// and above k).  x lives in registers; the next reflector is loaded while the
+// current one is applied, so each reflector costs one block reduction.
+__global__ void __launch_bounds__(TRD_T) trd_apply_q(const double* __restrict__ V, const double* __restrict__ tau,
+                                                     int n, double* __restrict__ x) {
+  __shared__ double red[2][TRD_W];
+  const int tid = threadIdx.x, w = tid / WAVE, lane = tid % WAVE;
+  double xr[TRD_R], cur[TRD_R], nxt[TRD_R];
+#pragma unroll
+  for (int r = 0; r < TRD_R; ++r) {
+    const int i = tid + r * TRD_T;
+    xr[r] = i < n ? x[i] : 0.0;
+    cur[r] = (n >= 2 && i < n) ? V[(size_t)(n - 2) * n + i] : 0.0;
+  }
+  int buf = 0;
+  for (int k = n - 2; k >= 0; --k) {
+    const double t = tau[k];
+#pragma unroll
+    for (int r = 0; r < TRD_R; ++r) {
+      const int i = tid + r * TRD_T;
+      nxt[r] = (k > 0 && i < n) ? V[(size_t)(k - 1) * n + i] : 0.0;
+    }
+    if (t != 0.0) {
+      double part = 0.0;
+#pragma unroll
+      for (int r = 0; r < TRD_R; ++r) part += cur[r] * xr[r];
+      part = wave_sum(part);
+      if (lane == 0) red[buf][w] = part;
+      __syncthreads();
+      double dot = 0.0;
+#pragma unroll
+      for (int i = 0; i < TRD_W; ++i) dot += red[buf][i];
+      dot *= t;
+#pragma unroll
+      for (int r = 0; r < TRD_R; ++r) xr[r] -= dot * cur[r];
+      buf ^= 1;
+    }
+#pragma unroll
+    for (int r = 0; r < TRD_R; ++r) cur[r] = nxt[r];
+  }
+#pragma unroll
+  for (int r = 0; r < TRD_R; ++r) {
+    const int i = tid + r * TRD_T;
+    if (i < n) x[i] = xr[r];
+  }
+}
+
+}  // namespace
+
+// Largest matrix: an n-vector is TRD_R elements per thread.
+constexpr int TRD_MAX_N = TRD_T * TRD_R;
+
+int trd_groups(int n) {
+  static int v = [] {
+    const char* s = getenv("VMC_JAX_AMD_TRIDIAG_GROUPS");
+    return s ? atoi(s) : 0;
+  }();
+  return v > 0 ? std::min(std::min(v, 1024), std::max(1, n / 2)) : std::max(1, std::min(256, n / 4));
+}
+
+int trd_min_n() {
+  static int v = [] {
+    const char* s = getenv("VMC_JAX_AMD_TRIDIAG_MIN_N");
+    return s ? atoi(s) : 256;
+  }();
+  return v;
+}
+
+bool trd_supported(int64_t n) { return n >= trd_min_n() && n >= 3 && n <= TRD_MAX_N; }
+
+// Eigen-solve of the real symmetric n x n matrix S (destroyed) restricted to
+// what the pseudo-inverse needs: ev = eigenvalues (ascending), c (length n,
+// in/out) = Z^T Q^T c, with Z (n x n, row j = eigenvector j of T) left in Zbuf,
+// and the reflectors in S for trd_apply.  Workspaces come from the caller.
+int trd_eig_solve(double* S, int n, double* ev, double* c, double* Zbuf, double* work, hipStream_t st) {
+  const int G = trd_groups(n);
+  TrdArgs a;
+  a.M = S;
+  a.d = ev;
+  a.e = work;
+  a.tau = work + n;
+  a.y = work + 2 * (size_t)n;
+  a.u = work + 4 * (size_t)n;
+  a.v0 = work + 6 * (size_t)n;
+  a.pd = work + 7 * (size_t)n;
+  a.c = c;
+  a.n = n;
+  a.G = G;
+  const size_t lds = sizeof(double) * (4 * (size_t)n + 2 * TRD_W);
+  if (lds > 64 * 1024) {
+    hipError_t e = hipFuncSetAttribute((const void*)trd_step, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute(trd_step)");
+  }
+  {
+    ScopedTimer tm("tridiag", st);
+    for (int k = 0; k < n; ++k) {
+      hipLaunchKernelGGL(trd_step, dim3((unsigned)(G + 1)), dim3(TRD_T), lds, st, a, k);
+    }
+    JV_LAUNCH_CHECK("trd_step");
+  }
+  // eigenvectors of T (rocSOLVER divide and conquer): column-major Z, so row j
+  // of the row-major buffer is eigenvector j
+  rocblas_handle h = blas_handle(st);
+  if (!h) return fail(JV_E_SOLVER, "rocblas_create_handle failed");
+  rocblas_int* info = (rocblas_int*)(work + 7 * (size_t)n + 2 * G + 8);
+  {
+    ScopedTimer tm("stedc", st);
+    rocblas_status s = rocsolver_dstedc(h, rocblas_evect_tridiagonal, n, ev, a.e, Zbuf, n, info);
+    if (s != rocblas_status_success)
+      return fail(JV_E_SOLVER, "rocsolver_dstedc status " + std::to_string((int)s));
+  }
+  rocblas_int hinfo = 0;
+  JV_HIP(hipMemcpyAsync(&hinfo, info, sizeof(hinfo), hipMemcpyDeviceToHost, st));
+  JV_HIP(hipStreamSynchronize(st));
+  if (hinfo != 0) return fail(JV_E_SOLVER, "tridiagonal eigensolver did not converge (info " + std::to_string(hinfo) + ")");
+  return JV_OK;
+}
+
+int trd_apply(const double* V, const double* tau, int n, double* x, hipStream_t st) {
+  hipLaunchKernelGGL(trd_apply_q, dim3(1), dim3(TRD_T), 0, st, V, tau, n, x);
+  JV_LAUNCH_CHECK("trd_apply_q");
+  return JV_OK;
+}
+
+size_t trd_work_doubles(int n) { return 7 * (size_t)n + 2 * (size_t)trd_groups(n) + 64; }
+
+}  // namespace jv
