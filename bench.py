@@ -129,7 +129,7 @@ def timer(jv, name):
     return tot.value, cnt.value
 
 
-KERNELS = ("gram", "gemm", "eigh", "tridiag", "stedc", "eloc", "mc_sweep", "net_grad", "net_forward")
+KERNELS = ("gram", "gemm", "eigh", "tridiag", "stedc", "trd_apply", "eloc", "mc_sweep", "net_grad", "net_forward")
 
 
 def run_gpu(workload, steps, warmup, mode):

@@ -180,10 +180,26 @@ __device__ inline void mc_draw(uint64_t seed, uint32_t step, uint32_t chain, uin
 }
 
 // ------------------------------------------------------------------ reductions
+// Wave-wide sum with DPP lane moves (no LDS round trips): a Hillis-Steele
+// scan inside each 16-lane row (row_shr 1, 2, 4, 8), then row_bcast 15 / 31
+// carry the row totals to lane 63, whose value every lane receives.  The
+// association order is fixed, so the result is deterministic and uniform.
+template <int CTRL, int ROWS>
+__device__ inline double dpp_shift(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWS, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWS, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
 __device__ inline double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+  v += dpp_shift<0x111, 0xf>(v);  // row_shr:1
+  v += dpp_shift<0x112, 0xf>(v);  // row_shr:2
+  v += dpp_shift<0x114, 0xf>(v);  // row_shr:4
+  v += dpp_shift<0x118, 0xf>(v);  // row_shr:8
+  v += dpp_shift<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+  v += dpp_shift<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+  return __hiloint2double(hi, lo);
 }
 
 // Block-wide sum; red must hold blockDim.x / 64 doubles.  All threads get the sum.

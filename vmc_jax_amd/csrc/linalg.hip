@@ -14,7 +14,8 @@ namespace jv {
 bool trd_supported(int64_t n);
 size_t trd_work_doubles(int n);
 int trd_eig_solve(double* S, int n, double* ev, double* c, double* Zbuf, double* work, hipStream_t st);
-int trd_apply(const double* V, const double* tau, int n, double* x, hipStream_t st);
+int trd_apply(const double* V, const double* tau, int n, double* x, double* Tbuf, hipStream_t st);
+size_t trd_wy_offset(int n);
 
 rocblas_handle blas_handle(hipStream_t stream);
 
@@ -927,7 +928,7 @@ int jv_minsr_solve_dual(const double* data, const double* e, int64_t n, int64_t 
     hipLaunchKernelGGL(pinv_scale, dim3(1), dim3(1024), 0, st, ev, p, rcond, 0, y, rank);
     JV_LAUNCH_CHECK("pinv_scale");
     if ((rc = gemv(true, Z, y, p, p, 0, 1, 1.0, x, st, 12))) return rc;
-    if ((rc = trd_apply(S, tw + p, (int)p, x, st))) return rc;
+    if ((rc = trd_apply(S, tw + p, (int)p, x, tw + trd_wy_offset((int)p), st))) return rc;
   } else {
     if ((rc = eigh_raw(p, cpx, S, ev, st))) return rc;
     if ((rc = gemv(false, S, b, p, p, cpx, 0, 1.0, y, st, 12))) return rc;

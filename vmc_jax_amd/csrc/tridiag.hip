@@ -241,45 +241,152 @@ __global__ void __launch_bounds__(TRD_T) trd_step(TrdArgs a, int k) {
   }
 }
 
-// x <- Q x = H_0 (H_1 (... H_{n-2} x)); reflector k in row k of V (zeros at
-// and above k).  x lives in registers; the next reflector is loaded while the
-// current one is applied, so each reflector costs one block reduction.
-__global__ void __launch_bounds__(TRD_T) trd_apply_q(const double* __restrict__ V, const double* __restrict__ tau,
-                                                     int n, double* __restrict__ x) {
-  __shared__ double red[2][TRD_W];
+// ---- compact WY form of reflector blocks: H_k0 ... H_k1 = I - V T V^T with
+// V = [v_k0 .. v_k1] and T upper triangular (LAPACK dlarft, forward, columnwise)
+constexpr int WY_B = 32;
+
+// T of reflector block blockIdx.x (reflectors blockIdx.x*WY_B ..), from the
+// Gram Y = V^T V accumulated over 256-row chunks staged in LDS.
+__global__ void __launch_bounds__(256) trd_wy_t(const double* __restrict__ V, const double* __restrict__ tau,
+                                                int n, int nref, double* __restrict__ Tout) {
+  __shared__ double Vs[WY_B][257];
+  __shared__ double Y[WY_B][WY_B + 1];
+  __shared__ double T[WY_B][WY_B + 1];
+  const int tid = threadIdx.x;
+  const int k0 = blockIdx.x * WY_B, nb = min(WY_B, nref - k0);
+  // pairs (p, q), p <= q, two per thread (528 pairs for WY_B = 32)
+  double acc[3] = {0.0, 0.0, 0.0};
+  int pp[3], qq[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    int t = tid + r * 256, p = 0;
+    while (p < WY_B && t >= WY_B - p) {
+      t -= WY_B - p;
+      ++p;
+    }
+    pp[r] = p < WY_B ? p : -1;
+    qq[r] = p < WY_B ? p + t : -1;
+  }
+  for (int c0 = k0 + 1; c0 < n; c0 += 256) {
+    __syncthreads();
+    double st[WY_B];
+    const int ic = min(c0 + tid, n - 1);
+#pragma unroll
+    for (int p = 0; p < WY_B; ++p) st[p] = V[(size_t)min(k0 + p, n - 1) * n + ic];
+#pragma unroll
+    for (int p = 0; p < WY_B; ++p) Vs[p][tid] = (p < nb && c0 + tid < n) ? st[p] : 0.0;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      if (pp[r] < 0) continue;
+      double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+      for (int i = 0; i < 256; i += 4) {
+        a0 = fma(Vs[pp[r]][i], Vs[qq[r]][i], a0);
+        a1 = fma(Vs[pp[r]][i + 1], Vs[qq[r]][i + 1], a1);
+        a2 = fma(Vs[pp[r]][i + 2], Vs[qq[r]][i + 2], a2);
+        a3 = fma(Vs[pp[r]][i + 3], Vs[qq[r]][i + 3], a3);
+      }
+      acc[r] += (a0 + a1) + (a2 + a3);
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+    if (pp[r] >= 0) {
+      Y[pp[r]][qq[r]] = acc[r];
+      Y[qq[r]][pp[r]] = acc[r];
+    }
+  __syncthreads();
+  if (tid < WAVE) {
+    const int j = tid;
+    for (int i = 0; i < WY_B; ++i) {
+      const double ti = i < nb ? tau[k0 + i] : 0.0;
+      double v = 0.0;
+      if (j < i) {
+        for (int l = j; l < i; ++l) v += T[j][l] * Y[l][i];
+        v *= -ti;
+      }
+      __builtin_amdgcn_wave_barrier();
+      if (j < WY_B) T[j][i] = j < i ? v : (j == i ? ti : 0.0);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < WY_B * WY_B; e += 256) Tout[(size_t)blockIdx.x * WY_B * WY_B + e] = T[e / WY_B][e % WY_B];
+}
+
+// x <- Q x, blocks applied last to first: x -= V (T (V^T x)).  The reflector
+// loop is unrolled to WY_B so each thread's loads of a block are issued
+// together (one memory latency per phase, not one per reflector).
+__global__ void __launch_bounds__(TRD_T) trd_apply_wy(const double* __restrict__ V, const double* __restrict__ Tg,
+                                                      int n, int nref, double* __restrict__ x) {
+  __shared__ double part[TRD_W][WY_B];
+  __shared__ double wv[WY_B], tv[WY_B];
   const int tid = threadIdx.x, w = tid / WAVE, lane = tid % WAVE;
-  double xr[TRD_R], cur[TRD_R], nxt[TRD_R];
+  double xr[TRD_R];
 #pragma unroll
   for (int r = 0; r < TRD_R; ++r) {
     const int i = tid + r * TRD_T;
     xr[r] = i < n ? x[i] : 0.0;
-    cur[r] = (n >= 2 && i < n) ? V[(size_t)(n - 2) * n + i] : 0.0;
   }
-  int buf = 0;
-  for (int k = n - 2; k >= 0; --k) {
-    const double t = tau[k];
+  const int nblk = (nref + WY_B - 1) / WY_B;
+  for (int b = nblk - 1; b >= 0; --b) {
+    const int k0 = b * WY_B, nb = min(WY_B, nref - k0);
+    // w = V^T x
+    double acc[WY_B];
+#pragma unroll
+    for (int p = 0; p < WY_B; ++p) acc[p] = 0.0;
+    // loads use clamped (always valid) indices instead of branches, so the
+    // WY_B loads of one r are in flight together
 #pragma unroll
     for (int r = 0; r < TRD_R; ++r) {
+      if (r * TRD_T >= n) break;
       const int i = tid + r * TRD_T;
-      nxt[r] = (k > 0 && i < n) ? V[(size_t)(k - 1) * n + i] : 0.0;
-    }
-    if (t != 0.0) {
-      double part = 0.0;
+      const int ic = i < n ? i : n - 1;
+      const double xv = i < n ? xr[r] : 0.0;
 #pragma unroll
-      for (int r = 0; r < TRD_R; ++r) part += cur[r] * xr[r];
-      part = wave_sum(part);
-      if (lane == 0) red[buf][w] = part;
-      __syncthreads();
-      double dot = 0.0;
-#pragma unroll
-      for (int i = 0; i < TRD_W; ++i) dot += red[buf][i];
-      dot *= t;
-#pragma unroll
-      for (int r = 0; r < TRD_R; ++r) xr[r] -= dot * cur[r];
-      buf ^= 1;
+      for (int p = 0; p < WY_B; ++p) acc[p] = fma(V[(size_t)min(k0 + p, n - 1) * n + ic], xv, acc[p]);
     }
 #pragma unroll
-    for (int r = 0; r < TRD_R; ++r) cur[r] = nxt[r];
+    for (int p = 0; p < WY_B; ++p) {
+      const double t = wave_sum(acc[p]);
+      if (lane == 0) part[w][p] = t;
+    }
+    __syncthreads();
+    if (tid < nb) {
+      double t = 0.0;
+#pragma unroll
+      for (int q = 0; q < TRD_W; ++q) t += part[q][tid];
+      wv[tid] = t;
+    }
+    __syncthreads();
+    // t = T w (zero past the block's last reflector, whose rows are never
+    // weighted: acc[p >= nb] is ignored and tv[p >= nb] = 0)
+    if (tid >= nb && tid < WY_B) tv[tid] = 0.0;
+    if (tid < nb) {
+      const double* Tb = Tg + (size_t)b * WY_B * WY_B + (size_t)tid * WY_B;
+      double tr[WY_B];
+#pragma unroll
+      for (int l = 0; l < WY_B; ++l) tr[l] = Tb[l];
+      double t = 0.0;
+#pragma unroll
+      for (int l = 0; l < WY_B; ++l)
+        if (l >= tid && l < nb) t = fma(tr[l], wv[l], t);
+      tv[tid] = t;
+    }
+    __syncthreads();
+    // x -= V t
+#pragma unroll
+    for (int r = 0; r < TRD_R; ++r) {
+      if (r * TRD_T >= n) break;
+      const int i = tid + r * TRD_T;
+      const int ic = i < n ? i : n - 1;
+      double t = 0.0;
+#pragma unroll
+      for (int p = 0; p < WY_B; ++p) t = fma(V[(size_t)min(k0 + p, n - 1) * n + ic], tv[p], t);
+      xr[r] -= i < n ? t : 0.0;
+    }
   }
 #pragma unroll
   for (int r = 0; r < TRD_R; ++r) {
@@ -359,12 +466,25 @@ int trd_eig_solve(double* S, int n, double* ev, double* c, double* Zbuf, double*
   return JV_OK;
 }
 
-int trd_apply(const double* V, const double* tau, int n, double* x, hipStream_t st) {
-  hipLaunchKernelGGL(trd_apply_q, dim3(1), dim3(TRD_T), 0, st, V, tau, n, x);
-  JV_LAUNCH_CHECK("trd_apply_q");
+int trd_wy_blocks(int n) { return (n - 1 + WY_B - 1) / WY_B; }
+
+// x <- Q x with the reflectors in the rows of V and tau; Tbuf holds
+// trd_wy_blocks(n) * WY_B^2 doubles.
+int trd_apply(const double* V, const double* tau, int n, double* x, double* Tbuf, hipStream_t st) {
+  const int nref = n - 1, nblk = trd_wy_blocks(n);
+  ScopedTimer tm("trd_apply", st);
+  hipLaunchKernelGGL(trd_wy_t, dim3((unsigned)nblk), dim3(256), 0, st, V, tau, n, nref, Tbuf);
+  JV_LAUNCH_CHECK("trd_wy_t");
+  hipLaunchKernelGGL(trd_apply_wy, dim3(1), dim3(TRD_T), 0, st, V, Tbuf, n, nref, x);
+  JV_LAUNCH_CHECK("trd_apply_wy");
   return JV_OK;
 }
 
-size_t trd_work_doubles(int n) { return 7 * (size_t)n + 2 * (size_t)trd_groups(n) + 64; }
+size_t trd_work_doubles(int n) {
+  return 7 * (size_t)n + 2 * (size_t)trd_groups(n) + 64 + (size_t)trd_wy_blocks(n) * WY_B * WY_B;
+}
+
+// Offset (doubles) of the WY T factors inside the trd workspace.
+size_t trd_wy_offset(int n) { return 7 * (size_t)n + 2 * (size_t)trd_groups(n) + 64; }
 
 }  // namespace jv
