@@ -158,7 +158,8 @@ def run_gpu(workload, steps, warmup, mode):
 # workload, from rocprofv3 FETCH_SIZE / WRITE_SIZE passes scaled by a copy
 # kernel of known size (x2.000 read, x1.000 written); the summary with the
 # commands is profiles/eloc_mc_gram_tfim10x10_cnn_hbm_traffic.md.
-MEASURED_TRAFFIC = {"eloc": 6.204e6 + 0.262e6, "mc_sweep": 0.763e6 + 3.653e6, "gram": 651.451e6 + 12.313e6}
+MEASURED_TRAFFIC = {"eloc": 6.203e6 + 0.262e6, "mc_sweep": 0.767e6 + 3.654e6,
+                    "gram": 673.600e6 + 51.710e6 + 50.189e6 + 32.478e6}
 
 
 def roofline_c3(kt, steps, mode):
