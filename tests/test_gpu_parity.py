@@ -298,6 +298,34 @@ def test_center_and_grams(cpx):
 
 
 @pytest.mark.parametrize("cpx", [False, True])
+def test_large_products_through_blas(cpx):
+    """Plain products above 2^31 flops run on rocBLAS (deterministic mode): a
+    GEMM with transposed second operand (the SNR projection's layout) and the
+    covariance product, against numpy; the Hermitian Grams at the same sizes
+    stay on the MFMA kernel."""
+    rng = np.random.default_rng(33)
+    z = (lambda *s: rng.normal(size=s) + 1j * rng.normal(size=s)) if cpx else (lambda *s: rng.normal(size=s))
+    M, N, K = (1536, 768, 1024) if not cpx else (1024, 512, 512)
+    A, B = z(M, K), z(N, K)
+    C = torch.empty((M, N) + ((2,) if cpx else ()), dtype=torch.float64, device="cuda")
+    Ad, Bd = dev(A).contiguous(), dev(B).contiguous()
+    if cpx:
+        Ad, Bd = torch.view_as_real(Ad).contiguous(), torch.view_as_real(Bd).contiguous()
+    jv._lib.call("jv_gemm", M, N, K, jv._lib.ptr(Ad), K, 1, 0, jv._lib.ptr(Bd), 1, K, 0, jv._lib.ptr(C), N,
+                 int(cpx), 0.5, jv._lib.stream_ptr())
+    got = to_np(torch.view_as_complex(C) if cpx else C)
+    assert rel(got, 0.5 * A @ B.T) < 1e-13
+    n, pp = (4096, 512) if not cpx else (2048, 512)
+    O = z(n, pp)
+    o = jv.stats.SampledObs(dev(O[None]), dev(np.full((1, n), 1.0 / n)))
+    oo = OObs(O, np.full(n, 1.0 / n))
+    assert rel(to_np(o.covar()), oo.covar()) < 1e-12
+    assert rel(to_np(o.tangent_kernel()), oo.tangent_kernel()) < 1e-12
+    # deterministic: a second evaluation is bit-identical
+    assert np.array_equal(to_np(o.covar()), to_np(o.covar()))
+
+
+@pytest.mark.parametrize("cpx", [False, True])
 def test_eigh(cpx):
     rng = np.random.default_rng(21)
     n = 200

@@ -104,6 +104,8 @@ rocblas_handle blas_handle(hipStream_t stream) {
   DeviceState& s = state();
   if (!s.blas) {
     if (rocblas_create_handle(&s.blas) != rocblas_status_success) return nullptr;
+    // deterministic kernels only (no atomic split-K): results must not depend on scheduling
+    rocblas_set_atomics_mode(s.blas, rocblas_atomics_not_allowed);
   }
   rocblas_set_stream(s.blas, stream);
   return s.blas;

@@ -4,6 +4,7 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <cmath>
 #include <vector>
 
 #include "common.hpp"
@@ -267,11 +268,73 @@ __global__ void gemm_reduce(GemmArgs g, int splits) {
   }
 }
 
+// Large plain products go to rocBLAS (Tensile reaches 85-90 % of the FP64
+// MFMA peak on them, e.g. the 16384 x 4224 x 4224 SNR projection); the
+// hand-written kernel keeps the Hermitian Grams (triangle tiles + split K beat
+// a full rocBLAS gemm, which computes both triangles) and small products.  Row-major C = opA opB
+// is computed as the column-major C^T = opB^T opA^T.  Returns 1 when the
+// strides or conjugations have no BLAS form.
+int gemm_blas(const GemmArgs& g, hipStream_t st) {
+  const bool cpx = g.cpx != 0;
+  rocblas_operation tx, ty;
+  int64_t ldx, ldy;
+  if (g.sbn == 1 && !g.conjB) {
+    tx = rocblas_operation_none;
+    ldx = g.sbk;
+  } else if (g.sbk == 1) {
+    tx = g.conjB ? rocblas_operation_conjugate_transpose : rocblas_operation_transpose;
+    ldx = g.sbn;
+  } else {
+    return 1;
+  }
+  if (g.sak == 1 && !g.conjA) {
+    ty = rocblas_operation_none;
+    ldy = g.sam;
+  } else if (g.sam == 1) {
+    ty = g.conjA ? rocblas_operation_conjugate_transpose : rocblas_operation_transpose;
+    ldy = g.sak;
+  } else {
+    return 1;
+  }
+  rocblas_handle h = blas_handle(st);
+  if (!h) return 1;
+  rocblas_status s;
+  if (cpx) {
+    const rocblas_double_complex al = {g.alpha, 0.0}, be = {0.0, 0.0};
+    s = rocblas_zgemm(h, tx, ty, (rocblas_int)g.N, (rocblas_int)g.M, (rocblas_int)g.K, &al,
+                      (const rocblas_double_complex*)g.B, (rocblas_int)ldx, (const rocblas_double_complex*)g.A,
+                      (rocblas_int)ldy, &be, (rocblas_double_complex*)g.C, (rocblas_int)g.ldc);
+  } else {
+    const double al = g.alpha, be = 0.0;
+    s = rocblas_dgemm(h, tx, ty, (rocblas_int)g.N, (rocblas_int)g.M, (rocblas_int)g.K, &al, g.B, (rocblas_int)ldx,
+                      g.A, (rocblas_int)ldy, &be, g.C, (rocblas_int)g.ldc);
+  }
+  return s == rocblas_status_success ? 0 : -1;
+}
+
+int gemm_blas_min_flops_log2() {
+  static int v = [] {
+    const char* s = getenv("VMC_JAX_AMD_BLAS_MIN_FLOPS_LOG2");
+    return s ? atoi(s) : 31;  // 2^31 flops
+  }();
+  return v;
+}
+
 int gemm(const GemmArgs& g0, hipStream_t st) {
   GemmArgs g = g0;
   g.part = nullptr;
   g.kchunk = 0;
   if (g.M <= 0 || g.N <= 0) return JV_OK;
+  {
+    const double flops = 2.0 * (double)g.M * (double)g.N * (double)g.K * (g.cpx ? 4.0 : 1.0);
+    const bool ours = g.sym != 0;  // Hermitian Grams: triangle tiles halve the work
+    if (!ours && flops >= std::ldexp(1.0, gemm_blas_min_flops_log2())) {
+      ScopedTimer tm(g.sym ? "gram" : "gemm", st);
+      const int r = gemm_blas(g, st);
+      if (r == 0) return JV_OK;
+      if (r < 0) return fail(JV_E_SOLVER, "rocblas gemm failed");
+    }
+  }
   const int64_t ntm = (g.M + BM - 1) / BM, ntn = (g.N + BN - 1) / BN;
   int64_t blocks = g.sym ? ntn * (ntn + 1) / 2 : ntm * ntn;
   if (g.sym && (g.M != g.N)) return fail(JV_E_ARG, "symmetric GEMM needs a square output");
