@@ -325,10 +325,11 @@ def test_large_products_through_blas(cpx):
     assert np.array_equal(to_np(o.covar()), to_np(o.covar()))
 
 
-@pytest.mark.parametrize("cpx", [False, True])
-def test_eigh(cpx):
+@pytest.mark.parametrize("cpx,n", [(False, 200), (True, 200), (False, 300), (False, 777)])
+def test_eigh(cpx, n):
+    """rocSOLVER below 256 (and for complex matrices); the Householder
+    tridiagonalisation + divide and conquer + dormtr path from 256 up."""
     rng = np.random.default_rng(21)
-    n = 200
     A = rng.normal(size=(n, n)) + (1j * rng.normal(size=(n, n)) if cpx else 0)
     A = A + A.conj().T
     Ad = dev(A).contiguous()
@@ -386,13 +387,14 @@ def test_minsr_holomorphic_rbm():
     assert abs(eq.get_energy_mean() - eqo.get_energy_mean()) < 1e-11
 
 
-@pytest.mark.parametrize("mode", ["real", "imag"])
-def test_tdvp_exact(mode):
+@pytest.mark.parametrize("mode,H", [("real", 8), ("imag", 8), ("real", 16)])
+def test_tdvp_exact(mode, H):
+    """H = 16 gives Pr = 304: the real S goes through the tridiagonal path."""
     rng = np.random.default_rng(24)
     L = 8
-    onet = onets.CpxRBM((L,), numHidden=8, bias=True, visibleBias=True)
+    onet = onets.CpxRBM((L,), numHidden=H, bias=True, visibleBias=True)
     P = 0.1 * rng.normal(size=onet.num_real_params())
-    psi = make_psi(jv.nets.CpxRBM(numHidden=8, bias=True, visibleBias=True), (L,), P)
+    psi = make_psi(jv.nets.CpxRBM(numHidden=H, bias=True, visibleBias=True), (L,), P)
     opsi = onets.NQS(onet, P)
     H, Ho = tfim_pair((L,), -1.0, -0.7)
     es, eso = jv.sampler.ExactSampler(psi, L), osamp.ExactSampler(opsi, L)

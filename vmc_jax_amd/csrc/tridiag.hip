@@ -466,6 +466,31 @@ int trd_eig_solve(double* S, int n, double* ev, double* c, double* Zbuf, double*
   return JV_OK;
 }
 
+size_t trd_work_doubles(int n);
+
+// Full eigendecomposition of the real symmetric S (row-major, destroyed):
+// ev ascending, S overwritten with the eigenvectors as rows (the same
+// convention as the rocSOLVER path).  The reflectors sit where LAPACK's dsytrd
+// ('L') stores them (v_k(k+2:n) in column k, tau_k), so rocSOLVER's dormtr
+// forms Q Z from the tridiagonal eigenvectors Z.
+int trd_eigh_full(double* S, int n, double* ev, hipStream_t st) {
+  double* Z = (double*)workspace(sizeof(double) * (size_t)n * n, 16);
+  double* tw = (double*)workspace(sizeof(double) * trd_work_doubles(n), 17);
+  if (!Z || !tw) return fail(JV_E_NOMEM, "eigh: workspace allocation failed");
+  int rc = trd_eig_solve(S, n, ev, nullptr, Z, tw, st);
+  if (rc) return rc;
+  rocblas_handle h = blas_handle(st);
+  if (!h) return fail(JV_E_SOLVER, "rocblas_create_handle failed");
+  {
+    ScopedTimer tm("ormtr", st);
+    rocblas_status s = rocsolver_dormtr(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, n, n, S, n,
+                                        tw + n, Z, n);
+    if (s != rocblas_status_success) return fail(JV_E_SOLVER, "rocsolver_dormtr status " + std::to_string((int)s));
+  }
+  JV_HIP(hipMemcpyAsync(S, Z, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToDevice, st));
+  return JV_OK;
+}
+
 int trd_wy_blocks(int n) { return (n - 1 + WY_B - 1) / WY_B; }
 
 // x <- Q x with the reflectors in the rows of V and tau; Tbuf holds
