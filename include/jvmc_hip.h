@@ -88,6 +88,25 @@ typedef struct jv_optable {
   const int32_t* diag;       /* [num_diag] indices of all-diagonal strings */
 } jv_optable;
 
+/* Symmetrisation of a network over an orbit of lattice symmetries (SymNet,
+ * sym_wrapper.py:29-83).  The orbit matrices of symmetries.py:75-206 are signed
+ * permutations; element g is stored as a site map: (g s)_i = s_j for
+ * map[g][i] = j >= 0 and 1 - s_j for map[g][i] = -1 - j, which is
+ * (o (2s - 1) + 1) // 2 of sym_wrapper.py:70-73 for a matrix row with a single
+ * entry +1 (-1) in column j.  All pointers are device pointers. */
+#define JV_AVG_EXP 0  /* avgFun_Coefficients_Exp: logsumexp(c, b = factor) (sym_wrapper.py:7-9) */
+#define JV_AVG_LOG 1  /* avgFun_Coefficients_Log: mean(c) (sym_wrapper.py:12-13) */
+#define JV_AVG_SEP 2  /* avgFun_Coefficients_Sep (sym_wrapper.py:16-19) */
+typedef struct jv_orbit {
+  int32_t num_elements;    /* |orbit| */
+  int32_t num_sites;       /* must equal the network's num_sites */
+  int32_t avg;             /* JV_AVG_* */
+  int32_t complex_factor;  /* factors have a complex dtype: log psi promotes to complex
+                              (jax logsumexp type promotion) */
+  const int32_t* map;      /* [num_elements][num_sites] */
+  const double* factor;    /* complex [num_elements] */
+} jv_orbit;
+
 /* Metropolis arguments (sampler.py:103-104, 195-358). */
 typedef struct jv_mc_args {
   int32_t num_chains;
@@ -146,6 +165,43 @@ int jv_net_forward(const jv_net* net, const double* params, const int32_t* confi
  * Replaces NQS.gradients / NQS._get_gradients (vqs.py:256-287). */
 int jv_net_gradients(const jv_net* net, const double* params, const int32_t* configs,
                      int64_t n, int32_t layout, double* grad, double* logpsi, void* stream);
+
+/* ------------------------------------------------------------------ symmetrised networks */
+/* log psi(s) = avg_g log psi_net(g s) (complex [n]).  EXP: log(sum_g b_g
+ * exp(c_g)) with the jax logsumexp conventions: entries with b_g = 0 are
+ * dropped; for a real network with real factors the result is log|sum| and NaN
+ * where the sum is negative.  LOG: the mean.  SEP: 0.5 log mean exp(2 Re c) +
+ * i angle(mean exp(i Im c)).
+ * Replaces NQS.__call__ with SymNet.__call__ (sym_wrapper.py:66-77, vqs.py:223-251). */
+int jv_sym_forward(const jv_net* net, const double* params, const jv_orbit* orbit,
+                   const int32_t* configs, int64_t n, double* logpsi, void* stream);
+
+/* Per-sample gradients of the symmetrised log psi: O(s) = sum_g w_g O_net(g s),
+ * w_g the derivative of the average with respect to c_g (EXP: b_g exp(c_g) /
+ * sum; LOG: 1/|orbit|; SEP: the softmax of 2 Re c, real-parameter networks
+ * only; real EXP samples whose sum is negative get w = 0, as jax's where()).
+ * Layouts as jv_net_gradients, except that layout 0 of a real network is
+ * complex [n][num_params] when the weights are complex (EXP with complex
+ * factors).  Replaces NQS.gradients for a SymNet (vqs.py:40-82, 256-287). */
+int jv_sym_gradients(const jv_net* net, const double* params, const jv_orbit* orbit,
+                     const int32_t* configs, int64_t n, int32_t layout, double* grad,
+                     double* logpsi, void* stream);
+
+/* O_loc(s) = sum_j m_j exp(log psi(s'_j) - logpsi_s) for a symmetrised network:
+ * s' materialised per sample chunk (jv_get_s_primes), log psi(s') from
+ * jv_sym_forward.  Replaces Operator.get_O_loc (base.py:162-192) for a SymNet. */
+int jv_sym_eloc(const jv_net* net, const double* params, const jv_orbit* orbit,
+                const jv_optable* op, const double* prefactor, const int32_t* configs,
+                const double* logpsi_s, int64_t n, double* out, void* stream);
+
+/* Metropolis sampling of a symmetrised network with single-spin-flip proposals:
+ * the same arguments, Philox draws and emission order as jv_mc_sample; each
+ * proposal is evaluated with jv_sym_forward on all chains at once and accepted
+ * with u < exp(mu Re(log psi' - log psi)).
+ * Replaces MCSampler._sweep / _get_samples_mcmc (sampler.py:264-367) for a SymNet. */
+int jv_sym_mc_sample(const jv_net* net, const double* params, const jv_orbit* orbit,
+                     const jv_mc_args* args, int32_t* walkers, int32_t* configs, double* logpsi,
+                     int64_t* counters, void* stream);
 
 /* ------------------------------------------------------------------ operators */
 /* s' enumeration with nonzero compaction.  Outputs, per sample, all num_ops

@@ -42,6 +42,16 @@ class JvOpTable(ctypes.Structure):
                 ("max_sites_changed", _i32), ("reserved", _i32), ("idx", _vp), ("map", _vp), ("matel", _vp), ("diag", _vp)]
 
 
+JV_AVG_EXP = 0
+JV_AVG_LOG = 1
+JV_AVG_SEP = 2
+
+
+class JvOrbit(ctypes.Structure):
+    _fields_ = [("num_elements", _i32), ("num_sites", _i32), ("avg", _i32), ("complex_factor", _i32),
+                ("map", _vp), ("factor", _vp)]
+
+
 class JvMcArgs(ctypes.Structure):
     _fields_ = [("num_chains", _i32), ("therm_steps", _i32), ("samples_per_chain", _i32),
                 ("sweep_steps", _i32), ("mu", _dbl), ("seed", ctypes.c_uint64),
@@ -64,6 +74,12 @@ _SIGS = {
     "jv_net_init": [ctypes.POINTER(JvNet)],
     "jv_net_forward": [ctypes.POINTER(JvNet), _vp, _vp, _i64, _vp, _vp],
     "jv_net_gradients": [ctypes.POINTER(JvNet), _vp, _vp, _i64, _i32, _vp, _vp, _vp],
+    "jv_sym_forward": [ctypes.POINTER(JvNet), _vp, ctypes.POINTER(JvOrbit), _vp, _i64, _vp, _vp],
+    "jv_sym_gradients": [ctypes.POINTER(JvNet), _vp, ctypes.POINTER(JvOrbit), _vp, _i64, _i32, _vp, _vp, _vp],
+    "jv_sym_eloc": [ctypes.POINTER(JvNet), _vp, ctypes.POINTER(JvOrbit), ctypes.POINTER(JvOpTable), _vp, _vp, _vp,
+                    _i64, _vp, _vp],
+    "jv_sym_mc_sample": [ctypes.POINTER(JvNet), _vp, ctypes.POINTER(JvOrbit), ctypes.POINTER(JvMcArgs), _vp, _vp,
+                         _vp, _vp, _vp],
     "jv_get_s_primes": [ctypes.POINTER(JvOpTable), _vp, _vp, _i64, _i32, _vp, _vp, _vp, _vp],
     "jv_eloc": [ctypes.POINTER(JvNet), _vp, ctypes.POINTER(JvOpTable), _vp, _vp, _vp, _i64, _vp, _vp],
     "jv_mc_sample": [ctypes.POINTER(JvNet), _vp, ctypes.POINTER(JvMcArgs), _vp, _vp, _vp, _vp, _vp],

@@ -11,6 +11,8 @@ import numpy as np
 from . import activation_functions
 from .activation_functions import log_cosh
 from .. import _lib
+from . import sym_wrapper
+from .sym_wrapper import SymNet
 
 
 class _NetBase:
@@ -149,4 +151,4 @@ class CNN(_NetBase):
         return P
 
 
-__all__ = ["CpxRBM", "RBM", "CNN", "activation_functions", "log_cosh"]
+__all__ = ["CpxRBM", "RBM", "CNN", "SymNet", "sym_wrapper", "activation_functions", "log_cosh"]

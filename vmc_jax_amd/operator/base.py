@@ -65,8 +65,13 @@ class Operator(metaclass=abc.ABCMeta):
         n = flat.shape[0]
         lps = logPsiS.reshape(-1).to(torch.complex128).contiguous()
         out = torch.empty(n, dtype=torch.complex128, device=flat.device)
-        _lib.call("jv_eloc", psi.desc, _lib.ptr(psi.params), self.table, _lib.ptr(pref), _lib.ptr(flat),
-                  _lib.ptr(lps), n, _lib.ptr(out), _lib.stream_ptr())
+        if psi._sym is not None:
+            # symmetrised networks: s' materialised per chunk, log psi(s') over the orbit
+            _lib.call("jv_sym_eloc", psi.desc, _lib.ptr(psi.params), psi._sym[0], self.table, _lib.ptr(pref),
+                      _lib.ptr(flat), _lib.ptr(lps), n, _lib.ptr(out), _lib.stream_ptr())
+        else:
+            _lib.call("jv_eloc", psi.desc, _lib.ptr(psi.params), self.table, _lib.ptr(pref), _lib.ptr(flat),
+                      _lib.ptr(lps), n, _lib.ptr(out), _lib.stream_ptr())
         return out.reshape(1, n)
 
     def get_O_loc_batched(self, samples, psi, logPsiS, batchSize, *args):

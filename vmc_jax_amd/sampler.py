@@ -86,8 +86,13 @@ class MCSampler:
         a.seed = self.key & 0xFFFFFFFFFFFFFFFF
         a.call_index = self.callIndex & 0xFFFFFFFF
         a.stream = 0
-        _lib.call("jv_mc_sample", self.net.desc, _lib.ptr(self.net.params), a, _lib.ptr(self.states),
-                  _lib.ptr(configs), _lib.ptr(logpsi), _lib.ptr(self.counters), _lib.stream_ptr())
+        if self.net._sym is not None:
+            _lib.call("jv_sym_mc_sample", self.net.desc, _lib.ptr(self.net.params), self.net._sym[0], a,
+                      _lib.ptr(self.states), _lib.ptr(configs), _lib.ptr(logpsi), _lib.ptr(self.counters),
+                      _lib.stream_ptr())
+        else:
+            _lib.call("jv_mc_sample", self.net.desc, _lib.ptr(self.net.params), a, _lib.ptr(self.states),
+                      _lib.ptr(configs), _lib.ptr(logpsi), _lib.ptr(self.counters), _lib.stream_ptr())
         self.callIndex += 1
         p = torch.empty(spc * C, dtype=torch.float64, device=dev)
         if spc * C > 0:
@@ -96,10 +101,7 @@ class MCSampler:
         if tmp is not None:
             self.net.set_parameters(tmp)
         configs = configs.reshape((1, spc * C) + self.sampleShape)
-        lp = logpsi.reshape(1, -1)
-        if not self.net.holomorphic:
-            lp = lp.real.contiguous()
-        return configs, lp, p.reshape(1, -1)
+        return configs, self.net._out(logpsi.reshape(1, -1)), p.reshape(1, -1)
 
     def acceptance_ratio(self):
         """sampler.py:369-380."""
@@ -144,10 +146,7 @@ class ExactSampler:
         _lib.call("jv_exact_weights", _lib.ptr(logpsi), n, float(self.lastNorm), float(self.logProbFactor),
                   _lib.ptr(p), _lib.ctypes.byref(nrm), _lib.stream_ptr())
         self.lastNorm += self.logProbFactor * math.log(nrm.value)
-        lp = logpsi.reshape(1, -1)
-        if not self.psi.holomorphic:
-            lp = lp.real.contiguous()
-        return self.basis, lp, p.reshape(1, -1)
+        return self.basis, self.psi._out(logpsi.reshape(1, -1)), p.reshape(1, -1)
 
     def set_number_of_samples(self, N):
         pass

@@ -46,6 +46,11 @@ class MinSR:
         """minsr.py:53-80 on centred data; gradients in the compact layout."""
         D = gradients._data.contiguous()
         e = eloc._data.reshape(-1).to(torch.complex128).contiguous()
+        if not holomorphic and torch.is_complex(D):
+            # real parameters with a complex output (a SymNet with complex factors):
+            # the stacked real system [Re O; Im O], [Re e; Im e] of minsr.py:68-78
+            D = torch.cat([D.real, D.imag], dim=0).contiguous()
+            e = torch.cat([e.real, e.imag]).to(torch.complex128).contiguous()
         n, p = D.shape
         cpx = torch.is_complex(D)
         upd = torch.empty(p, dtype=torch.complex128 if cpx else torch.float64, device=D.device)

@@ -11,6 +11,8 @@ import torch
 
 from . import _lib
 from . import global_defs
+from .nets import sym_wrapper
+from .nets.sym_wrapper import SymNet
 
 
 def _as_configs(s):
@@ -24,10 +26,12 @@ class NQS:
     def __init__(self, net, logarithmic=True, batchSize=1000, seed=1234, orbit=None, avgFun=None):
         if isinstance(net, (tuple, list)):
             raise NotImplementedError("two-network ansatz (TwoNets) is not implemented")
-        if orbit is not None:
-            raise NotImplementedError("symmetrised networks (SymNet) are not implemented")
         if not logarithmic:
             raise NotImplementedError("only logarithmic networks are implemented")
+        if orbit is not None:
+            # vqs.py:165-166: the network is wrapped in a SymNet
+            net = SymNet(orbit=orbit, net=net,
+                         avgFun=avgFun if avgFun is not None else sym_wrapper.avgFun_Coefficients_Exp)
         self.net = net
         self.logarithmic = True
         self.batchSize = batchSize
@@ -36,6 +40,7 @@ class NQS:
         self._isGenerator = False
         self.desc = None
         self._params = None
+        self._sym = None  # (JvOrbit, device map, device factors) of a SymNet
 
     # ------------------------------------------------------------ init
     def init_net(self, s):
@@ -48,10 +53,43 @@ class NQS:
         _lib.check(_lib.lib.jv_net_init(self.desc), "jv_net_init")
         self.holomorphic = bool(self.desc.is_complex)
         self.realParams = not self.holomorphic
+        self._cpx_out = self.holomorphic
+        self._cpx_weights = self.holomorphic
+        if isinstance(self.net, SymNet):
+            self._init_sym()
         rng = np.random.default_rng(self.seed)
         P = self.net.init_params(self.desc, rng)
         self._params = torch.as_tensor(P, dtype=torch.float64, device=global_defs.device())
         self.initialized = True
+
+    def _init_sym(self):
+        """Device form of the orbit; output and gradient types follow jax's
+        promotion in SymNet.__call__ (sym_wrapper.py:66-77) and the
+        Cauchy-Riemann test of vqs.py:199-210."""
+        orb = self.net.orbit
+        if orb.shape[1] != self.desc.num_sites:
+            raise ValueError("the orbit acts on %d sites, the lattice has %d" % (orb.shape[1], self.desc.num_sites))
+        dev = global_defs.device()
+        fac = np.asarray(orb.factor)
+        cfac = bool(np.iscomplexobj(fac))
+        m = torch.as_tensor(orb.device_map(), device=dev).contiguous()
+        f = torch.as_tensor(fac.astype(np.complex128), device=dev).contiguous()
+        o = _lib.JvOrbit()
+        o.num_elements = int(m.shape[0])
+        o.num_sites = int(m.shape[1])
+        o.avg = self.net.avg_code
+        o.complex_factor = int(cfac)
+        o.map = m.data_ptr()
+        o.factor = f.data_ptr()
+        self._sym = (o, m, f)
+        avg = self.net.avg_code
+        if avg == _lib.JV_AVG_SEP:
+            # Sep is not holomorphic; its value is complex for every network
+            self.holomorphic = False
+            self._cpx_out = True
+        elif avg == _lib.JV_AVG_EXP and cfac:
+            self._cpx_out = True
+            self._cpx_weights = True
 
     @property
     def numParameters(self):
@@ -88,29 +126,43 @@ class NQS:
         s = _as_configs(s)
         n = s.shape[0]
         out = torch.empty(n, dtype=torch.complex128, device=s.device)
-        _lib.call("jv_net_forward", self.desc, _lib.ptr(self._params), _lib.ptr(s), n, _lib.ptr(out),
-                  _lib.stream_ptr())
+        if self._sym is not None:
+            _lib.call("jv_sym_forward", self.desc, _lib.ptr(self._params), self._sym[0], _lib.ptr(s), n,
+                      _lib.ptr(out), _lib.stream_ptr())
+        else:
+            _lib.call("jv_net_forward", self.desc, _lib.ptr(self._params), _lib.ptr(s), n, _lib.ptr(out),
+                      _lib.stream_ptr())
         return out
+
+    def _out(self, lp):
+        """Complex log psi -> the network's output dtype (real for real outputs)."""
+        return lp if self._cpx_out else lp.real.contiguous()
 
     def __call__(self, s):
         """vqs.py:223-251."""
         s, flat = self._flat(s)
-        out = self.logpsi_complex(flat).reshape(s.shape[:2])
-        return out if self.holomorphic else out.real.contiguous()
+        return self._out(self.logpsi_complex(flat).reshape(s.shape[:2]))
 
     def gradients(self, s, layout=1):
         """vqs.py:271-287: O_k = d log psi / d theta_k, jVMC flat layout (layout=1)
         or the compact per-parameter layout (layout=0)."""
         s, flat = self._flat(s)
+        if self._sym is not None and not self.holomorphic and not self.realParams:
+            raise NotImplementedError("gradients of a non-holomorphic complex-parameter network "
+                                      "(avgFun_Coefficients_Sep over a complex network) are not implemented")
         n = flat.shape[0]
         if layout == 1:
             g = torch.empty((n, self.numRealParameters), dtype=torch.complex128, device=flat.device)
-        elif self.holomorphic:
+        elif self._cpx_weights:
             g = torch.empty((n, self.numParameters), dtype=torch.complex128, device=flat.device)
         else:
             g = torch.empty((n, self.numParameters), dtype=torch.float64, device=flat.device)
-        _lib.call("jv_net_gradients", self.desc, _lib.ptr(self._params), _lib.ptr(flat), n, layout, _lib.ptr(g),
-                  None, _lib.stream_ptr())
+        if self._sym is not None:
+            _lib.call("jv_sym_gradients", self.desc, _lib.ptr(self._params), self._sym[0], _lib.ptr(flat), n, layout,
+                      _lib.ptr(g), None, _lib.stream_ptr())
+        else:
+            _lib.call("jv_net_gradients", self.desc, _lib.ptr(self._params), _lib.ptr(flat), n, layout,
+                      _lib.ptr(g), None, _lib.stream_ptr())
         return g.reshape(s.shape[:2] + g.shape[1:])
 
     def get_sampler_net(self):
