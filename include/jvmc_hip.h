@@ -86,6 +86,11 @@ typedef struct jv_optable {
   const int32_t* map;        /* [num_ops][max_len][l_dim] new local state */
   const double* matel;       /* [num_ops][max_len][l_dim] complex factor */
   const int32_t* diag;       /* [num_diag] indices of all-diagonal strings */
+  const int32_t* fermi;      /* [num_ops][max_len] 1 for a fermionic factor (creation /
+                                annihilation, branch_free.py:149-191), else 0; NULL when the
+                                operator has none.  A fermionic factor on site i multiplies m
+                                by (-1)^(number of occupied sites j > i) of the configuration
+                                it acts on (branch_free.py:453,463-466). */
 } jv_optable;
 
 /* Symmetrisation of a network over an orbit of lattice symmetries (SymNet,

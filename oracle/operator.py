@@ -1,10 +1,12 @@
 """Branch-free operator strings, s' enumeration and local estimators.
 
 Oracle (test infrastructure only).  Follows
-* local-operator tables   branch_free.py:17-123 (Id, Sx, Sy, Sz, Sp, Sm)
+* local-operator tables   branch_free.py:17-191 (Id, Sx, Sy, Sz, Sp, Sm, and the
+                          fermionic number, creation, annihilation)
 * OpStr / scal_opstr       branch_free.py:195-276
 * compile (padding, diag)  branch_free.py:348-441
 * _get_s_primes            branch_free.py:443-487 (factors applied right to left,
+                           Jordan-Wigner sign mask of fermionic factors,
                            diagonal strings summed into the first diagonal slot)
 * nonzero compaction       base.py:91-114, get_s_primes base.py:116-160
 * O_loc                    base.py:162-212
@@ -57,6 +59,24 @@ def Sp(idx):
 def Sm(idx):
     return LocalOp(idx=idx, map=np.array([0, 0], np.int32),
                    matEls=np.array([0.0, 1.0], np.complex128), diag=False)
+
+
+def number(idx):
+    """branch_free.py:127-146."""
+    return LocalOp(idx=idx, map=np.array([0, 1], np.int32), matEls=np.array([0., 1.], np.complex128), diag=True,
+                   fermionic=False)
+
+
+def creation(idx):
+    """branch_free.py:149-168."""
+    return LocalOp(idx=idx, map=np.array([1, 0], np.int32), matEls=np.array([1., 0.], np.complex128), diag=False,
+                   fermionic=True)
+
+
+def annihilation(idx):
+    """branch_free.py:171-191."""
+    return LocalOp(idx=idx, map=np.array([1, 0], np.int32), matEls=np.array([0., 1.], np.complex128), diag=False,
+                   fermionic=True)
 
 
 def _id_prefactor(*args, val=1.0, **kwargs):
@@ -136,28 +156,31 @@ class BranchFreeOperator:
             n = len(op) - (1 if callable(op[0]) else 0)
             maxLen = max(maxLen, n)
         IdOp = Id(lDim=self.lDim)
-        idx, mp, me, diag, pref = [], [], [], [], []
+        idx, mp, me, diag, pref, fermi = [], [], [], [], [], []
         for o, op in enumerate(self.ops):
             k0 = 1 if callable(op[0]) else 0
             if k0:
                 pref.append((o, op[0]))
             isDiag = True
-            ri, rm, re = [], [], []
+            ri, rm, re, rf = [], [], [], []
             for k in range(maxLen):
                 kRev = len(op) - k - 1
                 if kRev >= k0:
                     if not op[kRev]["diag"]:
                         isDiag = False
                     ri.append(op[kRev]["idx"]); rm.append(op[kRev]["map"]); re.append(op[kRev]["matEls"])
+                    rf.append(1 if op[kRev].get("fermionic", False) else 0)
                 else:
                     ri.append(IdOp["idx"]); rm.append(IdOp["map"]); re.append(IdOp["matEls"])
-            idx.append(ri); mp.append(rm); me.append(re)
+                    rf.append(0)
+            idx.append(ri); mp.append(rm); me.append(re); fermi.append(rf)
             if isDiag:
                 diag.append(o)
         self.idxC = np.array(idx, dtype=np.int32)
         self.mapC = np.array(mp, dtype=np.int32)
         self.matElsC = np.array(me, dtype=np.complex128)
         self.diag = np.array(diag, dtype=np.int32)
+        self.fermiC = np.array(fermi, dtype=np.int64)
         self.prefactor = pref
         self.maxOpStrLength = maxLen
         self.compiled = True
@@ -184,10 +207,19 @@ class BranchFreeOperator:
         m = np.broadcast_to(self.prefactors(*args), (N, nOps)).copy()
         rows = np.arange(N)[:, None]
         opsr = np.arange(nOps)[None, :]
+        L = flat.shape[1]
+        # mask[i][j] = 1 for j > i (jnp.tril(ones, -1).T, branch_free.py:453)
+        mask = np.tril(np.ones((L, L), dtype=np.int64), -1).T
         for k in range(self.maxOpStrLength):
             site = self.idxC[:, k]  # (nOps,)
             cur = sp[rows, opsr, site[None, :]]  # (N, nOps)
+            # branch_free.py:463-466: prod_j ((1-2f)(2f mask[idx, j] + 1 - 2f) c_j + 1 - |c_j|)
             m = m * self.matElsC[opsr, k, cur]
+            if self.fermiC[:, k].any():  # the product is 1 for non-fermionic factors
+                f = self.fermiC[:, k][None, :, None]  # (1, nOps, 1)
+                c = sp.astype(np.int64)
+                term = (1 - 2 * f) * (2 * f * mask[site][None, :, :] + (1 - 2 * f)) * c + (1 - np.abs(c))
+                m = m * np.prod(term, axis=-1)
             sp[rows, opsr, site[None, :]] = self.mapC[opsr, k, cur]
         if len(self.diag) > 1:
             m[:, self.diag[0]] = np.sum(m[:, self.diag], axis=1)

@@ -39,7 +39,8 @@ class JvNet(ctypes.Structure):
 
 class JvOpTable(ctypes.Structure):
     _fields_ = [("num_ops", _i32), ("max_len", _i32), ("l_dim", _i32), ("num_diag", _i32),
-                ("max_sites_changed", _i32), ("reserved", _i32), ("idx", _vp), ("map", _vp), ("matel", _vp), ("diag", _vp)]
+                ("max_sites_changed", _i32), ("reserved", _i32), ("idx", _vp), ("map", _vp), ("matel", _vp), ("diag", _vp),
+                ("fermi", _vp)]
 
 
 JV_AVG_EXP = 0

@@ -21,6 +21,7 @@ struct OpDev {
   const double* matel;
   const int32_t* diag;
   const double* pref;
+  const int32_t* fermi;  // [num_ops][max_len] or nullptr
 };
 
 // Applies string o to s (branch_free.py:456-478, factors in stored order) and
@@ -40,6 +41,15 @@ __device__ inline cd apply_string(const OpDev& op, int o, const int32_t* s, int*
     const int cur = min(max(slot >= 0 ? tv[slot] : s[i], 0), 1);
     cd f = {op.matel[2 * (2 * base + cur)], op.matel[2 * (2 * base + cur) + 1]};
     m = m * f;
+    if (op.fermi && op.fermi[base]) {
+      // Jordan-Wigner sign: parity of the occupied sites after i in the
+      // configuration this factor acts on (branch_free.py:463-466)
+      int occ = 0;
+      for (int j = i + 1; j < op.L; ++j) occ += s[j];
+      for (int t = 0; t < nt; ++t)
+        if (ts[t] > i) occ += tv[t] - s[ts[t]];
+      if (occ & 1) m = {-m.re, -m.im};
+    }
     const int nv = min(max(op.map[2 * base + cur], 0), 1);
     if (slot >= 0) tv[slot] = nv;
     else { ts[nt] = i; tv[nt] = nv; ++nt; }
@@ -554,6 +564,7 @@ OpDev make_opdev(const jv_optable* op, const double* pref, int L) {
   d.matel = op->matel;
   d.diag = op->diag;
   d.pref = pref;
+  d.fermi = op->fermi;
   return d;
 }
 
@@ -595,8 +606,9 @@ int jv_get_s_primes(const jv_optable* op, const double* prefactor, const int32_t
 int jv_eloc(const jv_net* net, const double* params, const jv_optable* op, const double* prefactor,
             const int32_t* configs, const double* logpsi_s, int64_t n, double* out, void* stream) {
   int rc = check_op(op, prefactor);
-  const bool op_single_site = op->max_sites_changed == 1;
   if (rc) return rc;
+  // the window kernels classify strings without Jordan-Wigner signs
+  const bool op_single_site = op->max_sites_changed == 1 && !op->fermi;
   if (!net || !params || !configs || !logpsi_s || !out || n < 0) return fail(JV_E_ARG, "jv_eloc: bad arguments");
   if (n == 0) return JV_OK;
   hipStream_t st = (hipStream_t)stream;

@@ -67,6 +67,24 @@ def Sm(idx):
                    diag=False)
 
 
+def number(idx):
+    """Fermionic number operator c^dagger c (branch_free.py:127-146)."""
+    return LocalOp(idx=idx, map=np.array([0, 1], dtype=np.int32), matEls=np.array([0., 1.], dtype=opDtype),
+                   diag=True, fermionic=False)
+
+
+def creation(idx):
+    """Fermionic creation operator c^dagger (branch_free.py:149-168)."""
+    return LocalOp(idx=idx, map=np.array([1, 0], dtype=np.int32), matEls=np.array([1., 0.], dtype=opDtype),
+                   diag=False, fermionic=True)
+
+
+def annihilation(idx):
+    """Fermionic annihilation operator c (branch_free.py:171-191)."""
+    return LocalOp(idx=idx, map=np.array([1, 0], dtype=np.int32), matEls=np.array([0., 1.], dtype=opDtype),
+                   diag=False, fermionic=True)
+
+
 def _id_prefactor(*args, val=1.0, **kwargs):
     return val
 
@@ -148,24 +166,24 @@ class BranchFreeOperator(Operator):
             n = len(op) - (1 if callable(op[0]) else 0)
             maxLen = max(maxLen, n)
         IdOp = Id(lDim=self.lDim)
-        idx, mp, me, diag, pref = [], [], [], [], []
+        idx, mp, me, diag, pref, fermi = [], [], [], [], [], []
         for o, op in enumerate(self.ops):
             k0 = 1 if callable(op[0]) else 0
             if k0:
                 pref.append((o, op[0]))
             isDiag = True
-            ri, rm, re = [], [], []
+            ri, rm, re, rf = [], [], [], []
             for k in range(maxLen):
                 kRev = len(op) - k - 1
                 if kRev >= k0:
-                    if op[kRev].get("fermionic", False):
-                        raise NotImplementedError("fermionic operators are not implemented on the device")
                     if not op[kRev]["diag"]:
                         isDiag = False
                     ri.append(op[kRev]["idx"]); rm.append(op[kRev]["map"]); re.append(op[kRev]["matEls"])
+                    rf.append(1 if op[kRev].get("fermionic", False) else 0)
                 else:
                     ri.append(IdOp["idx"]); rm.append(IdOp["map"]); re.append(IdOp["matEls"])
-            idx.append(ri); mp.append(rm); me.append(re)
+                    rf.append(0)
+            idx.append(ri); mp.append(rm); me.append(re); fermi.append(rf)
             if isDiag:
                 diag.append(o)
         # upper bound on the sites one string changes: factors whose map is not
@@ -178,6 +196,7 @@ class BranchFreeOperator(Operator):
         self.mapH = np.array(mp, dtype=np.int32)
         self.matElsH = np.array(me, dtype=np.complex128)
         self.diagH = np.array(diag, dtype=np.int32)
+        self.fermionicH = np.array(fermi, dtype=np.int32)
         self.prefactor = pref
         self.maxOpStrLength = maxLen
         self._table = None
@@ -207,6 +226,12 @@ class BranchFreeOperator(Operator):
             t.map = self.mapC.data_ptr()
             t.matel = self.matElsC.data_ptr()
             t.diag = self.diagC.data_ptr()
+            # the sign mask is passed only when a string has a fermionic factor
+            if self.fermionicH.any():
+                self.fermiC = torch.as_tensor(self.fermionicH, device=dev).contiguous()
+                t.fermi = self.fermiC.data_ptr()
+            else:
+                t.fermi = None
             self._table = t
         return self._table
 
