@@ -34,7 +34,7 @@ namespace {
 
 constexpr int TRD_T = 512;
 constexpr int TRD_W = TRD_T / WAVE;
-constexpr int TRD_R = 8;  // n-vector elements per thread (n <= TRD_T * TRD_R)
+constexpr int TRD_R = 9;  // n-vector elements per thread (n <= TRD_T * TRD_R)
 constexpr int TRD_CP = 16;  // column rows per lane per load batch
 
 struct TrdArgs {
