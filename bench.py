@@ -22,6 +22,11 @@ exact sampler, imaginary-time TDVP (Euler dt=1e-2, snrTol=2, pinvTol=1e-8).
 ``--workload c2``: 1D TFIM L=32, h=1, complex RBM alpha=2, 1024 chains x 16384
 samples (25 thermalisation sweeps, sweep length 32), real-time TDVP with Heun
 dt=1e-3.
+``--workload c4``: the batched field sweep (64 independent TFIM chains of
+uneven size and parameter count, run 16 at a time on their own streams).
+``--workload sym``: 1D TFIM L=16, h=1, a SymNet of a complex RBM alpha=2 over
+the 64-element orbit (translation, reflection, spin flip), 1024 chains x 4096
+samples, imaginary-time TDVP (Euler dt=1e-2).
 
 Prints one JSON line with steps/s, samples/s, per-kernel times, the roofline of
 the dominant hand-written kernel (from HIP events recorded live on its stream)
@@ -112,6 +117,21 @@ class Workload:
             self.stepper = jv.util.stepper.Euler(timeStep=1e-2)
             self.desc = ("TFIM L=10 periodic, h=0.7; CpxRBM alpha=2 with visible bias (230 complex params); exact "
                          "sampler (1024 states); imaginary-time TDVP snrTol=2 pinvTol=1e-8; Euler dt=1e-2")
+        elif name == "sym":
+            self.shape, self.samples = (16,), 4096
+            orbit = jv.util.symmetries.get_orbit_1D(16, "translation", "reflection", "spinflip")
+            net = jv.nets.CpxRBM(numHidden=32, bias=True)
+            psi = jv.NQS(net, orbit=orbit)
+            psi(torch.zeros((1, 1) + self.shape, dtype=torch.int32))
+            psi.set_parameters(0.01 * rng.normal(size=psi.numRealParameters))
+            self.H = tfim_strings(jv.operator, self.shape, -1.0, -1.0)
+            self.sampler = jv.sampler.MCSampler(psi, self.shape, 1234, updateProposer=jv.sampler.propose_spin_flip,
+                                                numChains=1024, numSamples=self.samples)
+            self.eq = jv.util.TDVP(self.sampler, snrTol=2, pinvTol=1e-8, rhsPrefactor=1., makeReal='real')
+            self.stepper = jv.util.stepper.Euler(timeStep=1e-2)
+            self.desc = ("TFIM L=16 periodic, h=1; SymNet(CpxRBM alpha=2, 544 complex params) over the 64-element "
+                         "orbit (translation, reflection, spin flip); 1024 chains, 4096 samples; imaginary-time TDVP; "
+                         "Euler dt=1e-2")
         else:
             raise ValueError(name)
         self.psi = psi
@@ -129,7 +149,8 @@ def timer(jv, name):
     return tot.value, cnt.value
 
 
-KERNELS = ("gram", "gemm", "eigh", "tridiag", "stedc", "trd_apply", "eloc", "mc_sweep", "net_grad", "net_forward")
+KERNELS = ("gram", "gemm", "eigh", "tridiag", "stedc", "trd_apply", "eloc", "mc_sweep", "net_grad", "net_forward",
+           "sym_mc", "sym_eloc", "sym_combine")
 
 
 def run_gpu(workload, steps, warmup, mode):
@@ -180,6 +201,49 @@ def roofline_c3(kt, steps, mode):
             "frac": ach / FP64_PEAK_TFLOPS, "traffic": MEASURED_TRAFFIC.get(dom) if mode == "dual" else None,
             "kernel": dom, "avg_launch_ms": ms,
             "launches": launches}
+
+
+def batched_sweep(num_instances=64, steps=2, slots=16, seed=11):
+    """Batched field sweep: independent 1D TFIM instances with L ~ U{16, 24, 32,
+    48, 64}, h ~ U[0.5, 1.5], complex RBM alpha ~ U{1, 2, 4} (per-instance seed),
+    4096 Metropolis samples (256 chains) per instance and step, imaginary-time
+    Euler steps (dt = 1e-2).  Instances with Pr <= 4224 solve the TDVP equation
+    through the eigendecomposition of S; larger ones (Pr up to 33280, where S
+    alone would be 17.7 GB) use MinSR, whose T is 4096 x 4096.  All instances
+    are submitted at once and run `slots` at a time on their own streams."""
+    import torch
+    import vmc_jax_amd as jv
+    from vmc_jax_amd import serving
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for i in range(num_instances):
+        L = int(rng.choice([16, 24, 32, 48, 64]))
+        h = float(rng.uniform(0.5, 1.5))
+        alpha = int(rng.choice([1, 2, 4]))
+        jseed = int(rng.integers(1 << 30))
+        pr = 2 * (alpha * L + alpha * L * L)
+        tdvp = pr <= 4224
+
+        def make(L=L, h=h, alpha=alpha, jseed=jseed, tdvp=tdvp):
+            psi = jv.NQS(jv.nets.CpxRBM(numHidden=alpha * L, bias=True), seed=jseed)
+            psi(torch.zeros((1, 1, L), dtype=torch.int32))
+            H = tfim_strings(jv.operator, (L,), -1.0, -h)
+            ms = jv.sampler.MCSampler(psi, (L,), jseed, updateProposer=jv.sampler.propose_spin_flip,
+                                      numChains=256, numSamples=4096)
+            eq = jv.util.TDVP(ms, snrTol=2, pinvTol=1e-8, rhsPrefactor=1., makeReal='real') if tdvp else \
+                jv.util.MinSR(ms, pinvTol=1e-8, solver="auto")
+            return psi, H, eq, jv.util.stepper.Euler(1e-2)
+        fp = serving.footprint_bytes(4096, pr, L, True, "tdvp" if tdvp else "minsr", 256)
+        jobs.append(serving.Job(i, make, steps, fp, arrival=0.0,
+                                meta={"kind": "L%d-a%d-%s" % (L, alpha, "tdvp" if tdvp else "minsr")}))
+    srv = serving.Server(max_slots=slots)
+    for j in jobs:
+        srv.submit(j)
+    rep = srv.run()
+    rep["instances"] = num_instances
+    rep["slots"] = slots
+    rep["failures"] = [j.error for j in srv.finished if j.state == "failed"]
+    return rep
 
 
 def serve_mix(num_jobs=16, slots=8, seed=7, rate=8.0):
@@ -302,7 +366,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3"])
+    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c4", "sym"])
     ap.add_argument("--mode", default="dual", choices=["T", "dual"], help="MinSR solve form (c3)")
     ap.add_argument("--t-steps", type=int, default=3, help="c3: steps of the other MinSR form reported beside")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -315,6 +379,19 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (every box has one MI355X); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "c4":
+        t0 = time.perf_counter()
+        rep = batched_sweep(steps=max(1, min(args.steps, 4)))
+        wall = time.perf_counter() - t0
+        print(json.dumps({
+            "metric": "instance TDVP steps/s (c4: batched field sweep)", "value": rep["steps_per_s"],
+            "unit": "instance-steps/s", "n_gpus": 1, "steps": rep["steps"], "warmup": 0,
+            "ms_per_step": rep["p50_step_ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f64", "data": "synthetic (random network parameters; samples drawn from the networks)",
+            "config": {"workload": batched_sweep.__doc__.split("\n")[0].strip(), "instances": rep["instances"],
+                       "slots": rep["slots"]},
+            "report": rep, "wall_s_including_setup": wall}))
+        return
     w, elapsed, kt, energies = run_gpu(args.workload, args.steps, args.warmup, args.mode)
     steps_s = args.steps / elapsed
     out = {

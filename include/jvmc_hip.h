@@ -328,6 +328,22 @@ int jv_tdvp_solve(const double* data, const double* e, const double* w, int64_t 
                   double* update, double* S, double* F, double* ev, double* snr,
                   jv_solve_info* info, void* stream);
 
+/* jv_tdvp_solve for a holomorphic network, whose flat gradient rows are
+ * [g, i g] per complex leaf (vqs.py:66-69): column col_ig[c] of D equals i times
+ * column col_g[c] (device int64 [pc] each, pr = 2 pc).  Then S0 = O^H O is
+ * [[A, iA], [-iA, A]] in split order with A = G^H G (G = D[:, col_g]), so S0 is
+ * assembled from the Pc x Pc Gram and S is diagonalised through A: 'real'
+ * S = Re S0 has A's eigenvalues twice with eigenvectors [a; b], [-b; a] for
+ * u = a + ib; 'imag' S = i Im S0 has eigenvalues +-lambda(A) with eigenvectors
+ * [u; -iu] / sqrt2 and [conj u; i conj u] / sqrt2.  Same outputs as jv_tdvp_solve
+ * (within a degenerate eigenspace the basis may differ; the update does not).
+ * No separate reference function (exact rewrite of tdvp.py:134-171). */
+int jv_tdvp_solve_holomorphic(const double* data, const double* e, const double* w, int64_t n, int64_t pr,
+                              const int64_t* col_g, const int64_t* col_ig, int64_t pc, const double* x,
+                              int32_t make_real, double diag_shift, int32_t use_snr, double glob_num_samples,
+                              double snr_tol, double pinv_tol, double pinv_cutoff, double* update, double* S,
+                              double* F, double* ev, double* snr, jv_solve_info* info, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -413,6 +413,42 @@ def test_tdvp_exact(mode, H):
     assert abs(eq.get_energy_variance() - eqo.get_energy_variance()) < 1e-10
 
 
+@pytest.mark.parametrize("mode", ["imag", "real"])
+def test_tdvp_holomorphic_solve_equals_general(mode, monkeypatch):
+    """jv_tdvp_solve_holomorphic (S through the Pc x Pc Gram G^H G) against
+    jv_tdvp_solve on the same centred data: 'imag' with Metropolis samples and
+    the SNR regulariser (eigenvalues +-lambda, non-degenerate), 'real' with the
+    exact sampler (doubly degenerate spectrum; the update is basis independent)."""
+    rng = np.random.default_rng(26)
+    L = 8
+    onet = onets.CpxRBM((L,), numHidden=16, bias=True, visibleBias=True)
+    P = 0.1 * rng.normal(size=onet.num_real_params())
+    psi = make_psi(jv.nets.CpxRBM(numHidden=16, bias=True, visibleBias=True), (L,), P)
+    H, _ = tfim_pair((L,), -1.0, -0.7)
+    if mode == "imag":
+        sm = jv.sampler.MCSampler(psi, (L,), 6, updateProposer=jv.sampler.propose_spin_flip, numChains=64)
+        s, lp, p = sm.sample(numSamples=2048)
+    else:
+        sm = jv.sampler.ExactSampler(psi, L)
+        s, lp, p = sm.sample()
+    x = 1.j if mode == "imag" else 1.0
+    El = jv.stats.SampledObs(H.get_O_loc(s, psi, lp, 0.0), p)
+    G = jv.stats.SampledObs(psi.gradients(s), p)
+    eq = jv.util.TDVP(sm, snrTol=2, pinvTol=1e-8, rhsPrefactor=x, makeReal=mode)
+    u1, r1, c1 = eq.solve(El, G)
+    ev1, S1, snr1 = to_np(eq.ev), to_np(eq.S), to_np(eq.snr)
+    eq._holo_cols = psi.holomorphic_columns()
+    monkeypatch.setitem(jv.util.tdvp.HOLOMORPHIC_SOLVE, mode, 1)  # route Pc = 152 through the pair solve
+    u2, r2, c2 = eq.solve(El, G)
+    assert psi.numRealParameters == 2 * eq._holo_cols[0].numel() == 304
+    assert rel(to_np(eq.ev), ev1) < 1e-12
+    assert rel(to_np(eq.S), S1) < 1e-13
+    if mode == "imag":
+        assert rel(to_np(eq.snr), snr1) < 1e-8
+    assert c1 == c2 and abs(r1 - r2) < 1e-10
+    assert rel(to_np(u2), to_np(u1)) < 1e-9
+
+
 def test_tdvp_mc_snr_real_rbm():
     """MC mode with the SNR regulariser.  A real RBM has a non-degenerate
     spectrum, so the eigenbasis (and hence SNR) is unique up to sign."""

@@ -788,6 +788,129 @@ __global__ void tdvp_error_k(const double* __restrict__ u, const double* __restr
   if (threadIdx.x == 0) *out = fabs(1.0 + s / *evar);
 }
 
+// ---- holomorphic structure of the jVMC layout.  For a holomorphic network the
+// flat gradient row is [g, i g] per complex leaf (vqs.py:66-69): column
+// col_ig[c] = i * column col_g[c] of the compact gradient G.  Then
+//   S0 = O^H O  = [[A, iA], [-iA, A]] (split order), A = G^H G  (Pc x Pc),
+//   'real': S = Re S0 = [[Ar, -Ai], [Ai, Ar]], the real form of A: every
+//           eigenvalue of A twice, eigenvectors [a; b] and [-b; a] for u = a + i b;
+//   'imag': S = i Im S0 = i [[Ai, Ar], [-Ar, Ai]]: eigenvalues +-lambda_k(A),
+//           eigenvectors [u; -i u] / sqrt2 (+lambda) and [conj u; i conj u] / sqrt2
+//           (-lambda).
+// So one Pc x Pc Hermitian eigendecomposition replaces the 2Pc x 2Pc one.
+
+// G[r][c] = D[r][col_g[c]]
+__global__ void holo_gather(const double* __restrict__ D, int64_t n, int64_t pr, const int64_t* __restrict__ col_g,
+                            int64_t pc, double* __restrict__ G) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * pc) return;
+  const int64_t r = t / pc, c = t - r * pc;
+  const int64_t src = r * pr + col_g[c];
+  G[2 * t] = D[2 * src];
+  G[2 * t + 1] = D[2 * src + 1];
+}
+
+// S0 (flat order, pr x pr complex) from A (pc x pc complex)
+__global__ void holo_expand_S0(const double* __restrict__ A, int64_t pc, const int64_t* __restrict__ col_g,
+                               const int64_t* __restrict__ col_ig, int64_t pr, double* __restrict__ S0) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= pc * pc) return;
+  const int64_t c = t / pc, d = t - c * pc;
+  const double ar = A[2 * t], ai = A[2 * t + 1];
+  const int64_t gc = col_g[c], ic = col_ig[c], gd = col_g[d], id = col_ig[d];
+  double* p;
+  p = S0 + 2 * (gc * pr + gd); p[0] = ar;  p[1] = ai;   // A
+  p = S0 + 2 * (gc * pr + id); p[0] = -ai; p[1] = ar;   // i A
+  p = S0 + 2 * (ic * pr + gd); p[0] = ai;  p[1] = -ar;  // -i A
+  p = S0 + 2 * (ic * pr + id); p[0] = ar;  p[1] = ai;   // A
+}
+
+// A <- A + shift * diag(A) on the diagonal ('real' mode, tdvp.py:145-146: the
+// shift of S = Re S0 is the shift of A on both halves)
+__global__ void holo_shift(double* __restrict__ A, int64_t pc, double shift) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= pc || !(shift > 1e-10)) return;
+  A[2 * (c * pc + c)] += shift * A[2 * (c * pc + c)];
+}
+
+// Eigenpairs of S from those of A.  R (pc x pc complex) holds conj(u_k) in row
+// k (the eigh_raw convention), lam ascending.  Z rows receive conj(v_j) in flat
+// order with ev ascending.
+__global__ void holo_eig(const double* __restrict__ R, const double* __restrict__ lam, int64_t pc,
+                         const int64_t* __restrict__ col_g, const int64_t* __restrict__ col_ig, int64_t pr, int imag,
+                         double* __restrict__ Z, double* __restrict__ ev) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= pc * pc) return;
+  const int64_t k = t / pc, c = t - k * pc;
+  const double rr = R[2 * t], ri = R[2 * t + 1];  // conj(u_k)[c]
+  const int64_t gc = col_g[c], ic = col_ig[c];
+  if (imag) {
+    const double h = 0.70710678118654752440;
+    // row pc-1-k: eigenvalue -lam_k, conj(v) = [u; -i u] / sqrt2, u = conj(R_k)
+    double* z = Z + 2 * ((pc - 1 - k) * pr);
+    z[2 * gc] = h * rr;   z[2 * gc + 1] = -h * ri;
+    z[2 * ic] = -h * ri;  z[2 * ic + 1] = -h * rr;
+    // row pc+k: eigenvalue +lam_k, conj(v) = [conj u; i conj u] / sqrt2 = [R_k; i R_k] / sqrt2
+    z = Z + 2 * ((pc + k) * pr);
+    z[2 * gc] = h * rr;   z[2 * gc + 1] = h * ri;
+    z[2 * ic] = -h * ri;  z[2 * ic + 1] = h * rr;
+    if (c == 0) {
+      ev[pc - 1 - k] = -lam[k];
+      ev[pc + k] = lam[k];
+    }
+  } else {
+    // rows 2k, 2k+1: [a; b] and [-b; a] with u_k = a + i b = conj(R_k)
+    const double a = rr, b = -ri;
+    double* z = Z + (2 * k) * pr;
+    z[gc] = a;
+    z[ic] = b;
+    z = Z + (2 * k + 1) * pr;
+    z[gc] = -b;
+    z[ic] = a;
+    if (c == 0) {
+      ev[2 * k] = lam[k];
+      ev[2 * k + 1] = lam[k];
+    }
+  }
+}
+
+// SNR projection through A's eigenvectors.  With Qc[r][c] = q_g + i q_ig (the
+// real SNR data of the pair; 'imag' stores q as the imaginary part) and
+// M = Qc R_A^T (R_A rows = conj(u_k)), the projection Q Z^T onto S's
+// eigenvectors is, per pair k:
+//   'imag': column pc+k = i M / sqrt2, column pc-1-k = i conj(M) / sqrt2;
+//   'real': column 2k = Re M, column 2k+1 = Im M.
+__global__ void holo_qc(const double* __restrict__ Q, int64_t n, int64_t pr, const int64_t* __restrict__ col_g,
+                        const int64_t* __restrict__ col_ig, int64_t pc, int imag, double* __restrict__ Qc) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * pc) return;
+  const int64_t r = t / pc, c = t - r * pc;
+  const int64_t a = r * pr + col_g[c], b = r * pr + col_ig[c];
+  Qc[2 * t] = imag ? Q[2 * a + 1] : Q[a];
+  Qc[2 * t + 1] = imag ? Q[2 * b + 1] : Q[b];
+}
+
+__global__ void holo_snr_scatter(const double* __restrict__ M, int64_t n, int64_t pc, int imag,
+                                 double* __restrict__ R) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * pc) return;
+  const int64_t r = t / pc, k = t - r * pc;
+  const double mr = M[2 * t], mi = M[2 * t + 1];
+  const int64_t pr = 2 * pc;
+  if (imag) {
+    const double h = 0.70710678118654752440;
+    double* p = R + 2 * (r * pr + pc + k);  // i M / sqrt2
+    p[0] = -h * mi;
+    p[1] = h * mr;
+    p = R + 2 * (r * pr + pc - 1 - k);      // i conj(M) / sqrt2
+    p[0] = h * mi;
+    p[1] = h * mr;
+  } else {
+    R[r * pr + 2 * k] = mr;
+    R[r * pr + 2 * k + 1] = mi;
+  }
+}
+
 __global__ void sum_vec(const double* __restrict__ v, int64_t n, double* __restrict__ out) {
   __shared__ double red[16];
   double a = 0.0;
@@ -819,12 +942,22 @@ int jv_gemv(int32_t trans, const double* M, const double* x, int64_t rows, int64
   return gemv(trans != 0, M, x, rows, cols, is_complex, conj, alpha, y, (hipStream_t)stream, 14);
 }
 
-int jv_tdvp_solve(const double* D, const double* e, const double* w, int64_t n, int64_t pr, const double* x,
-                  int32_t make_real, double diag_shift, int32_t use_snr, double glob_num_samples, double snr_tol,
-                  double pinv_tol, double pinv_cutoff, double* update, double* S, double* F, double* ev,
-                  double* snr, jv_solve_info* info, void* stream) {
+}  // extern "C"
+
+namespace jv {
+// The TDVP solve; with col_g / col_ig (device, pc entries each) the data has
+// the holomorphic [g, i g] structure and S is built and diagonalised through
+// the Pc x Pc Gram A = G^H G.
+static int tdvp_solve_impl(const double* D, const double* e, const double* w, int64_t n, int64_t pr,
+                           const double* x, int32_t make_real, double diag_shift, int32_t use_snr,
+                           double glob_num_samples, double snr_tol, double pinv_tol, double pinv_cutoff,
+                           double* update, double* S, double* F, double* ev, double* snr, jv_solve_info* info,
+                           const int64_t* col_g, const int64_t* col_ig, int64_t pc, void* stream) {
   if (!D || !e || !w || !x || !update || !S || !F || !ev || !snr || n < 1 || pr < 1)
     return fail(JV_E_ARG, "jv_tdvp_solve: bad arguments");
+  const bool holo = col_g != nullptr;
+  if (holo && (!col_ig || pc < 1 || 2 * pc != pr))
+    return fail(JV_E_ARG, "jv_tdvp_solve_holomorphic: the column maps must split pr = 2 pc columns");
   if (!(pinv_tol < 1.0)) return fail(JV_E_ARG, "pinvTol >= 1 leaves the regulariser undefined (tdvp.py:195-198)");
   hipStream_t st = (hipStream_t)stream;
   const int imag = make_real != 0;
@@ -849,7 +982,20 @@ int jv_tdvp_solve(const double* D, const double* e, const double* w, int64_t n, 
   const unsigned bp = (unsigned)((pr + 255) / 256);
   int rc;
   // S0 = D^dagger D, F0 = -x D^dagger e  (tdvp.py:139-142)
-  if ((rc = jv_gram_pp(D, n, pr, 1, S0, stream))) return rc;
+  double* A = nullptr;
+  if (holo) {
+    double* G = (double*)workspace(sizeof(double) * 2 * (size_t)n * pc, 38);
+    A = (double*)workspace(sizeof(double) * 2 * (size_t)pc * pc, 39);
+    if (!G || !A) return fail(JV_E_NOMEM, "jv_tdvp_solve: workspace allocation failed");
+    hipLaunchKernelGGL(holo_gather, dim3((unsigned)((n * pc + 255) / 256)), dim3(256), 0, st, D, n, pr, col_g, pc, G);
+    JV_LAUNCH_CHECK("holo_gather");
+    if ((rc = jv_gram_pp(G, n, pc, 1, A, stream))) return rc;
+    hipLaunchKernelGGL(holo_expand_S0, dim3((unsigned)((pc * pc + 255) / 256)), dim3(256), 0, st, A, pc, col_g,
+                       col_ig, pr, S0);
+    JV_LAUNCH_CHECK("holo_expand_S0");
+  } else {
+    if ((rc = jv_gram_pp(D, n, pr, 1, S0, stream))) return rc;
+  }
   if ((rc = gemv(true, D, e, n, pr, 1, 1, 1.0, raw, st, 34))) return rc;
   hipLaunchKernelGGL(tdvp_force, dim3(bp), dim3(256), 0, st, raw, pr, xr, xi, imag, F0, F);
   JV_LAUNCH_CHECK("tdvp_force");
@@ -857,8 +1003,21 @@ int jv_tdvp_solve(const double* D, const double* e, const double* w, int64_t n, 
                      diag_shift, S);
   JV_LAUNCH_CHECK("tdvp_project_S");
   // eigendecomposition of S (tdvp.py:153-171); Z rows hold z_k
-  JV_HIP(hipMemcpyAsync(Z, S, sizeof(double) * (imag ? 2 : 1) * P2, hipMemcpyDeviceToDevice, st));
-  if ((rc = eigh_raw(pr, imag, Z, ev, st))) return rc;
+  if (holo) {
+    double* lam = (double*)workspace(sizeof(double) * (size_t)pc, 44);
+    if (!lam) return fail(JV_E_NOMEM, "jv_tdvp_solve: workspace allocation failed");
+    if (!imag) {
+      hipLaunchKernelGGL(holo_shift, dim3((unsigned)((pc + 255) / 256)), dim3(256), 0, st, A, pc, diag_shift);
+      JV_LAUNCH_CHECK("holo_shift");
+    }
+    if ((rc = eigh_raw(pc, 1, A, lam, st))) return rc;
+    hipLaunchKernelGGL(holo_eig, dim3((unsigned)((pc * pc + 255) / 256)), dim3(256), 0, st, A, lam, pc, col_g,
+                       col_ig, pr, imag, Z, ev);
+    JV_LAUNCH_CHECK("holo_eig");
+  } else {
+    JV_HIP(hipMemcpyAsync(Z, S, sizeof(double) * (imag ? 2 : 1) * P2, hipMemcpyDeviceToDevice, st));
+    if ((rc = eigh_raw(pr, imag, Z, ev, st))) return rc;
+  }
   // VtF_k = v_k^dagger F = sum_i z_k[i] F_i
   if (imag) {
     if ((rc = gemv(false, Z, F, pr, pr, 1, 0, 1.0, VtF, st, 34))) return rc;
@@ -871,7 +1030,18 @@ int jv_tdvp_solve(const double* D, const double* e, const double* w, int64_t n, 
   hipLaunchKernelGGL(tdvp_snr_data, dim3((unsigned)((n * pr + 255) / 256)), dim3(256), 0, st, D, e, w, n, pr, xr,
                      xi, imag, Q);
   JV_LAUNCH_CHECK("tdvp_snr_data");
-  {
+  if (holo) {
+    double* Qc = (double*)workspace(sizeof(double) * 2 * (size_t)n * pc, 38);
+    double* M = (double*)workspace(sizeof(double) * 2 * (size_t)n * pc, 45);
+    if (!Qc || !M) return fail(JV_E_NOMEM, "jv_tdvp_solve: workspace allocation failed");
+    const unsigned nb = (unsigned)((n * pc + 255) / 256);
+    hipLaunchKernelGGL(holo_qc, dim3(nb), dim3(256), 0, st, Q, n, pr, col_g, col_ig, pc, imag, Qc);
+    JV_LAUNCH_CHECK("holo_qc");
+    GemmArgs g{n, pc, pc, Qc, pc, 1, 0, A, 1, pc, 0, M, pc, 1, 0, 1.0};
+    if ((rc = gemm(g, st))) return rc;
+    hipLaunchKernelGGL(holo_snr_scatter, dim3(nb), dim3(256), 0, st, M, n, pc, imag, R);
+    JV_LAUNCH_CHECK("holo_snr_scatter");
+  } else {
     GemmArgs g{n, pr, pr, Q, pr, 1, 0, Z, 1, pr, 0, R, pr, imag, 0, 1.0};
     if ((rc = gemm(g, st))) return rc;
   }
@@ -937,6 +1107,27 @@ int jv_tdvp_solve(const double* D, const double* e, const double* w, int64_t n, 
     info->tdvp_error = hs[2];
   }
   return JV_OK;
+}
+}  // namespace jv
+
+extern "C" {
+
+int jv_tdvp_solve(const double* D, const double* e, const double* w, int64_t n, int64_t pr, const double* x,
+                  int32_t make_real, double diag_shift, int32_t use_snr, double glob_num_samples, double snr_tol,
+                  double pinv_tol, double pinv_cutoff, double* update, double* S, double* F, double* ev,
+                  double* snr, jv_solve_info* info, void* stream) {
+  return tdvp_solve_impl(D, e, w, n, pr, x, make_real, diag_shift, use_snr, glob_num_samples, snr_tol, pinv_tol,
+                         pinv_cutoff, update, S, F, ev, snr, info, nullptr, nullptr, 0, stream);
+}
+
+int jv_tdvp_solve_holomorphic(const double* D, const double* e, const double* w, int64_t n, int64_t pr,
+                              const int64_t* col_g, const int64_t* col_ig, int64_t pc, const double* x,
+                              int32_t make_real, double diag_shift, int32_t use_snr, double glob_num_samples,
+                              double snr_tol, double pinv_tol, double pinv_cutoff, double* update, double* S,
+                              double* F, double* ev, double* snr, jv_solve_info* info, void* stream) {
+  if (!col_g || !col_ig) return fail(JV_E_ARG, "jv_tdvp_solve_holomorphic: null column maps");
+  return tdvp_solve_impl(D, e, w, n, pr, x, make_real, diag_shift, use_snr, glob_num_samples, snr_tol, pinv_tol,
+                         pinv_cutoff, update, S, F, ev, snr, info, col_g, col_ig, pc, stream);
 }
 
 }  // extern "C"

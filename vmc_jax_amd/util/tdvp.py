@@ -1,7 +1,10 @@
 """TDVP right-hand side (tdvp.py:26-326) with the solve on the GPU.
 
 The S / F build, FP64 eigendecomposition, SNR projection and the regulariser
-loop (all cutoff candidates at once) run inside ``jv_tdvp_solve``.
+loop (all cutoff candidates at once) run inside ``jv_tdvp_solve``.  For a
+holomorphic network the gradient columns come in pairs (g, i g), and
+``jv_tdvp_solve_holomorphic`` builds S from the Pc x Pc Gram G^H G and
+diagonalises S through it (an exact rewrite, see include/jvmc_hip.h).
 """
 import ctypes
 import time
@@ -35,6 +38,14 @@ class _Timer:
         if self.outp is not None:
             torch.cuda.synchronize()
             self.outp.stop_timing(name)
+
+
+# Smallest Pc for which each solve mode uses the holomorphic pair structure
+# ('imag': S has +-eig(G^H G), always cheaper than the 2Pc eigendecomposition;
+# 'real': S has every eigenvalue of G^H G twice, and for small Pc the real
+# tridiagonal path on the 2Pc matrix beats the complex Pc one: 6.4 vs 5.5 ms
+# at Pc = 230, 70 vs 154 ms at Pc = 2112 on MI355X).
+HOLOMORPHIC_SOLVE = {"real": 512, "imag": 1}
 
 
 class TDVP:
@@ -106,11 +117,17 @@ class TDVP:
         snr = torch.empty(pr, dtype=torch.float64, device=dev)
         x = (ctypes.c_double * 2)(complex(self.rhsPrefactor).real, complex(self.rhsPrefactor).imag)
         info = _lib.JvSolveInfo()
-        _lib.call("jv_tdvp_solve", _lib.ptr(D), _lib.ptr(e), _lib.ptr(w), n, pr, x, int(self._imag),
-                  float(self.diagonalShift), int(not isinstance(self.sampler, ExactSampler)),
+        common = (x, int(self._imag), float(self.diagonalShift), int(not isinstance(self.sampler, ExactSampler)),
                   float(self._num_samples()), float(self.snrTol), float(self.pinvTol), float(self.pinvCutoff),
                   _lib.ptr(update), _lib.ptr(S), _lib.ptr(F), _lib.ptr(ev), _lib.ptr(snr), ctypes.byref(info),
                   _lib.stream_ptr())
+        cols = getattr(self, "_holo_cols", None)
+        if cols is not None and cols[0].numel() * 2 == pr and \
+                cols[0].numel() >= HOLOMORPHIC_SOLVE["imag" if self._imag else "real"]:
+            _lib.call("jv_tdvp_solve_holomorphic", _lib.ptr(D), _lib.ptr(e), _lib.ptr(w), n, pr, _lib.ptr(cols[0]),
+                      _lib.ptr(cols[1]), cols[0].numel(), *common)
+        else:
+            _lib.call("jv_tdvp_solve", _lib.ptr(D), _lib.ptr(e), _lib.ptr(w), n, pr, *common)
         self.S, self.F, self.ev, self.snr = S, F, ev, snr
         self.ElocMean = complex(Eloc.mean()[0].item())
         self.ElocVar = info.e_var
@@ -122,6 +139,7 @@ class TDVP:
         from ..stats import SampledObs
         tmpParameters = psi.get_parameters()
         psi.set_parameters(netParameters)
+        self._holo_cols = psi.holomorphic_columns() if psi.holomorphic else None
         timer = _Timer(rhsArgs.get("outp", None))
         numSamples = rhsArgs.get("numSamples", None)
         timer.start("sampling")

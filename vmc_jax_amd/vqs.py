@@ -202,6 +202,20 @@ class NQS:
             leaves.append((d.off_visible_bias, d.num_sites))
         return leaves
 
+    def holomorphic_columns(self):
+        """Device int64 maps (col_g, col_ig) of a holomorphic network: flat
+        gradient column col_ig[c] is i times column col_g[c] ([g, i g] per
+        complex leaf, vqs.py:66-69)."""
+        if getattr(self, "_holo_cols", None) is None:
+            g, ig = [], []
+            for off, size in self._complex_leaves():
+                g.append(np.arange(off, off + size))
+                ig.append(np.arange(off + size, off + 2 * size))
+            dev = global_defs.device()
+            self._holo_cols = (torch.as_tensor(np.concatenate(g), dtype=torch.int64, device=dev).contiguous(),
+                               torch.as_tensor(np.concatenate(ig), dtype=torch.int64, device=dev).contiguous())
+        return self._holo_cols
+
     def set_parameters(self, P):
         """vqs.py:409-427."""
         if not self.initialized:
