@@ -149,3 +149,64 @@ def test_ctypes_signatures_match_header_arity(lib):
         if name == "jv_last_error":
             continue
         assert len(lib._SIGS[name]) == n, "%s: header has %d parameters, binding %d" % (name, n, len(lib._SIGS[name]))
+
+
+def test_sym_entry_points_reject_bad_orbits(lib):
+    """Host-side validation of jv_sym_* (no device memory is touched)."""
+    from vmc_jax_amd import _lib as L
+    d = L.JvNet()
+    d.kind = L.JV_NET_CPXRBM
+    d.ndim = 1
+    d.shape[0] = 6
+    d.shape[1] = 1
+    d.num_hidden = 4
+    assert lib.lib.jv_net_init(ctypes.byref(d)) == 0
+    dummy = ctypes.c_void_p(1)  # never dereferenced: validation fails first
+    o = L.JvOrbit()
+    o.num_elements = 3
+    o.num_sites = 5  # the network has 6 sites
+    o.avg = L.JV_AVG_EXP
+    o.map = dummy
+    o.factor = dummy
+    assert lib.lib.jv_sym_forward(ctypes.byref(d), dummy, ctypes.byref(o), dummy, 4, dummy, None) == -1
+    assert b"different number of sites" in lib.lib.jv_last_error()
+    o.num_sites = 6
+    o.avg = 7
+    assert lib.lib.jv_sym_gradients(ctypes.byref(d), dummy, ctypes.byref(o), dummy, 4, 1, dummy, None, None) == -1
+    o.avg = L.JV_AVG_SEP
+    # Sep over a complex-parameter network is not holomorphic: no gradients
+    assert lib.lib.jv_sym_gradients(ctypes.byref(d), dummy, ctypes.byref(o), dummy, 4, 1, dummy, None, None) == -1
+    assert b"non-holomorphic" in lib.lib.jv_last_error()
+    o.num_elements = 0
+    assert lib.lib.jv_sym_forward(ctypes.byref(d), dummy, ctypes.byref(o), dummy, 4, dummy, None) == -1
+    assert lib.lib.jv_sym_mc_sample(ctypes.byref(d), dummy, None, None, dummy, dummy, dummy, dummy, None) == -1
+
+
+def test_holomorphic_tdvp_rejects_bad_column_maps(lib):
+    dummy = ctypes.c_void_p(1)
+    x = (ctypes.c_double * 2)(1.0, 0.0)
+    info = lib.JvSolveInfo()
+    args = (dummy, dummy, dummy, 10, 8)
+    tail = (x, 0, 0.0, 0, 10.0, 2.0, 1e-8, 1e-8, dummy, dummy, dummy, dummy, dummy, ctypes.byref(info), None)
+    # pr must be 2 * pc
+    assert lib.lib.jv_tdvp_solve_holomorphic(*args, dummy, dummy, 3, *tail) == -1
+    assert b"split pr = 2 pc" in lib.lib.jv_last_error()
+    assert lib.lib.jv_tdvp_solve_holomorphic(*args, None, dummy, 4, *tail) == -1
+
+
+def test_fermionic_table_host_side():
+    """The compiled host tables carry the fermionic mask in factor order."""
+    from vmc_jax_amd import operator as dop
+    H = dop.BranchFreeOperator()
+    H.add(dop.scal_opstr(1.0, (dop.creation(2), dop.annihilation(0))))
+    H.add(dop.scal_opstr(0.5, (dop.number(1),)))
+    H.compile()
+    # factors are stored right to left: annihilation(0) first
+    np.testing.assert_array_equal(H.fermionicH, [[1, 1], [0, 0]])
+    np.testing.assert_array_equal(H.idxH, [[0, 2], [1, 0]])
+    assert H.max_sites_changed == 2
+    Ho = oop.BranchFreeOperator()
+    Ho.add(oop.scal_opstr(1.0, (oop.creation(2), oop.annihilation(0))))
+    Ho.add(oop.scal_opstr(0.5, (oop.number(1),)))
+    Ho.compile()
+    np.testing.assert_array_equal(Ho.fermiC, H.fermionicH)
