@@ -149,8 +149,8 @@ def timer(jv, name):
     return tot.value, cnt.value
 
 
-KERNELS = ("gram", "gemm", "eigh", "tridiag", "stedc", "trd_apply", "eloc", "mc_sweep", "net_grad", "net_forward",
-           "sym_mc", "sym_eloc", "sym_combine")
+KERNELS = ("gram", "gemm", "eigh", "tridiag", "stedc", "ormtr", "trd_apply", "eloc", "mc_sweep", "net_grad",
+           "net_forward", "sym_mc", "sym_eloc", "sym_combine")
 
 
 def run_gpu(workload, steps, warmup, mode):

@@ -325,10 +325,11 @@ def test_large_products_through_blas(cpx):
     assert np.array_equal(to_np(o.covar()), to_np(o.covar()))
 
 
-@pytest.mark.parametrize("cpx,n", [(False, 200), (True, 200), (False, 300), (False, 777)])
+@pytest.mark.parametrize("cpx,n", [(False, 200), (True, 200), (False, 300), (False, 777), (True, 300), (True, 777)])
 def test_eigh(cpx, n):
-    """rocSOLVER below 256 (and for complex matrices); the Householder
-    tridiagonalisation + divide and conquer + dormtr path from 256 up."""
+    """rocSOLVER below 256; from 256 up the Householder tridiagonalisation
+    (real: dsytrd conventions; complex Hermitian: zhetrd conventions, real T)
+    + divide and conquer + dormtr / zunmtr."""
     rng = np.random.default_rng(21)
     A = rng.normal(size=(n, n)) + (1j * rng.normal(size=(n, n)) if cpx else 0)
     A = A + A.conj().T
@@ -387,9 +388,11 @@ def test_minsr_holomorphic_rbm():
     assert abs(eq.get_energy_mean() - eqo.get_energy_mean()) < 1e-11
 
 
-@pytest.mark.parametrize("mode,H", [("real", 8), ("imag", 8), ("real", 16)])
+@pytest.mark.parametrize("mode,H", [("real", 8), ("imag", 8), ("real", 16), ("imag", 32)])
 def test_tdvp_exact(mode, H):
-    """H = 16 gives Pr = 304: the real S goes through the tridiagonal path."""
+    """H = 16 gives Pr = 304: the real S goes through the tridiagonal path;
+    H = 32 gives Pc = 296: the 'imag' solve diagonalises G^H G through the
+    complex Hermitian tridiagonal path."""
     rng = np.random.default_rng(24)
     L = 8
     onet = onets.CpxRBM((L,), numHidden=H, bias=True, visibleBias=True)

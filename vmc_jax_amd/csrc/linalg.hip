@@ -18,6 +18,8 @@ int trd_eig_solve(double* S, int n, double* ev, double* c, double* Zbuf, double*
 int trd_apply(const double* V, const double* tau, int n, double* x, double* Tbuf, hipStream_t st);
 size_t trd_wy_offset(int n);
 int trd_eigh_full(double* S, int n, double* ev, hipStream_t st);
+bool trdc_supported(int64_t n);
+int trdc_eigh_full(double* S, int n, double* ev, hipStream_t st);
 
 rocblas_handle blas_handle(hipStream_t stream);
 
@@ -455,6 +457,7 @@ int gemv(bool trans, const double* M, const double* x, int64_t rows, int64_t col
 int eigh_raw(int64_t n, int cpx, double* A, double* w, hipStream_t st) {
   ScopedTimer tm("eigh", st);
   if (!cpx && trd_supported(n)) return trd_eigh_full(A, (int)n, w, st);
+  if (cpx && trdc_supported(n)) return trdc_eigh_full(A, (int)n, w, st);
   rocblas_handle h = blas_handle(st);
   if (!h) return fail(JV_E_SOLVER, "rocblas_create_handle failed");
   double* E = (double*)workspace(sizeof(double) * (size_t)n + 64, 3);

@@ -241,6 +241,206 @@ __global__ void __launch_bounds__(TRD_T) trd_step(TrdArgs a, int k) {
   }
 }
 
+// ---- complex Hermitian version (LAPACK zhetrd 'L' conventions: H_k = I - tau_k
+// v_k v_k^H, beta real, so T is real symmetric tridiagonal).  Row j of M holds
+// A[j][:]; the rank-2 update of step k-1 is A - v w^H - w v^H with
+// w = tau y - alpha' v, alpha' = |tau|^2 / 2 Re(y^H v), y = A v; column k+1
+// after update k is x = u + gamma v with u_j = A[j][k+1] - tau y_j and
+// gamma = 2 alpha' - conj(tau y_{k+1}).  Step 0 reads column 0 as conj(row 0).
+constexpr int TRDC_R = 5;   // complex n-vector elements per thread
+constexpr int TRDC_CP = 8;  // row elements per lane per load batch
+
+struct TrdcArgs {
+  double* M;    // n x n Hermitian, complex interleaved (row k <- reflector k after step k)
+  double* d;    // [n]
+  double* e;    // [n-1]
+  double* tau;  // complex [n]
+  double* y;    // complex [2][n]
+  double* u;    // complex [2][n]
+  double* pd;   // [2][G]
+  double* v0;   // complex [n]: reflector 0 until step 1 copies it into row 0
+  int n, G;
+};
+
+__device__ inline cd cmul(cd a, cd b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+__device__ inline cd cmulc(cd a, cd b) { return {a.re * b.re + a.im * b.im, a.im * b.re - a.re * b.im}; }  // a conj(b)
+
+__global__ void __launch_bounds__(TRD_T) trd_step_c(TrdcArgs a, int k) {
+  extern __shared__ double sh[];
+  const int n = a.n, G = a.G, tid = threadIdx.x, g = blockIdx.x;
+  int j0 = g + ((k + 1 - g + G - 1) / G) * G;
+  if (g >= k + 1) j0 = g;
+  const bool owns = j0 < n;
+  if (!owns && g != 0) {
+    if (threadIdx.x == 0) a.pd[(k & 1) * G + g] = 0.0;
+    return;
+  }
+  const int w = tid / WAVE, lane = tid % WAVE;
+  const int jw = j0 + w * G;
+  cd* vk = (cd*)sh;
+  cd* vp = vk + n;
+  cd* wp = vp + n;
+  cd* x = wp + n;
+  double* red = (double*)(x + n);  // 2 * TRD_W
+  const cd* Mc = (const cd*)a.M;
+  if (k == 0) {
+    cd xr[TRDC_R];
+#pragma unroll
+    for (int r = 0; r < TRDC_R; ++r) {
+      const int i = tid + r * TRD_T;
+      xr[r] = i < n ? Mc[i] : cd{0.0, 0.0};
+    }
+#pragma unroll
+    for (int r = 0; r < TRDC_R; ++r) {
+      const int i = tid + r * TRD_T;
+      if (i < n) {
+        x[i] = cconj(xr[r]);
+        vp[i] = {0.0, 0.0};
+        wp[i] = {0.0, 0.0};
+      }
+    }
+  } else {
+    const int pb = (k - 1) & 1;
+    const cd* yv = (const cd*)a.y + (size_t)pb * n;
+    const cd* uv = (const cd*)a.u + (size_t)pb * n;
+    const cd* vrow = (const cd*)(k == 1 ? a.v0 : a.M + 2 * (size_t)(k - 1) * n);
+    cd yr[TRDC_R], ur[TRDC_R], vr[TRDC_R];
+#pragma unroll
+    for (int r = 0; r < TRDC_R; ++r) {
+      const int i = tid + r * TRD_T;
+      const bool in = i >= k && i < n;
+      yr[r] = in ? yv[i] : cd{0.0, 0.0};
+      ur[r] = in ? uv[i] : cd{0.0, 0.0};
+      vr[r] = in ? vrow[i] : cd{0.0, 0.0};
+    }
+    double pdv = 0.0;
+    for (int q = tid; q < G; q += TRD_T) pdv += a.pd[pb * G + q];
+    const cd tp = {a.tau[2 * (k - 1)], a.tau[2 * (k - 1) + 1]};
+    const cd yk = yv[k];
+    const double sdot = trd_block_sum(pdv, red);
+    const double ap = 0.5 * cabs2(tp) * sdot;
+    const cd ty = cmul(tp, yk);
+    const cd gam = {2.0 * ap - ty.re, ty.im};  // 2 alpha' - conj(tau y_k)
+#pragma unroll
+    for (int r = 0; r < TRDC_R; ++r) {
+      const int i = tid + r * TRD_T;
+      if (i < n) {
+        vp[i] = vr[r];
+        const cd t = cmul(tp, yr[r]);
+        wp[i] = {t.re - ap * vr[r].re, t.im - ap * vr[r].im};
+        const cd gv = cmul(gam, vr[r]);
+        x[i] = {ur[r].re + gv.re, ur[r].im + gv.im};
+      }
+    }
+  }
+  __syncthreads();
+  cd tk = {0.0, 0.0};
+  if (k <= n - 2) {
+    double part = 0.0;
+#pragma unroll
+    for (int r = 0; r < TRDC_R; ++r) {
+      const int i = tid + r * TRD_T;
+      if (i >= k + 2 && i < n) part += cabs2(x[i]);
+    }
+    const double sigma = trd_block_sum(part, red + TRD_W);
+    const cd alpha = x[k + 1];
+    double beta = alpha.re;
+    cd scale = {0.0, 0.0};
+    if (sigma != 0.0 || alpha.im != 0.0) {
+      beta = -copysign(sqrt(cabs2(alpha) + sigma), alpha.re);
+      tk = {(beta - alpha.re) / beta, -alpha.im / beta};
+      const cd dn = {alpha.re - beta, alpha.im};
+      const double inv = 1.0 / cabs2(dn);
+      scale = {dn.re * inv, -dn.im * inv};
+    }
+    for (int i = tid; i < n; i += TRD_T)
+      vk[i] = i < k + 1 ? cd{0.0, 0.0} : (i == k + 1 ? cd{1.0, 0.0} : cmul(x[i], scale));
+    if (g == 0 && tid == 0) {
+      a.e[k] = beta;
+      a.tau[2 * k] = tk.re;
+      a.tau[2 * k + 1] = tk.im;
+    }
+  } else {
+    for (int i = tid; i < n; i += TRD_T) vk[i] = {0.0, 0.0};
+  }
+  if (g == 0 && tid == 0) a.d[k] = x[k].re;
+  __syncthreads();
+  if (k > n - 2) return;
+  if (g == 0) {
+    cd* row = (cd*)(k == 0 ? a.v0 : a.M + 2 * (size_t)k * n);
+    for (int i = tid; i < n; i += TRD_T) row[i] = vk[i];
+    if (k == 1)
+      for (int i = tid; i < n; i += TRD_T) ((cd*)a.M)[i] = ((const cd*)a.v0)[i];
+  }
+  if (!owns) {
+    if (tid == 0) a.pd[(k & 1) * G + g] = 0.0;
+    return;
+  }
+  const int cb = k & 1;
+  double ppart = 0.0;
+  for (int j = jw; j < n; j += TRD_W * G) {
+    cd* row = (cd*)(a.M + 2 * (size_t)j * n);
+    const cd vj = vp[j], wj = wp[j];
+    double accr = 0.0, acci = 0.0;
+    cd cj = {0.0, 0.0};
+    for (int base = k + 1; base < n; base += TRDC_CP * WAVE) {
+      cd cv[TRDC_CP];
+#pragma unroll
+      for (int t = 0; t < TRDC_CP; ++t) {
+        const int i = base + t * WAVE + lane;
+        cv[t] = i < n ? row[i] : cd{0.0, 0.0};
+      }
+#pragma unroll
+      for (int t = 0; t < TRDC_CP; ++t) {
+        const int i = base + t * WAVE + lane;
+        if (i < n) {
+          cd v = cv[t];
+          if (k >= 1) {
+            const cd p1 = cmulc(vj, wp[i]), p2 = cmulc(wj, vp[i]);
+            v = {v.re - p1.re - p2.re, v.im - p1.im - p2.im};
+          }
+          row[i] = v;
+          const cd q = cmul(v, vk[i]);
+          accr += q.re;
+          acci += q.im;
+          if (i == k + 1) cj = v;
+        }
+      }
+    }
+    const cd yj = {wave_sum(accr), wave_sum(acci)};
+    if (lane == 0) {
+      ((cd*)a.y)[(size_t)cb * n + j] = yj;
+      const cd ty = cmul(tk, yj);
+      ((cd*)a.u)[(size_t)cb * n + j] = {cj.re - ty.re, cj.im - ty.im};
+      ppart += yj.re * vk[j].re + yj.im * vk[j].im;  // Re(conj(y_j) v_j)
+    }
+  }
+  __syncthreads();
+  if (lane == 0) red[w] = ppart;
+  __syncthreads();
+  if (tid == 0) {
+    double s = 0.0;
+    for (int i = 0; i < TRD_W; ++i) s += red[i];
+    a.pd[cb * G + g] = s;
+  }
+}
+
+// Real eigenvectors of T (column-major n x n) -> complex (imaginary part 0).
+__global__ void trdc_expand(const double* __restrict__ Z, int64_t len, double* __restrict__ Zc) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= len) return;
+  Zc[2 * t] = Z[t];
+  Zc[2 * t + 1] = 0.0;
+}
+
+// out = conj(in) elementwise (complex interleaved)
+__global__ void trdc_conj(const double* __restrict__ in, int64_t len, double* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= len) return;
+  out[2 * t] = in[2 * t];
+  out[2 * t + 1] = -in[2 * t + 1];
+}
+
 // ---- compact WY form of reflector blocks: H_k0 ... H_k1 = I - V T V^T with
 // V = [v_k0 .. v_k1] and T upper triangular (LAPACK dlarft, forward, columnwise)
 constexpr int WY_B = 32;
@@ -418,6 +618,10 @@ int trd_min_n() {
 
 bool trd_supported(int64_t n) { return n >= trd_min_n() && n >= 3 && n <= TRD_MAX_N; }
 
+// Complex: 4 complex n-vectors in LDS and TRDC_R elements per thread.
+constexpr int TRDC_MAX_N = 2496;
+bool trdc_supported(int64_t n) { return n >= trd_min_n() && n >= 3 && n <= TRDC_MAX_N && n <= TRD_T * TRDC_R; }
+
 // Eigen-solve of the real symmetric n x n matrix S (destroyed) restricted to
 // what the pseudo-inverse needs: ev = eigenvalues (ascending), c (length n,
 // in/out) = Z^T Q^T c, with Z (n x n, row j = eigenvector j of T) left in Zbuf,
@@ -488,6 +692,65 @@ int trd_eigh_full(double* S, int n, double* ev, hipStream_t st) {
     if (s != rocblas_status_success) return fail(JV_E_SOLVER, "rocsolver_dormtr status " + std::to_string((int)s));
   }
   JV_HIP(hipMemcpyAsync(S, Z, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToDevice, st));
+  return JV_OK;
+}
+
+// Full eigendecomposition of the complex Hermitian S (row-major, destroyed):
+// ev ascending, row k of S <- conj(v_k) (the convention of the rocSOLVER path,
+// which decomposes the column-major view conj(S)).  Householder reduction to a
+// real tridiagonal T (trd_step_c), rocSOLVER dstedc for T, zunmtr for Q Z.
+int trdc_eigh_full(double* S, int n, double* ev, hipStream_t st) {
+  const int G = trd_groups(n);
+  const size_t wd = 13 * (size_t)n + 2 * (size_t)G + 64;
+  double* work = (double*)workspace(sizeof(double) * wd, 46);
+  double* Z = (double*)workspace(sizeof(double) * (size_t)n * n, 16);
+  double* Zc = (double*)workspace(sizeof(double) * 2 * (size_t)n * n, 47);
+  if (!work || !Z || !Zc) return fail(JV_E_NOMEM, "eigh: workspace allocation failed");
+  TrdcArgs a;
+  a.M = S;
+  a.d = ev;
+  a.e = work;
+  a.tau = work + n;
+  a.y = work + 3 * (size_t)n;
+  a.u = work + 7 * (size_t)n;
+  a.v0 = work + 11 * (size_t)n;
+  a.pd = work + 13 * (size_t)n;
+  a.n = n;
+  a.G = G;
+  const size_t lds = sizeof(double) * (8 * (size_t)n + 2 * TRD_W);
+  if (lds > 64 * 1024) {
+    hipError_t e = hipFuncSetAttribute((const void*)trd_step_c, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute(trd_step_c)");
+  }
+  {
+    ScopedTimer tm("tridiag", st);
+    for (int k = 0; k < n; ++k) hipLaunchKernelGGL(trd_step_c, dim3((unsigned)G), dim3(TRD_T), lds, st, a, k);
+    JV_LAUNCH_CHECK("trd_step_c");
+  }
+  rocblas_handle h = blas_handle(st);
+  if (!h) return fail(JV_E_SOLVER, "rocblas_create_handle failed");
+  rocblas_int* info = (rocblas_int*)(work + 13 * (size_t)n + 2 * G + 8);
+  {
+    ScopedTimer tm("stedc", st);
+    rocblas_status s = rocsolver_dstedc(h, rocblas_evect_tridiagonal, n, ev, a.e, Z, n, info);
+    if (s != rocblas_status_success) return fail(JV_E_SOLVER, "rocsolver_dstedc status " + std::to_string((int)s));
+  }
+  rocblas_int hinfo = 0;
+  JV_HIP(hipMemcpyAsync(&hinfo, info, sizeof(hinfo), hipMemcpyDeviceToHost, st));
+  JV_HIP(hipStreamSynchronize(st));
+  if (hinfo != 0) return fail(JV_E_SOLVER, "tridiagonal eigensolver did not converge (info " + std::to_string(hinfo) + ")");
+  const int64_t nn = (int64_t)n * n;
+  hipLaunchKernelGGL(trdc_expand, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, Z, nn, Zc);
+  JV_LAUNCH_CHECK("trdc_expand");
+  {
+    ScopedTimer tm("ormtr", st);
+    rocblas_status s = rocsolver_zunmtr(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, n, n,
+                                        (rocblas_double_complex*)S, n, (rocblas_double_complex*)a.tau,
+                                        (rocblas_double_complex*)Zc, n);
+    if (s != rocblas_status_success) return fail(JV_E_SOLVER, "rocsolver_zunmtr status " + std::to_string((int)s));
+  }
+  hipLaunchKernelGGL(trdc_conj, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, Zc, nn, S);
+  JV_LAUNCH_CHECK("trdc_conj");
   return JV_OK;
 }
 
