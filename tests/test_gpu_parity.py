@@ -325,9 +325,10 @@ def test_large_products_through_blas(cpx):
     assert np.array_equal(to_np(o.covar()), to_np(o.covar()))
 
 
-@pytest.mark.parametrize("cpx,n", [(False, 200), (True, 200), (False, 300), (False, 777), (True, 300), (True, 777)])
+@pytest.mark.parametrize("cpx,n", [(False, 100), (True, 100), (False, 200), (True, 200), (False, 300), (False, 777),
+                                   (True, 300), (True, 777)])
 def test_eigh(cpx, n):
-    """rocSOLVER below 256; from 256 up the Householder tridiagonalisation
+    """rocSOLVER below 128; from 128 up the Householder tridiagonalisation
     (real: dsytrd conventions; complex Hermitian: zhetrd conventions, real T)
     + divide and conquer + dormtr / zunmtr."""
     rng = np.random.default_rng(21)
@@ -390,9 +391,9 @@ def test_minsr_holomorphic_rbm():
 
 @pytest.mark.parametrize("mode,H", [("real", 8), ("imag", 8), ("real", 16), ("imag", 32)])
 def test_tdvp_exact(mode, H):
-    """H = 16 gives Pr = 304: the real S goes through the tridiagonal path;
-    H = 32 gives Pc = 296: the 'imag' solve diagonalises G^H G through the
-    complex Hermitian tridiagonal path."""
+    """H = 8 gives Pc = 80 (rocSOLVER for G^H G); H = 16 gives Pc = 152 and
+    H = 32 Pc = 296: G^H G goes through the complex Hermitian tridiagonal
+    path in both modes."""
     rng = np.random.default_rng(24)
     L = 8
     onet = onets.CpxRBM((L,), numHidden=H, bias=True, visibleBias=True)

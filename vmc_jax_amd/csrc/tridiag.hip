@@ -611,7 +611,7 @@ int trd_groups(int n) {
 int trd_min_n() {
   static int v = [] {
     const char* s = getenv("VMC_JAX_AMD_TRIDIAG_MIN_N");
-    return s ? atoi(s) : 256;
+    return s ? atoi(s) : 128;
   }();
   return v;
 }

@@ -42,10 +42,10 @@ class _Timer:
 
 # Smallest Pc for which each solve mode uses the holomorphic pair structure
 # ('imag': S has +-eig(G^H G), always cheaper than the 2Pc eigendecomposition;
-# 'real': S has every eigenvalue of G^H G twice, and for small Pc the real
-# tridiagonal path on the 2Pc matrix beats the complex Pc one: 6.4 vs 5.5 ms
-# at Pc = 230, 70 vs 154 ms at Pc = 2112 on MI355X).
-HOLOMORPHIC_SOLVE = {"real": 512, "imag": 1}
+# 'real': S has every eigenvalue of G^H G twice; the complex Pc
+# eigendecomposition beats the real 2Pc one once both run through the
+# Householder tridiagonal paths, i.e. from Pc = 128).
+HOLOMORPHIC_SOLVE = {"real": 128, "imag": 1}
 
 
 class TDVP:
