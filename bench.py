@@ -153,9 +153,11 @@ KERNELS = ("gram", "gemm", "eigh", "tridiag", "stedc", "ormtr", "trd_apply", "el
            "net_forward", "sym_mc", "sym_eloc", "sym_combine")
 
 
-def run_gpu(workload, steps, warmup, mode):
+def run_gpu(workload, steps, warmup, mode, overlap=True):
     import torch
     w = Workload(workload, mode)
+    if hasattr(w.eq, "overlap"):
+        w.eq.overlap = overlap
     jv = w.jv
     for _ in range(warmup):
         w.step()
@@ -370,6 +372,8 @@ def main():
     ap.add_argument("--mode", default="dual", choices=["T", "dual"], help="MinSR solve form (c3)")
     ap.add_argument("--t-steps", type=int, default=3, help="c3: steps of the other MinSR form reported beside")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-overlap", action="store_true",
+                    help="c3: compute E_loc before the MinSR solve instead of beside it (isolated kernel timings)")
     ap.add_argument("--serve-jobs", type=int, default=16, help="c3: jobs of the served mix (0 = skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="end with an elementwise 512 MiB -> 512 MiB FP64 device copy (a kernel of known "
@@ -392,7 +396,7 @@ def main():
                        "slots": rep["slots"]},
             "report": rep, "wall_s_including_setup": wall}))
         return
-    w, elapsed, kt, energies = run_gpu(args.workload, args.steps, args.warmup, args.mode)
+    w, elapsed, kt, energies = run_gpu(args.workload, args.steps, args.warmup, args.mode, not args.no_overlap)
     steps_s = args.steps / elapsed
     out = {
         "metric": METRIC if args.workload == "c3" else "TDVP steps/s (%s)" % args.workload,
@@ -414,6 +418,7 @@ def main():
     }
     if args.workload == "c3":
         out["config"]["solver_form"] = args.mode
+        out["config"]["eloc_beside_solve"] = not args.no_overlap and args.mode == "dual"
         out["roofline"] = roofline_c3(kt, args.steps, args.mode)
         if args.t_steps > 0:
             om = "T" if args.mode == "dual" else "dual"

@@ -315,6 +315,19 @@ int jv_minsr_solve_dual(const double* data, const double* e, int64_t n, int64_t 
                         int32_t is_complex, double rcond, double diag_shift, double* update,
                         jv_solve_info* info, void* stream);
 
+/* jv_minsr_solve_dual with the local estimator still in flight: e_raw is the
+ * uncentred E_loc (complex [n]) with the sample weights w [n], complete once the
+ * event e_ready (a hipEvent_t; NULL = ready in stream order) has fired.  The
+ * centring of e_raw (stats.py:197-205), b = D^dagger e and, on the real
+ * tridiagonal path, Q^T b run on the library's auxiliary stream, so the
+ * producer of e_raw overlaps the Gram and the Householder reduction, and Q^T b
+ * overlaps the tridiagonal eigensolver.  Same update as jv_minsr_solve_dual on
+ * the centred data.  No separate reference function (minsr.py:82-165 computes
+ * E_loc before the solve; the order does not change the result). */
+int jv_minsr_solve_dual_deferred(const double* data, const double* e_raw, const double* w, void* e_ready,
+                                 int64_t n, int64_t p, int32_t is_complex, double rcond, double diag_shift,
+                                 double* update, jv_solve_info* info, void* stream);
+
 /* TDVP solve (tdvp.py:134-213).  data: centred O in jVMC's flat layout, complex
  * [n][pr]; e: centred E_loc, complex [n]; w: weights [n]; x = rhsPrefactor
  * (complex, host); make_real 0 = 'real', 1 = 'imag'.  use_snr selects the SNR

@@ -545,6 +545,49 @@ def test_minsr_dual_equals_reference_form():
     assert rel(d, c) < 1e-9
 
 
+@pytest.mark.parametrize("shift", [0.0, 1e-3])
+def test_minsr_dual_deferred_equals_dual(shift):
+    """The deferred dual solve (E_loc produced on another stream, centred and
+    projected on the library's auxiliary stream) gives the dual update."""
+    psi, opsi, H, Ho, s, lp, p = _cnn_case((6, 6), (4, 4, 4), 1024, 27)
+    El = jv.stats.SampledObs(H.get_O_loc(s, psi, lp, 0.0), p)
+    G = jv.stats.SampledObs(psi.gradients(s, layout=0), p)
+    eq = jv.util.MinSR(None, pinvTol=1e-10, diagonalShift=shift, solver="dual")
+    u1 = to_np(eq.solve(El, G, False))
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        e2 = H.get_O_loc(s, psi, lp, 0.0).reshape(-1).contiguous()
+        ready = torch.cuda.Event()
+        ready.record(side)
+    u2 = to_np(eq._solve_deferred(e2, ready, G, p.reshape(-1).contiguous()))
+    torch.cuda.current_stream().wait_stream(side)
+    assert psi.numParameters == 332 and eq.last_info.rank > 0
+    assert rel(u2, u1) < 1e-12
+
+
+def test_minsr_dual_overlapped_call_matches_oracle():
+    """MinSR.__call__ with the dual form runs E_loc beside the solve; exact
+    sampler on a 3x3 lattice (512 states > P = 184)."""
+    rng = np.random.default_rng(28)
+    shape = (3, 3)
+    onet = onets.CNN(shape, F=(3, 3), channels=(4, 4), strides=(1, 1))
+    net = jv.nets.CNN(F=(3, 3), channels=(4, 4), strides=(1, 1))
+    psi = jv.NQS(net)
+    psi(torch.zeros((1, 1) + shape, dtype=torch.int32))
+    P = net.he_normal(psi.desc, rng)
+    psi.set_parameters(P)
+    opsi = onets.NQS(onet, P)
+    H, Ho = tfim_pair(shape, -1.0, -3.0)
+    es, eso = jv.sampler.ExactSampler(psi, shape), osamp.ExactSampler(opsi, shape)
+    eq, eqo = jv.util.MinSR(es, pinvTol=1e-10, solver="dual"), osolv.MinSR(eso, pinvTol=1e-10)
+    assert eq._overlapped(psi, 512, None)
+    upd = to_np(eq(psi.get_parameters(), 0.0, psi=psi, hamiltonian=H, intStep=0))
+    ref = eqo(opsi.get_parameters(), 0.0, psi=opsi, hamiltonian=Ho, intStep=0)
+    assert rel(upd, ref) < 1e-7
+    assert abs(eq.get_energy_mean() - eqo.get_energy_mean()) < 1e-11 * abs(eqo.get_energy_mean())
+
+
 @pytest.mark.parametrize("shape,ch,N,seed", [((10, 10), (8, 8, 8), 2048, 31), ((6, 6), (4, 4, 4), 128, 32)])
 def test_minsr_dual_tridiagonal(shape, ch, N, seed):
     """The dual solve through the Householder tridiagonalisation (P >= 256) equals

@@ -27,6 +27,8 @@ struct DeviceState {
   std::vector<void*> ptr;
   std::vector<size_t> size;
   rocblas_handle blas = nullptr;
+  hipStream_t aux = nullptr;             // second stream for work that overlaps the caller's
+  hipEvent_t ev[2] = {nullptr, nullptr};
 };
 // Workspaces and solver handles are per host thread and device, so threads
 // that drive different streams (one per job) never share scratch memory.
@@ -50,13 +52,17 @@ void release(DeviceState& s, int d) {
   s.size.clear();
   if (s.blas) rocblas_destroy_handle(s.blas);
   s.blas = nullptr;
+  for (auto& e : s.ev)
+    if (e) (void)hipEventDestroy(e), e = nullptr;
+  if (s.aux) (void)hipStreamDestroy(s.aux);
+  s.aux = nullptr;
   (void)hipSetDevice(cur);
 }
 
 ThreadState::~ThreadState() {
   std::lock_guard<std::mutex> lk(g_mu);
   for (size_t d = 0; d < dev.size(); ++d)
-    if (!dev[d].ptr.empty() || dev[d].blas) release(dev[d], (int)d);
+    if (!dev[d].ptr.empty() || dev[d].blas || dev[d].aux) release(dev[d], (int)d);
   for (size_t i = 0; i < g_threads.size(); ++i)
     if (g_threads[i] == this) { g_threads.erase(g_threads.begin() + i); break; }
 }
@@ -100,6 +106,24 @@ void* workspace(size_t bytes, int slot) {
   return s.ptr[slot];
 }
 
+// The calling thread's auxiliary stream on the current device (default
+// priority, non-blocking) and two timing-free events for joining it.
+bool aux_stream(hipStream_t* st, hipEvent_t* e0, hipEvent_t* e1) {
+  DeviceState& s = state();
+  if (!s.aux) {
+    if (hipStreamCreateWithFlags(&s.aux, hipStreamNonBlocking) != hipSuccess) {
+      s.aux = nullptr;
+      return false;
+    }
+    for (auto& e : s.ev)
+      if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return false;
+  }
+  *st = s.aux;
+  *e0 = s.ev[0];
+  *e1 = s.ev[1];
+  return true;
+}
+
 rocblas_handle blas_handle(hipStream_t stream) {
   DeviceState& s = state();
   if (!s.blas) {
@@ -128,7 +152,7 @@ int jv_shutdown(void) {
   // releases the calling thread's workspaces and handles on every device
   for (size_t d = 0; d < jv::t_state.dev.size(); ++d) {
     auto& s = jv::t_state.dev[d];
-    if (!s.ptr.empty() || s.blas) jv::release(s, (int)d);
+    if (!s.ptr.empty() || s.blas || s.aux) jv::release(s, (int)d);
   }
   return JV_OK;
 }

@@ -20,6 +20,11 @@ size_t trd_wy_offset(int n);
 int trd_eigh_full(double* S, int n, double* ev, hipStream_t st);
 bool trdc_supported(int64_t n);
 int trdc_eigh_full(double* S, int n, double* ev, hipStream_t st);
+int trd_reduce(double* S, int n, double* ev, double* c, double* work, hipStream_t st);
+int trd_tridiag_eig(int n, double* ev, double* work, double* Zbuf, hipStream_t st);
+int trd_wy_build(const double* V, const double* tau, int n, double* Tbuf, hipStream_t st);
+int trd_wy_apply(const double* V, int n, double* x, const double* Tbuf, bool transpose, hipStream_t st);
+bool aux_stream(hipStream_t* st, hipEvent_t* e0, hipEvent_t* e1);
 
 rocblas_handle blas_handle(hipStream_t stream);
 
@@ -67,10 +72,10 @@ __global__ void center_apply(const double* __restrict__ O, const double* __restr
 }
 
 int center(const double* O, const double* w, int64_t n, int64_t p, int cpx, double* mean, double* D,
-           hipStream_t st) {
+           hipStream_t st, int ws_slot = 2) {
   int nchunks = (int)((n + CHUNK_ROWS - 1) / CHUNK_ROWS);
   int64_t len = p * (cpx ? 2 : 1);
-  double* part = (double*)workspace(sizeof(double) * (size_t)nchunks * len, 2);
+  double* part = (double*)workspace(sizeof(double) * (size_t)nchunks * len, ws_slot);
   if (!part) return fail(JV_E_NOMEM, "jv_center: workspace allocation failed");
   hipLaunchKernelGGL(col_wsum_partial, dim3((unsigned)((p + 255) / 256), nchunks), dim3(256), 0, st, O, w, n,
                      p, cpx, part);
@@ -1141,6 +1146,10 @@ __global__ void neg_copy(const double* __restrict__ in, int64_t len, double* __r
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < len) out[i] = -in[i];
 }
+static int minsr_dual(const double* data, const double* e, const double* e_raw, const double* w,
+                      hipEvent_t e_ready, int64_t n, int64_t p, int32_t is_complex, double rcond, double diag_shift,
+                      double* update, hipStream_t st);
+static int minsr_dual_finish_info(int rc, jv_solve_info* info, int64_t p, double rcond, hipStream_t st);
 }  // namespace jv
 
 extern "C" {
@@ -1149,6 +1158,33 @@ int jv_minsr_solve_dual(const double* data, const double* e, int64_t n, int64_t 
                         double rcond, double diag_shift, double* update, jv_solve_info* info, void* stream) {
   if (!data || !e || !update || n < 1 || p < 1) return fail(JV_E_ARG, "jv_minsr_solve_dual: bad arguments");
   hipStream_t st = (hipStream_t)stream;
+  return minsr_dual_finish_info(minsr_dual(data, e, nullptr, nullptr, nullptr, n, p, is_complex, rcond, diag_shift,
+                                           update, st), info, p, rcond, st);
+}
+
+int jv_minsr_solve_dual_deferred(const double* data, const double* e_raw, const double* w, void* e_ready, int64_t n,
+                                 int64_t p, int32_t is_complex, double rcond, double diag_shift, double* update,
+                                 jv_solve_info* info, void* stream) {
+  if (!data || !e_raw || !w || !update || n < 1 || p < 1)
+    return fail(JV_E_ARG, "jv_minsr_solve_dual_deferred: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  return minsr_dual_finish_info(minsr_dual(data, nullptr, e_raw, w, (hipEvent_t)e_ready, n, p, is_complex, rcond,
+                                           diag_shift, update, st), info, p, rcond, st);
+}
+
+}  // extern "C"
+
+namespace jv {
+// Dual MinSR solve.  Either e (centred) is given, or e_raw with the weights w
+// and an event after which e_raw is complete (NULL: ready in stream order); e_raw
+// is then centred here (SampledObs, stats.py:197-205).  On the real tridiagonal
+// path the Householder reduction does not need b = D^T e: the producer of e_raw
+// overlaps the Gram and the reduction, and the centring, b and Q^T b run on the
+// auxiliary stream during dstedc.
+static int minsr_dual(const double* data, const double* e, const double* e_raw, const double* w,
+                      hipEvent_t e_ready, int64_t n, int64_t p, int32_t is_complex, double rcond, double diag_shift,
+                      double* update, hipStream_t st) {
+  void* stream = (void*)st;
   const int cpx = is_complex;
   double* S = (double*)workspace(sizeof(double) * (size_t)p * p * (cpx ? 2 : 1), 6);
   double* ev = (double*)workspace(sizeof(double) * (size_t)p, 7);
@@ -1165,13 +1201,64 @@ int jv_minsr_solve_dual(const double* data, const double* e, int64_t n, int64_t 
     hipLaunchKernelGGL(add_diag, dim3((unsigned)((p + 255) / 256)), dim3(256), 0, st, S, p, diag_shift, 0);
     JV_LAUNCH_CHECK("add_diag");
   }
-  const double* rhs = e;
-  if (!cpx) {
-    hipLaunchKernelGGL(real_part, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, e, n, er);
-    JV_LAUNCH_CHECK("real_part");
-    rhs = er;
+  const bool deferred = e == nullptr;
+  double* emean = deferred ? (double*)workspace(sizeof(double) * 2, 50) : nullptr;
+  double* ec = deferred ? (double*)workspace(sizeof(double) * 2 * (size_t)n, 51) : nullptr;
+  if (deferred && (!emean || !ec)) return fail(JV_E_NOMEM, "jv_minsr_solve_dual: workspace allocation failed");
+  // centred e (when deferred) and b = D^dagger e on stream s
+  auto rhs_b = [&](hipStream_t s) -> int {
+    const double* ee = e;
+    if (deferred) {
+      if (e_ready) JV_HIP(hipStreamWaitEvent(s, e_ready, 0));
+      int r = center(e_raw, w, n, 1, 1, emean, ec, s, 48);
+      if (r) return r;
+      ee = ec;
+    }
+    const double* rhs = ee;
+    if (!cpx) {
+      hipLaunchKernelGGL(real_part, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ee, n, er);
+      JV_LAUNCH_CHECK("real_part");
+      rhs = er;
+    }
+    return gemv(true, data, rhs, n, p, cpx, 1, 1.0, b, s, deferred ? 49 : 12);  // b = D^dagger e
+  };
+  if (deferred && !cpx && trd_supported(p)) {
+    hipStream_t s2;
+    hipEvent_t ev_red, ev_rhs;
+    if (!aux_stream(&s2, &ev_red, &ev_rhs)) return fail(JV_E_HIP, "jv_minsr_solve_dual: auxiliary stream creation failed");
+    double* Z = (double*)workspace(sizeof(double) * (size_t)p * p, 16);
+    double* tw = (double*)workspace(sizeof(double) * trd_work_doubles((int)p), 17);
+    if (!Z || !tw) return fail(JV_E_NOMEM, "jv_minsr_solve_dual: workspace allocation failed");
+    double* Tbuf = tw + trd_wy_offset((int)p);
+    {
+      ScopedTimer tm("eigh", st);
+      if ((rc = trd_reduce(S, (int)p, ev, nullptr, tw, st))) return rc;
+      JV_HIP(hipEventRecord(ev_red, st));
+      // auxiliary stream, after the reduction (so D and the reflectors are
+      // complete) and after e_raw: x = Q^T D^T e, overlapping dstedc below
+      JV_HIP(hipStreamWaitEvent(s2, ev_red, 0));
+      if ((rc = rhs_b(s2))) return rc;
+      if ((rc = trd_wy_build(S, tw + p, (int)p, Tbuf, s2))) return rc;
+      JV_HIP(hipMemcpyAsync(x, b, sizeof(double) * (size_t)p, hipMemcpyDeviceToDevice, s2));
+      if ((rc = trd_wy_apply(S, (int)p, x, Tbuf, true, s2))) return rc;
+      JV_HIP(hipEventRecord(ev_rhs, s2));
+      if ((rc = trd_tridiag_eig((int)p, ev, tw, Z, st))) return rc;
+    }
+    JV_HIP(hipStreamWaitEvent(st, ev_rhs, 0));
+    if ((rc = gemv(false, Z, x, p, p, 0, 0, 1.0, y, st, 12))) return rc;
+    hipLaunchKernelGGL(pinv_scale, dim3(1), dim3(1024), 0, st, ev, p, rcond, 0, y, rank);
+    JV_LAUNCH_CHECK("pinv_scale");
+    if ((rc = gemv(true, Z, y, p, p, 0, 1, 1.0, x, st, 12))) return rc;
+    {
+      ScopedTimer tm("trd_apply", st);
+      if ((rc = trd_wy_apply(S, (int)p, x, Tbuf, false, st))) return rc;
+    }
+    int64_t len = p;
+    hipLaunchKernelGGL(neg_copy, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, st, x, len, update);
+    JV_LAUNCH_CHECK("neg_copy");
+    return JV_OK;
   }
-  if ((rc = gemv(true, data, rhs, n, p, cpx, 1, 1.0, b, st, 12))) return rc;  // b = D^dagger e
+  if ((rc = rhs_b(st))) return rc;
   if (!cpx && trd_supported(p)) {
     // S = Q Z diag(ev) Z^T Q^T: Householder tridiagonalisation (Q^T b formed on
     // the way), divide and conquer on T, then x = Q Z pinv(ev) Z^T Q^T b
@@ -1198,16 +1285,22 @@ int jv_minsr_solve_dual(const double* data, const double* e, int64_t n, int64_t 
   int64_t len = p * (cpx ? 2 : 1);
   hipLaunchKernelGGL(neg_copy, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, st, x, len, update);
   JV_LAUNCH_CHECK("neg_copy");
-  if (info) {
-    int hr = 0;
-    JV_HIP(hipMemcpyAsync(&hr, rank, sizeof(int), hipMemcpyDeviceToHost, st));
-    JV_HIP(hipStreamSynchronize(st));
-    info->rank = hr;
-    info->iterations = 0;
-    info->residual = 0.0;
-    info->pinv_cutoff = rcond;
-  }
   return JV_OK;
 }
 
-}  // extern "C"
+// Reads the pseudo-inverse rank back into info (after a successful solve).
+static int minsr_dual_finish_info(int rc, jv_solve_info* info, int64_t p, double rcond, hipStream_t st) {
+  if (rc || !info) return rc;
+  int* rank = (int*)workspace(64, 11);
+  if (!rank) return fail(JV_E_NOMEM, "jv_minsr_solve_dual: workspace allocation failed");
+  int hr = 0;
+  JV_HIP(hipMemcpyAsync(&hr, rank, sizeof(int), hipMemcpyDeviceToHost, st));
+  JV_HIP(hipStreamSynchronize(st));
+  info->rank = hr;
+  info->iterations = 0;
+  info->residual = 0.0;
+  info->pinv_cutoff = rcond;
+  (void)p;
+  return JV_OK;
+}
+}  // namespace jv

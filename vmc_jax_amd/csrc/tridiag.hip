@@ -519,6 +519,9 @@ __global__ void __launch_bounds__(256) trd_wy_t(const double* __restrict__ V, co
 // x <- Q x, blocks applied last to first: x -= V (T (V^T x)).  The reflector
 // loop is unrolled to WY_B so each thread's loads of a block are issued
 // together (one memory latency per phase, not one per reflector).
+// x <- Q x (TR = false: blocks last to first with T) or x <- Q^T x (TR = true:
+// blocks first to last with T^T).
+template <bool TR>
 __global__ void __launch_bounds__(TRD_T) trd_apply_wy(const double* __restrict__ V, const double* __restrict__ Tg,
                                                       int n, int nref, double* __restrict__ x) {
   __shared__ double part[TRD_W][WY_B];
@@ -531,7 +534,8 @@ __global__ void __launch_bounds__(TRD_T) trd_apply_wy(const double* __restrict__
     xr[r] = i < n ? x[i] : 0.0;
   }
   const int nblk = (nref + WY_B - 1) / WY_B;
-  for (int b = nblk - 1; b >= 0; --b) {
+  for (int bi = 0; bi < nblk; ++bi) {
+    const int b = TR ? bi : nblk - 1 - bi;
     const int k0 = b * WY_B, nb = min(WY_B, nref - k0);
     // w = V^T x
     double acc[WY_B];
@@ -561,18 +565,18 @@ __global__ void __launch_bounds__(TRD_T) trd_apply_wy(const double* __restrict__
       wv[tid] = t;
     }
     __syncthreads();
-    // t = T w (zero past the block's last reflector, whose rows are never
-    // weighted: acc[p >= nb] is ignored and tv[p >= nb] = 0)
+    // t = T w or T^T w (zero past the block's last reflector, whose rows are
+    // never weighted: acc[p >= nb] is ignored and tv[p >= nb] = 0)
     if (tid >= nb && tid < WY_B) tv[tid] = 0.0;
     if (tid < nb) {
-      const double* Tb = Tg + (size_t)b * WY_B * WY_B + (size_t)tid * WY_B;
+      const double* Tb = Tg + (size_t)b * WY_B * WY_B;
       double tr[WY_B];
 #pragma unroll
-      for (int l = 0; l < WY_B; ++l) tr[l] = Tb[l];
+      for (int l = 0; l < WY_B; ++l) tr[l] = TR ? Tb[(size_t)l * WY_B + tid] : Tb[(size_t)tid * WY_B + l];
       double t = 0.0;
 #pragma unroll
       for (int l = 0; l < WY_B; ++l)
-        if (l >= tid && l < nb) t = fma(tr[l], wv[l], t);
+        if ((TR ? l <= tid : l >= tid) && l < nb) t = fma(tr[l], wv[l], t);
       tv[tid] = t;
     }
     __syncthreads();
@@ -626,7 +630,7 @@ bool trdc_supported(int64_t n) { return n >= trd_min_n() && n >= 3 && n <= TRDC_
 // what the pseudo-inverse needs: ev = eigenvalues (ascending), c (length n,
 // in/out) = Z^T Q^T c, with Z (n x n, row j = eigenvector j of T) left in Zbuf,
 // and the reflectors in S for trd_apply.  Workspaces come from the caller.
-int trd_eig_solve(double* S, int n, double* ev, double* c, double* Zbuf, double* work, hipStream_t st) {
+int trd_reduce(double* S, int n, double* ev, double* c, double* work, hipStream_t st) {
   const int G = trd_groups(n);
   TrdArgs a;
   a.M = S;
@@ -652,14 +656,20 @@ int trd_eig_solve(double* S, int n, double* ev, double* c, double* Zbuf, double*
     }
     JV_LAUNCH_CHECK("trd_step");
   }
-  // eigenvectors of T (rocSOLVER divide and conquer): column-major Z, so row j
-  // of the row-major buffer is eigenvector j
+  return JV_OK;
+}
+
+// Eigenvalues (ev: the diagonal of T on entry) and eigenvectors of the
+// tridiagonal T left by trd_reduce in work (rocSOLVER divide and conquer):
+// column-major Z, so row j of the row-major buffer is eigenvector j.
+int trd_tridiag_eig(int n, double* ev, double* work, double* Zbuf, hipStream_t st) {
+  const int G = trd_groups(n);
   rocblas_handle h = blas_handle(st);
   if (!h) return fail(JV_E_SOLVER, "rocblas_create_handle failed");
   rocblas_int* info = (rocblas_int*)(work + 7 * (size_t)n + 2 * G + 8);
   {
     ScopedTimer tm("stedc", st);
-    rocblas_status s = rocsolver_dstedc(h, rocblas_evect_tridiagonal, n, ev, a.e, Zbuf, n, info);
+    rocblas_status s = rocsolver_dstedc(h, rocblas_evect_tridiagonal, n, ev, work, Zbuf, n, info);
     if (s != rocblas_status_success)
       return fail(JV_E_SOLVER, "rocsolver_dstedc status " + std::to_string((int)s));
   }
@@ -668,6 +678,11 @@ int trd_eig_solve(double* S, int n, double* ev, double* c, double* Zbuf, double*
   JV_HIP(hipStreamSynchronize(st));
   if (hinfo != 0) return fail(JV_E_SOLVER, "tridiagonal eigensolver did not converge (info " + std::to_string(hinfo) + ")");
   return JV_OK;
+}
+
+int trd_eig_solve(double* S, int n, double* ev, double* c, double* Zbuf, double* work, hipStream_t st) {
+  int rc = trd_reduce(S, n, ev, c, work, st);
+  return rc ? rc : trd_tridiag_eig(n, ev, work, Zbuf, st);
 }
 
 size_t trd_work_doubles(int n);
@@ -758,14 +773,27 @@ int trd_wy_blocks(int n) { return (n - 1 + WY_B - 1) / WY_B; }
 
 // x <- Q x with the reflectors in the rows of V and tau; Tbuf holds
 // trd_wy_blocks(n) * WY_B^2 doubles.
-int trd_apply(const double* V, const double* tau, int n, double* x, double* Tbuf, hipStream_t st) {
+int trd_wy_build(const double* V, const double* tau, int n, double* Tbuf, hipStream_t st) {
   const int nref = n - 1, nblk = trd_wy_blocks(n);
-  ScopedTimer tm("trd_apply", st);
   hipLaunchKernelGGL(trd_wy_t, dim3((unsigned)nblk), dim3(256), 0, st, V, tau, n, nref, Tbuf);
   JV_LAUNCH_CHECK("trd_wy_t");
-  hipLaunchKernelGGL(trd_apply_wy, dim3(1), dim3(TRD_T), 0, st, V, Tbuf, n, nref, x);
+  return JV_OK;
+}
+
+// x <- Q x (transpose false) or Q^T x with the T factors of trd_wy_build.
+int trd_wy_apply(const double* V, int n, double* x, const double* Tbuf, bool transpose, hipStream_t st) {
+  if (transpose)
+    hipLaunchKernelGGL(trd_apply_wy<true>, dim3(1), dim3(TRD_T), 0, st, V, Tbuf, n, n - 1, x);
+  else
+    hipLaunchKernelGGL(trd_apply_wy<false>, dim3(1), dim3(TRD_T), 0, st, V, Tbuf, n, n - 1, x);
   JV_LAUNCH_CHECK("trd_apply_wy");
   return JV_OK;
+}
+
+int trd_apply(const double* V, const double* tau, int n, double* x, double* Tbuf, hipStream_t st) {
+  ScopedTimer tm("trd_apply", st);
+  int rc = trd_wy_build(V, tau, n, Tbuf, st);
+  return rc ? rc : trd_wy_apply(V, n, x, Tbuf, false, st);
 }
 
 size_t trd_work_doubles(int n) {

@@ -64,26 +64,68 @@ class MinSR:
             return psi.expand_compact(0.5 * upd)
         return upd
 
+    # Real dual solves compute E_loc on a side stream while the solve builds and
+    # reduces the Gram matrix (set False to run the phases one after another).
+    overlap = True
+
+    def _overlapped(self, psi, n, outp):
+        """The real dual solve runs with E_loc still in flight (not when per-phase
+        timing is requested: phases then overlap)."""
+        if not self.overlap or outp is not None or psi.holomorphic or getattr(psi, "_cpx_weights", False):
+            return False
+        return self.solver == "dual" or (self.solver == "auto" and n > psi.numParameters)
+
+    def _solve_deferred(self, Eraw, ready, gradients, p):
+        D = gradients._data.contiguous()
+        n, P = D.shape
+        upd = torch.empty(P, dtype=torch.float64, device=D.device)
+        info = _lib.JvSolveInfo()
+        _lib.call("jv_minsr_solve_dual_deferred", _lib.ptr(D), _lib.ptr(Eraw), _lib.ptr(p), ready._as_parameter_,
+                  n, P, 0, float(self.pinvTol), float(self.diagonalShift), _lib.ptr(upd), ctypes.byref(info),
+                  _lib.stream_ptr())
+        self.last_info = info
+        return upd
+
     def __call__(self, netParameters, t, *, psi, hamiltonian, **rhsArgs):
         """minsr.py:82-165."""
         from ..stats import SampledObs
         tmpParameters = psi.get_parameters()
         psi.set_parameters(netParameters)
-        timer = _Timer(rhsArgs.get("outp", None))
+        outp = rhsArgs.get("outp", None)
+        timer = _Timer(outp)
         numSamples = rhsArgs.get("numSamples", None)
         timer.start("sampling")
         sampleConfigs, sampleLogPsi, p = self.sampler.sample(numSamples=numSamples)
         timer.stop("sampling")
-        timer.start("compute Eloc")
-        Eloc = hamiltonian.get_O_loc(sampleConfigs, psi, sampleLogPsi, t)
-        timer.stop("compute Eloc")
-        Eloc = SampledObs(Eloc, p)
-        timer.start("compute gradients")
-        sampleGradients = SampledObs(psi.gradients(sampleConfigs, layout=0), p)
-        timer.stop("compute gradients")
-        timer.start("solve MinSR eqn.")
-        update = self.solve(Eloc, sampleGradients, psi.holomorphic, psi)
-        timer.stop("solve MinSR eqn.")
+        if self._overlapped(psi, sampleConfigs.shape[1], outp):
+            # E_loc on a side stream; the solve waits for it only where it needs it
+            main = torch.cuda.current_stream()
+            if getattr(self, "_side", None) is None:
+                self._side = torch.cuda.Stream()
+            side = self._side
+            side.wait_stream(main)
+            with torch.cuda.stream(side):
+                Eraw = hamiltonian.get_O_loc(sampleConfigs, psi, sampleLogPsi, t)
+                Eraw = Eraw.reshape(-1).to(torch.complex128).contiguous()
+                ready = torch.cuda.Event()
+                ready.record(side)
+            sampleGradients = SampledObs(psi.gradients(sampleConfigs, layout=0), p)
+            w = p.reshape(-1).to(torch.float64).contiguous()
+            update = self._solve_deferred(Eraw, ready, sampleGradients, w)
+            main.wait_stream(side)
+            Eraw.record_stream(main)
+            Eloc = SampledObs(Eraw.reshape(1, -1), p)
+        else:
+            timer.start("compute Eloc")
+            Eloc = hamiltonian.get_O_loc(sampleConfigs, psi, sampleLogPsi, t)
+            timer.stop("compute Eloc")
+            Eloc = SampledObs(Eloc, p)
+            timer.start("compute gradients")
+            sampleGradients = SampledObs(psi.gradients(sampleConfigs, layout=0), p)
+            timer.stop("compute gradients")
+            timer.start("solve MinSR eqn.")
+            update = self.solve(Eloc, sampleGradients, psi.holomorphic, psi)
+            timer.stop("solve MinSR eqn.")
         psi.set_parameters(tmpParameters)
         if rhsArgs.get("intStep", None) == 0:
             self.ElocMean0 = complex(Eloc.mean()[0].item())
