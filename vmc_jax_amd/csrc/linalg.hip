@@ -328,12 +328,12 @@ int gemm_blas_min_flops_log2() {
   return v;
 }
 
-int gemm(const GemmArgs& g0, hipStream_t st) {
+int gemm(const GemmArgs& g0, hipStream_t st, bool allow_blas = true) {
   GemmArgs g = g0;
   g.part = nullptr;
   g.kchunk = 0;
   if (g.M <= 0 || g.N <= 0) return JV_OK;
-  {
+  if (allow_blas) {
     const double flops = 2.0 * (double)g.M * (double)g.N * (double)g.K * (g.cpx ? 4.0 : 1.0);
     const bool ours = g.sym != 0;  // Hermitian Grams: triangle tiles halve the work
     if (!ours && flops >= std::ldexp(1.0, gemm_blas_min_flops_log2())) {
@@ -536,6 +536,19 @@ __global__ void pinv_scale(const double* __restrict__ ev, int64_t n, double rcon
 __global__ void add_diag(double* __restrict__ T, int64_t n, double shift, int cpx) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) T[(i * n + i) * (cpx ? 2 : 1)] += shift;
+}
+
+__global__ void set_identity(double* __restrict__ Q, int64_t n) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n * n) Q[t] = (t / n == t % n) ? 1.0 : 0.0;
+}
+
+// Q[off + i][off + j] -= U[i][j] for i, j < m
+__global__ void sub_trailing(double* __restrict__ Q, int64_t n, int64_t off, int64_t m, const double* __restrict__ U) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m * m) return;
+  const int64_t i = t / m, j = t - i * m;
+  Q[(off + i) * n + off + j] -= U[t];
 }
 
 __global__ void real_part(const double* __restrict__ in, int64_t n, double* __restrict__ out) {
@@ -1175,6 +1188,35 @@ int jv_minsr_solve_dual_deferred(const double* data, const double* e_raw, const 
 }  // extern "C"
 
 namespace jv {
+// Q = H_0 ... H_{n-2} (reflectors in the rows of V, compact-WY T factors in
+// Tbuf) formed explicitly: blocks applied last to first to the identity, each
+// touching only the trailing (n - k0 - 1) square, as three products
+// W = V_b^T Q, W2 = T_b W, U = V_b W2 and Q -= U.  Own GEMM kernels only (no
+// rocBLAS handle), so it may run beside a rocSOLVER call on another stream.
+static int trd_form_q(const double* V, int n, const double* Tbuf, double* Q, double* W, double* W2, double* U,
+                      hipStream_t st) {
+  const int64_t nn = (int64_t)n * n;
+  hipLaunchKernelGGL(set_identity, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, Q, (int64_t)n);
+  JV_LAUNCH_CHECK("set_identity");
+  const int nref = n - 1, wb = 32, nblk = (nref + wb - 1) / wb;
+  for (int b = nblk - 1; b >= 0; --b) {
+    const int64_t k0 = (int64_t)b * wb, nb = std::min<int64_t>(wb, nref - k0);
+    const int64_t off = k0 + 1, m = n - off;
+    const double* Vb = V + k0 * n + off;  // V_b^T(p, i) = Vb[p * n + i]
+    const double* Tb = Tbuf + (size_t)b * wb * wb;
+    int rc;
+    GemmArgs g1{nb, m, m, Vb, n, 1, 0, Q + off * n + off, n, 1, 0, W, m, 0, 0, 1.0};
+    if ((rc = gemm(g1, st, false))) return rc;
+    GemmArgs g2{nb, m, nb, Tb, wb, 1, 0, W, m, 1, 0, W2, m, 0, 0, 1.0};
+    if ((rc = gemm(g2, st, false))) return rc;
+    GemmArgs g3{m, m, nb, Vb, 1, n, 0, W2, m, 1, 0, U, m, 0, 0, 1.0};
+    if ((rc = gemm(g3, st, false))) return rc;
+    hipLaunchKernelGGL(sub_trailing, dim3((unsigned)((m * m + 255) / 256)), dim3(256), 0, st, Q, (int64_t)n, off, m, U);
+    JV_LAUNCH_CHECK("sub_trailing");
+  }
+  return JV_OK;
+}
+
 // Dual MinSR solve.  Either e (centred) is given, or e_raw with the weights w
 // and an event after which e_raw is complete (NULL: ready in stream order); e_raw
 // is then centred here (SampledObs, stats.py:197-205).  On the real tridiagonal
@@ -1230,17 +1272,23 @@ static int minsr_dual(const double* data, const double* e, const double* e_raw, 
     double* tw = (double*)workspace(sizeof(double) * trd_work_doubles((int)p), 17);
     if (!Z || !tw) return fail(JV_E_NOMEM, "jv_minsr_solve_dual: workspace allocation failed");
     double* Tbuf = tw + trd_wy_offset((int)p);
+    const size_t pp = (size_t)p * p;
+    double* Q = (double*)workspace(sizeof(double) * pp, 52);
+    double* Wq = (double*)workspace(sizeof(double) * 2 * 32 * (size_t)p, 53);
+    double* Uq = (double*)workspace(sizeof(double) * pp, 54);
+    if (!Q || !Wq || !Uq) return fail(JV_E_NOMEM, "jv_minsr_solve_dual: workspace allocation failed");
     {
       ScopedTimer tm("eigh", st);
       if ((rc = trd_reduce(S, (int)p, ev, nullptr, tw, st))) return rc;
       JV_HIP(hipEventRecord(ev_red, st));
       // auxiliary stream, after the reduction (so D and the reflectors are
-      // complete) and after e_raw: x = Q^T D^T e, overlapping dstedc below
+      // complete) and after e_raw: the explicit Q and x = Q^T D^T e, both
+      // overlapping dstedc below
       JV_HIP(hipStreamWaitEvent(s2, ev_red, 0));
-      if ((rc = rhs_b(s2))) return rc;
       if ((rc = trd_wy_build(S, tw + p, (int)p, Tbuf, s2))) return rc;
-      JV_HIP(hipMemcpyAsync(x, b, sizeof(double) * (size_t)p, hipMemcpyDeviceToDevice, s2));
-      if ((rc = trd_wy_apply(S, (int)p, x, Tbuf, true, s2))) return rc;
+      if ((rc = trd_form_q(S, (int)p, Tbuf, Q, Wq, Wq + 32 * (size_t)p, Uq, s2))) return rc;
+      if ((rc = rhs_b(s2))) return rc;
+      if ((rc = gemv(true, Q, b, p, p, 0, 0, 1.0, x, s2, 49))) return rc;
       JV_HIP(hipEventRecord(ev_rhs, s2));
       if ((rc = trd_tridiag_eig((int)p, ev, tw, Z, st))) return rc;
     }
@@ -1251,10 +1299,10 @@ static int minsr_dual(const double* data, const double* e, const double* e_raw, 
     if ((rc = gemv(true, Z, y, p, p, 0, 1, 1.0, x, st, 12))) return rc;
     {
       ScopedTimer tm("trd_apply", st);
-      if ((rc = trd_wy_apply(S, (int)p, x, Tbuf, false, st))) return rc;
+      if ((rc = gemv(false, Q, x, p, p, 0, 0, 1.0, y, st, 12))) return rc;  // y = Q x
     }
     int64_t len = p;
-    hipLaunchKernelGGL(neg_copy, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, st, x, len, update);
+    hipLaunchKernelGGL(neg_copy, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, st, y, len, update);
     JV_LAUNCH_CHECK("neg_copy");
     return JV_OK;
   }

@@ -516,11 +516,10 @@ __global__ void __launch_bounds__(256) trd_wy_t(const double* __restrict__ V, co
   for (int e = tid; e < WY_B * WY_B; e += 256) Tout[(size_t)blockIdx.x * WY_B * WY_B + e] = T[e / WY_B][e % WY_B];
 }
 
-// x <- Q x, blocks applied last to first: x -= V (T (V^T x)).  The reflector
-// loop is unrolled to WY_B so each thread's loads of a block are issued
-// together (one memory latency per phase, not one per reflector).
-// x <- Q x (TR = false: blocks last to first with T) or x <- Q^T x (TR = true:
-// blocks first to last with T^T).
+// x <- Q x (TR = false: blocks last to first, x -= V (T (V^T x))) or
+// x <- Q^T x (TR = true: blocks first to last with T^T).  The reflector loop
+// is unrolled to WY_B so each thread's loads of a block are issued together
+// (one memory latency per phase, not one per reflector).
 template <bool TR>
 __global__ void __launch_bounds__(TRD_T) trd_apply_wy(const double* __restrict__ V, const double* __restrict__ Tg,
                                                       int n, int nref, double* __restrict__ x) {
