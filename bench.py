@@ -30,7 +30,9 @@ samples, imaginary-time TDVP (Euler dt=1e-2).
 
 Prints one JSON line with steps/s, samples/s, per-kernel times, the roofline of
 the dominant hand-written kernel (from HIP events recorded live on its stream)
-and the CPU baseline (the FP64 oracle on a bounded sample of the same step).
+and the CPU baseline (the FP64 oracle on a bounded sample of the same step);
+the same oracle leg checks |dE/E| and the MinSR update against the FP64 oracle
+on fresh samples of the final state (`accuracy`).
 """
 import argparse
 import ctypes
@@ -363,6 +365,32 @@ def cpu_baseline(n_sub=1024, chains=512):
                                                          {k: round(v, 3) for k, v in t.items()}))}
 
 
+def accuracy_check(w, n=512):
+    """|dE/E| and the relative error of the MinSR update against the FP64 oracle
+    on the same configurations and weights (n fresh samples of the workload's
+    current state): the accuracy condition of the metric."""
+    import torch
+    from oracle import nets as onets, operator as oop, solvers as osolv
+    from oracle.stats import SampledObs as OObs
+    jv = w.jv
+    s, lp, p = w.sampler.sample(numSamples=n)
+    El = jv.stats.SampledObs(w.H.get_O_loc(s, w.psi, lp, 0.0), p)
+    G = jv.stats.SampledObs(w.psi.gradients(s, layout=0), p)
+    upd = jv.util.MinSR(None, pinvTol=1e-10, solver="dual").solve(El, G, False).cpu().numpy()
+    e_gpu = complex(El.mean()[0].item())
+    onet = onets.CNN(w.shape, F=(3, 3), channels=(8, 8, 8), strides=(1, 1))
+    opsi = onets.NQS(onet, w.psi.get_parameters().cpu().numpy(), batchSize=4096)
+    cs = s[0].cpu().numpy()
+    pw = p[0].cpu().numpy()
+    Ho = tfim_strings(oop, w.shape)
+    Eo = OObs(Ho.get_O_loc(cs, opsi, opsi(cs), 0.0), pw)
+    ref = osolv.MinSR(None, pinvTol=1e-10).solve(Eo, OObs(opsi.gradients(cs), pw), False)
+    e_o = complex(Eo.mean()[0])
+    return {"samples": int(n), "against": "FP64 numpy oracle (T-form MinSR) on the same configurations and weights",
+            "energy": e_gpu.real, "dE_rel": abs(e_gpu - e_o) / abs(e_o),
+            "dtheta_rel": float(np.linalg.norm(upd - ref) / np.linalg.norm(ref))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -432,7 +460,11 @@ def main():
                 out["served"] = serve_mix(args.serve_jobs)
             except Exception as e:
                 out["served"] = {"error": repr(e)}
-        if not args.no_cpu_baseline:
+        if not args.no_cpu_baseline:  # the oracle leg: accuracy checker and CPU baseline
+            try:
+                out["accuracy"] = accuracy_check(w)
+            except Exception as e:
+                out["accuracy"] = {"error": repr(e)}
             try:
                 out["cpu_baseline"] = cpu_baseline()
             except Exception as e:  # report, never hide the GPU number

@@ -111,12 +111,19 @@ void* workspace(size_t bytes, int slot) {
 bool aux_stream(hipStream_t* st, hipEvent_t* e0, hipEvent_t* e1) {
   DeviceState& s = state();
   if (!s.aux) {
-    if (hipStreamCreateWithFlags(&s.aux, hipStreamNonBlocking) != hipSuccess) {
-      s.aux = nullptr;
+    hipStream_t a = nullptr;
+    hipEvent_t e[2] = {nullptr, nullptr};
+    bool ok = hipStreamCreateWithFlags(&a, hipStreamNonBlocking) == hipSuccess;
+    for (auto& x : e) ok = ok && hipEventCreateWithFlags(&x, hipEventDisableTiming) == hipSuccess;
+    if (!ok) {  // leave nothing half made
+      for (auto& x : e)
+        if (x) (void)hipEventDestroy(x);
+      if (a) (void)hipStreamDestroy(a);
       return false;
     }
-    for (auto& e : s.ev)
-      if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return false;
+    s.aux = a;
+    s.ev[0] = e[0];
+    s.ev[1] = e[1];
   }
   *st = s.aux;
   *e0 = s.ev[0];
