@@ -289,11 +289,24 @@ int jv_gram_nn(const double* data, int64_t n, int64_t p, int32_t is_complex, dou
 int jv_gram_pp(const double* data, int64_t n, int64_t p, int32_t is_complex, double* S,
                void* stream);
 
-/* Symmetric / Hermitian eigendecomposition (ascending eigenvalues) in FP64 via
- * rocSOLVER syevd / heevd.  A [n][n] is overwritten by eigenvectors stored as
- * COLUMNS of the row-major matrix (A[i][k] = component i of eigenvector k).
+/* Symmetric / Hermitian eigendecomposition (ascending eigenvalues) in FP64:
+ * for n >= 128 own Householder tridiagonalisation kernels, the tridiagonal
+ * divide and conquer of jv_stedc (rocSOLVER dstedc with
+ * VMC_JAX_AMD_STEDC=rocsolver) and the back-transformation; below that rocSOLVER
+ * syevd / heevd.  A [n][n] is overwritten by eigenvectors stored as COLUMNS of
+ * the row-major matrix (A[i][k] = component i of eigenvector k).
  * Replaces jnp.linalg.eigh (tdvp.py:153-171). */
 int jv_eigh(int64_t n, int32_t is_complex, double* A, double* w, void* stream);
+
+/* Eigenvalues and eigenvectors of the real symmetric tridiagonal matrix with
+ * diagonal d[n] and off-diagonal e[n-1], by divide and conquer on the device
+ * (stream ordered, no host synchronisation in steady state).  On exit d holds
+ * the eigenvalues ascending and z [n][n] the eigenvectors as ROWS (z[k][i] =
+ * component i of eigenvector k); e is not modified.  n <= 3276.  No
+ * counterpart in the reference, where jnp.linalg.eigh (tdvp.py:153-171,
+ * minsr.py:74) runs the whole decomposition: it is the tridiagonal stage of
+ * jv_eigh, exposed for callers that bring their own reduction. */
+int jv_stedc(int64_t n, double* d, const double* e, double* z, void* stream);
 
 /* MinSR update (minsr.py:53-80) from centred data.  Holomorphic (is_complex
  * with holo=1): T = D D^dagger, update = -D^dagger pinv(T) e with the jVMC flat

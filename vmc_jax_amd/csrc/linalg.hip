@@ -328,7 +328,9 @@ int gemm_blas_min_flops_log2() {
   return v;
 }
 
-int gemm(const GemmArgs& g0, hipStream_t st, bool allow_blas = true) {
+// Split-K partial tiles live in workspace slot part_slot: callers on different
+// streams that may run at the same time pass different slots.
+int gemm(const GemmArgs& g0, hipStream_t st, bool allow_blas = true, int part_slot = 18) {
   GemmArgs g = g0;
   g.part = nullptr;
   g.kchunk = 0;
@@ -352,7 +354,7 @@ int gemm(const GemmArgs& g0, hipStream_t st, bool allow_blas = true) {
   ScopedTimer tm(g.sym ? "gram" : "gemm", st);
   if (splits > 1) {
     const size_t bytes = sizeof(double) * (size_t)splits * g.M * g.N * (g.cpx ? 2 : 1);
-    g.part = (double*)workspace(bytes, 18);
+    g.part = (double*)workspace(bytes, part_slot);
     if (!g.part) return fail(JV_E_NOMEM, "gemm: split-K workspace allocation failed");
     g.kchunk = ((g.K + splits - 1) / splits + BK - 1) / BK * BK;
     splits = (int)((g.K + g.kchunk - 1) / g.kchunk);
@@ -428,6 +430,13 @@ __global__ void gemv_t_partial(const double* __restrict__ M, const double* __res
 __global__ void scale_vec(double* __restrict__ v, int64_t len, double alpha) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < len) v[i] *= alpha;
+}
+
+// Real C[M][N] = opA opB with the strides of GemmArgs (no conjugation, alpha 1).
+int gemm_real(int64_t M, int64_t N, int64_t K, const double* A, int64_t sam, int64_t sak, const double* B,
+              int64_t sbk, int64_t sbn, double* C, int64_t ldc, hipStream_t st) {
+  GemmArgs g{M, N, K, A, sam, sak, 0, B, sbk, sbn, 0, C, ldc, 0, 0, 1.0};
+  return gemm(g, st);
 }
 
 int gemv(bool trans, const double* M, const double* x, int64_t rows, int64_t cols, int cpx, int conjM,
@@ -1192,7 +1201,9 @@ namespace jv {
 // Tbuf) formed explicitly: blocks applied last to first to the identity, each
 // touching only the trailing (n - k0 - 1) square, as three products
 // W = V_b^T Q, W2 = T_b W, U = V_b W2 and Q -= U.  Own GEMM kernels only (no
-// rocBLAS handle), so it may run beside a rocSOLVER call on another stream.
+// rocBLAS handle) with split-K scratch in slot 94, so it may run beside the
+// tridiagonal eigensolver (rocSOLVER's or the GEMMs of dc.hip) on another
+// stream.
 static int trd_form_q(const double* V, int n, const double* Tbuf, double* Q, double* W, double* W2, double* U,
                       hipStream_t st) {
   const int64_t nn = (int64_t)n * n;
@@ -1206,11 +1217,11 @@ static int trd_form_q(const double* V, int n, const double* Tbuf, double* Q, dou
     const double* Tb = Tbuf + (size_t)b * wb * wb;
     int rc;
     GemmArgs g1{nb, m, m, Vb, n, 1, 0, Q + off * n + off, n, 1, 0, W, m, 0, 0, 1.0};
-    if ((rc = gemm(g1, st, false))) return rc;
+    if ((rc = gemm(g1, st, false, 94))) return rc;
     GemmArgs g2{nb, m, nb, Tb, wb, 1, 0, W, m, 1, 0, W2, m, 0, 0, 1.0};
-    if ((rc = gemm(g2, st, false))) return rc;
+    if ((rc = gemm(g2, st, false, 94))) return rc;
     GemmArgs g3{m, m, nb, Vb, 1, n, 0, W2, m, 1, 0, U, m, 0, 0, 1.0};
-    if ((rc = gemm(g3, st, false))) return rc;
+    if ((rc = gemm(g3, st, false, 94))) return rc;
     hipLaunchKernelGGL(sub_trailing, dim3((unsigned)((m * m + 255) / 256)), dim3(256), 0, st, Q, (int64_t)n, off, m, U);
     JV_LAUNCH_CHECK("sub_trailing");
   }

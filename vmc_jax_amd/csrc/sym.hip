@@ -18,9 +18,12 @@
 namespace jv {
 
 namespace {
-// workspace slots of this file
-constexpr int WS_ROWS = 50, WS_LP = 51, WS_SP = 52, WS_ME = 53, WS_CNT = 54, WS_LPSP = 55, WS_GI = 56,
-              WS_W = 57, WS_MC_LP = 58, WS_MC_PROP = 59, WS_MC_U = 60, WS_MC_NACC = 61, WS_MC_LPP = 62;
+// workspace slots of this file.  jv_sym_eloc may run on a side stream while
+// jv_sym_gradients runs on the main stream, so the forward pass inside
+// jv_sym_eloc has rows / log-psi slots of its own (WS_EROWS, WS_ELP).
+constexpr int WS_ROWS = 70, WS_LP = 71, WS_EROWS = 72, WS_ELP = 73, WS_SP = 74, WS_ME = 75, WS_CNT = 76,
+              WS_LPSP = 77, WS_GI = 78, WS_W = 79, WS_GLO = 80, WS_MC_LP = 81, WS_MC_PROP = 82, WS_MC_U = 83,
+              WS_MC_NACC = 84, WS_MC_LPP = 85;
 constexpr size_t ROW_BYTES_MAX = size_t(256) << 20;  // transformed configurations per chunk
 constexpr size_t GRAD_BYTES_MAX = size_t(1) << 30;   // inner gradient rows per chunk
 }  // namespace
@@ -287,15 +290,10 @@ int launch_expand(const int32_t* cfg, int64_t ns, const jv_orbit* o, int L, int3
   return JV_OK;
 }
 
-}  // namespace
-}  // namespace jv
-
-using namespace jv;
-
-extern "C" {
-
-int jv_sym_forward(const jv_net* net, const double* params, const jv_orbit* orbit, const int32_t* configs,
-                   int64_t n, double* logpsi, void* stream) {
+// Symmetrised log-amplitudes with the expanded rows and per-element log-psi
+// in workspace slots ws_rows / ws_lp.
+int sym_forward(const jv_net* net, const double* params, const jv_orbit* orbit, const int32_t* configs, int64_t n,
+                double* logpsi, void* stream, int ws_rows, int ws_lp) {
   int rc = check_orbit(net, orbit, "jv_sym_forward");
   if (rc) return rc;
   if (!params || !configs || !logpsi || n < 0) return fail(JV_E_ARG, "jv_sym_forward: bad arguments");
@@ -303,8 +301,8 @@ int jv_sym_forward(const jv_net* net, const double* params, const jv_orbit* orbi
   hipStream_t st = (hipStream_t)stream;
   const int L = net->num_sites, G = orbit->num_elements;
   const int64_t cs = std::min<int64_t>(n, rows_per_chunk(net, orbit, 0));
-  int32_t* rows = (int32_t*)workspace(sizeof(int32_t) * (size_t)cs * G * L, WS_ROWS);
-  double* lp = (double*)workspace(sizeof(double) * 2 * (size_t)cs * G, WS_LP);
+  int32_t* rows = (int32_t*)workspace(sizeof(int32_t) * (size_t)cs * G * L, ws_rows);
+  double* lp = (double*)workspace(sizeof(double) * 2 * (size_t)cs * G, ws_lp);
   if (!rows || !lp) return fail(JV_E_NOMEM, "jv_sym_forward: workspace allocation failed");
   const int cpx = sym_cpx(net, orbit);
   for (int64_t s0 = 0; s0 < n; s0 += cs) {
@@ -314,6 +312,18 @@ int jv_sym_forward(const jv_net* net, const double* params, const jv_orbit* orbi
     if ((rc = launch_avg(lp, orbit, ns, cpx, logpsi + 2 * s0, nullptr, st))) return rc;
   }
   return JV_OK;
+}
+
+}  // namespace
+}  // namespace jv
+
+using namespace jv;
+
+extern "C" {
+
+int jv_sym_forward(const jv_net* net, const double* params, const jv_orbit* orbit, const int32_t* configs,
+                   int64_t n, double* logpsi, void* stream) {
+  return sym_forward(net, params, orbit, configs, n, logpsi, stream, WS_ROWS, WS_LP);
 }
 
 int jv_sym_gradients(const jv_net* net, const double* params, const jv_orbit* orbit, const int32_t* configs,
@@ -339,7 +349,7 @@ int jv_sym_gradients(const jv_net* net, const double* params, const jv_orbit* or
   double* lp = (double*)workspace(sizeof(double) * 2 * (size_t)cs * G, WS_LP);
   double* gi = (double*)workspace(in_row * (size_t)cs * G, WS_GI);
   double* W = (double*)workspace(sizeof(double) * 2 * (size_t)cs * G, WS_W);
-  double* lo = logpsi ? nullptr : (double*)workspace(sizeof(double) * 2 * (size_t)cs, WS_LPSP);
+  double* lo = logpsi ? nullptr : (double*)workspace(sizeof(double) * 2 * (size_t)cs, WS_GLO);
   if (!rows || !lp || !gi || !W || (!logpsi && !lo)) return fail(JV_E_NOMEM, "jv_sym_gradients: workspace allocation failed");
   for (int64_t s0 = 0; s0 < n; s0 += cs) {
     const int64_t ns = std::min(cs, n - s0);
@@ -382,7 +392,7 @@ int jv_sym_eloc(const jv_net* net, const double* params, const jv_orbit* orbit, 
   for (int64_t s0 = 0; s0 < n; s0 += cs) {
     const int64_t ns = std::min(cs, n - s0);
     if ((rc = jv_get_s_primes(op, prefactor, configs + s0 * L, ns, L, sp, me, cnt, stream))) return rc;
-    if ((rc = jv_sym_forward(net, params, orbit, sp, ns * nops, lpsp, stream))) return rc;
+    if ((rc = sym_forward(net, params, orbit, sp, ns * nops, lpsp, stream, WS_EROWS, WS_ELP))) return rc;
     hipLaunchKernelGGL(sym_oloc, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, st, me, lpsp,
                        logpsi_s + 2 * s0, ns, nops, out + 2 * s0);
     JV_LAUNCH_CHECK("sym_oloc");

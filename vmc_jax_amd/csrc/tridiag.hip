@@ -661,22 +661,38 @@ int trd_reduce(double* S, int n, double* ev, double* c, double* work, hipStream_
 // Eigenvalues (ev: the diagonal of T on entry) and eigenvectors of the
 // tridiagonal T left by trd_reduce in work (rocSOLVER divide and conquer):
 // column-major Z, so row j of the row-major buffer is eigenvector j.
-int trd_tridiag_eig(int n, double* ev, double* work, double* Zbuf, hipStream_t st) {
-  const int G = trd_groups(n);
+int dc_tridiag_eig(int n, double* d, const double* e, double* Z, hipStream_t st);
+
+bool dc_supported(int64_t n);
+
+// Tridiagonal eigensolver: the divide and conquer of dc.hip (stream ordered),
+// or rocSOLVER's dstedc (which destroys e and needs a host read of its
+// convergence flag) with VMC_JAX_AMD_STEDC=rocsolver.  The variable is read on
+// every call.
+bool stedc_own(int n) {
+  const char* s = getenv("VMC_JAX_AMD_STEDC");
+  return !(s && std::string(s) == "rocsolver") && dc_supported(n);
+}
+
+int stedc(int n, double* ev, double* e, double* Z, rocblas_int* info, hipStream_t st) {
+  if (stedc_own(n)) return dc_tridiag_eig(n, ev, e, Z, st);
   rocblas_handle h = blas_handle(st);
   if (!h) return fail(JV_E_SOLVER, "rocblas_create_handle failed");
-  rocblas_int* info = (rocblas_int*)(work + 7 * (size_t)n + 2 * G + 8);
   {
     ScopedTimer tm("stedc", st);
-    rocblas_status s = rocsolver_dstedc(h, rocblas_evect_tridiagonal, n, ev, work, Zbuf, n, info);
-    if (s != rocblas_status_success)
-      return fail(JV_E_SOLVER, "rocsolver_dstedc status " + std::to_string((int)s));
+    rocblas_status s = rocsolver_dstedc(h, rocblas_evect_tridiagonal, n, ev, e, Z, n, info);
+    if (s != rocblas_status_success) return fail(JV_E_SOLVER, "rocsolver_dstedc status " + std::to_string((int)s));
   }
   rocblas_int hinfo = 0;
   JV_HIP(hipMemcpyAsync(&hinfo, info, sizeof(hinfo), hipMemcpyDeviceToHost, st));
   JV_HIP(hipStreamSynchronize(st));
   if (hinfo != 0) return fail(JV_E_SOLVER, "tridiagonal eigensolver did not converge (info " + std::to_string(hinfo) + ")");
   return JV_OK;
+}
+
+int trd_tridiag_eig(int n, double* ev, double* work, double* Zbuf, hipStream_t st) {
+  const int G = trd_groups(n);
+  return stedc(n, ev, work, Zbuf, (rocblas_int*)(work + 7 * (size_t)n + 2 * G + 8), st);
 }
 
 int trd_eig_solve(double* S, int n, double* ev, double* c, double* Zbuf, double* work, hipStream_t st) {
@@ -741,18 +757,10 @@ int trdc_eigh_full(double* S, int n, double* ev, hipStream_t st) {
     for (int k = 0; k < n; ++k) hipLaunchKernelGGL(trd_step_c, dim3((unsigned)G), dim3(TRD_T), lds, st, a, k);
     JV_LAUNCH_CHECK("trd_step_c");
   }
+  int rc = stedc(n, ev, a.e, Z, (rocblas_int*)(work + 13 * (size_t)n + 2 * G + 8), st);
+  if (rc) return rc;
   rocblas_handle h = blas_handle(st);
   if (!h) return fail(JV_E_SOLVER, "rocblas_create_handle failed");
-  rocblas_int* info = (rocblas_int*)(work + 13 * (size_t)n + 2 * G + 8);
-  {
-    ScopedTimer tm("stedc", st);
-    rocblas_status s = rocsolver_dstedc(h, rocblas_evect_tridiagonal, n, ev, a.e, Z, n, info);
-    if (s != rocblas_status_success) return fail(JV_E_SOLVER, "rocsolver_dstedc status " + std::to_string((int)s));
-  }
-  rocblas_int hinfo = 0;
-  JV_HIP(hipMemcpyAsync(&hinfo, info, sizeof(hinfo), hipMemcpyDeviceToHost, st));
-  JV_HIP(hipStreamSynchronize(st));
-  if (hinfo != 0) return fail(JV_E_SOLVER, "tridiagonal eigensolver did not converge (info " + std::to_string(hinfo) + ")");
   const int64_t nn = (int64_t)n * n;
   hipLaunchKernelGGL(trdc_expand, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, Z, nn, Zc);
   JV_LAUNCH_CHECK("trdc_expand");
