@@ -151,7 +151,8 @@ def timer(jv, name):
     return tot.value, cnt.value
 
 
-KERNELS = ("gram", "gemm", "eigh", "tridiag", "stedc", "ormtr", "trd_apply", "eloc", "mc_sweep", "net_grad",
+KERNELS = ("gram", "gemm", "eigh", "tridiag", "stedc", "ormtr", "trd_apply", "cholesky", "chol_solve", "eloc",
+           "mc_sweep", "net_grad",
            "net_forward", "sym_mc", "sym_eloc", "sym_combine")
 
 
@@ -169,13 +170,17 @@ def run_gpu(workload, steps, warmup, mode, overlap=True):
     energies = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    routes = []
     for _ in range(steps):
         w.step()
         energies.append(w.eq.get_energy_mean())
+        info = getattr(w.eq, "last_info", None)
+        routes.append(int(info.iterations) if info is not None else 0)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     jv._lib.call("jv_timer_enable", 0)
     kt = {k: timer(jv, k) for k in KERNELS}
+    w.cholesky_steps = sum(routes) if isinstance(w.eq, jv.util.MinSR) else None
     return w, elapsed, kt, energies
 
 
@@ -442,6 +447,7 @@ def main():
         "config": {"workload": w.desc, "samples_per_step": w.samples},
         "samples_per_s": steps_s * w.samples,
         "kernel_ms_per_step": {k: v[0] / args.steps for k, v in kt.items() if v[1] > 0},
+        "minsr_cholesky_steps": w.cholesky_steps,
         "energy_last": energies[-1] if energies else None,
     }
     if args.workload == "c3":

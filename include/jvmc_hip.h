@@ -128,7 +128,10 @@ typedef struct jv_mc_args {
 typedef struct jv_solve_info {
   double residual;           /* tdvp_residual (tdvp.py:209) */
   double pinv_cutoff;        /* max(cutoff, pinvCutoff) (tdvp.py:213) */
-  int32_t iterations;        /* regulariser iterations taken (tdvp.py:198) */
+  int32_t iterations;        /* TDVP: regulariser iterations taken (tdvp.py:198);
+                                MinSR dual, real: 1 when no eigenvalue lies at or
+                                below the cutoff and the update came from a
+                                Cholesky solve, 0 from the eigendecomposition */
   int32_t rank;              /* MinSR: eigenvalues kept by the pseudo-inverse */
   double e_var;              /* TDVP: sum_n |e_n|^2 = Var E_loc (stats.py:248-252) */
   double tdvp_error;         /* TDVP: tdvp_error metadata (tdvp.py:102-104) */

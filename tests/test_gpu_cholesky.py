@@ -1,0 +1,97 @@
+"""The Cholesky route of the real dual MinSR solve (jv_minsr_solve_dual) on
+synthetic centred data, against the pseudo-inverse of numpy's eigh (the
+algorithm of minsr.py:68-78 in its dual form): the route is taken exactly when
+no eigenvalue of S = D^T D (+ shift) is at or below pinvTol * lambda_max, and
+then gives the same update; otherwise the eigendecomposition runs.  Sizes
+cover one partial panel, panel boundaries (63, 64, 65, 128, 129) and the 10x10
+CNN's parameter count."""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+jv = pytest.importorskip("vmc_jax_amd")
+
+
+def _solve(D, e, rcond, shift=0.0):
+    n, p = D.shape
+    Dd = torch.as_tensor(D, device="cuda").contiguous()
+    ed = torch.as_tensor(e.astype(np.complex128), device="cuda").contiguous()
+    upd = torch.empty(p, dtype=torch.float64, device="cuda")
+    info = jv._lib.JvSolveInfo()
+    jv._lib.call("jv_minsr_solve_dual", jv._lib.ptr(Dd), jv._lib.ptr(ed), n, p, 0, float(rcond), float(shift),
+                 jv._lib.ptr(upd), ctypes.byref(info), jv._lib.stream_ptr())
+    return upd.cpu().numpy(), info
+
+
+def _reference(D, e, rcond, shift=0.0):
+    S = D.T @ D + shift * np.eye(D.shape[1])
+    w, V = np.linalg.eigh(S)
+    keep = np.abs(w) > rcond * np.abs(w).max()
+    inv = np.where(keep, 1.0 / np.where(keep, w, 1.0), 0.0)
+    return -(V * inv) @ (V.T @ (D.T @ e.real)), int(keep.sum())
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+@pytest.mark.parametrize("p", [1, 2, 63, 64, 65, 128, 129, 300, 1240, 2000])
+def test_full_rank_takes_cholesky(p):
+    rng = np.random.default_rng(p)
+    n = p + 64 + p // 2
+    D = rng.normal(size=(n, p)) / np.sqrt(n)
+    e = rng.normal(size=n) + 1j * rng.normal(size=n)
+    print("cholesky route p", p, flush=True)
+    u, info = _solve(D, e, 1e-10)
+    ref, rank = _reference(D, e, 1e-10)
+    assert info.iterations == 1 and info.rank == p == rank
+    assert _rel(u, ref) < 1e-11
+
+
+@pytest.mark.parametrize("shift", [1e-3, 1.0])
+def test_shift_takes_cholesky(shift):
+    rng = np.random.default_rng(7)
+    n, p = 300, 500  # N < P: singular D^T D, regular after the shift
+    D = rng.normal(size=(n, p)) / np.sqrt(n)
+    e = rng.normal(size=n) + 0j
+    u, info = _solve(D, e, 1e-10, shift)
+    ref, rank = _reference(D, e, 1e-10, shift)
+    assert info.iterations == 1 and info.rank == p == rank
+    assert _rel(u, ref) < 1e-10
+
+
+@pytest.mark.parametrize("p,n", [(200, 150), (1240, 1000)])
+def test_rank_deficient_uses_eigendecomposition(p, n):
+    rng = np.random.default_rng(p + n)
+    D = rng.normal(size=(n, p)) / np.sqrt(n)
+    e = rng.normal(size=n) + 0j
+    u, info = _solve(D, e, 1e-10)
+    ref, rank = _reference(D, e, 1e-10)
+    assert info.iterations == 0 and info.rank == rank == n
+    assert _rel(u, ref) < 1e-8
+
+
+def test_cut_spectrum_uses_eigendecomposition():
+    """Singular values 1 .. 1e-7 (eigenvalues of S down to 1e-14): pinvTol 1e-10
+    cuts the smallest, so the eigendecomposition runs.  Singular values 1 ..
+    1e-5 (eigenvalues down to 1e-10) with pinvTol 1e-12: nothing is cut and the
+    route's margin (8 p eps ||S||_F, 5e-13 here) admits the Cholesky solve."""
+    rng = np.random.default_rng(11)
+    n, p = 400, 256
+    U, _ = np.linalg.qr(rng.normal(size=(n, p)))
+    V, _ = np.linalg.qr(rng.normal(size=(p, p)))
+    e = rng.normal(size=n) + 0j
+    D = (U * np.logspace(0, -7, p)) @ V.T
+    u, info = _solve(D, e, 1e-10)
+    ref, rank = _reference(D, e, 1e-10)
+    assert info.iterations == 0 and info.rank == rank < p
+    assert _rel(u, ref) < 1e-6
+    D = (U * np.logspace(0, -5, p)) @ V.T
+    u, info = _solve(D, e, 1e-12)
+    ref, rank = _reference(D, e, 1e-12)
+    assert info.iterations == 1 and rank == p
+    assert _rel(u, ref) < 1e-5  # condition number 1e10: both accurate to ~ cond * eps

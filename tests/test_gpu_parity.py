@@ -553,16 +553,18 @@ def test_minsr_dual_equals_reference_form():
     assert rel(d, c) < 1e-9
 
 
-@pytest.mark.parametrize("shift,shape,ch,N,P,tol", [(0.0, (6, 6), (4, 4, 4), 1024, 332, 1e-12),
-                                                   (1e-3, (6, 6), (4, 4, 4), 1024, 332, 1e-12),
+@pytest.mark.parametrize("shift,shape,ch,N,P,tol", [(0.0, (6, 6), (4, 4, 4), 1024, 332, 1e-10),
+                                                   (1e-3, (6, 6), (4, 4, 4), 1024, 332, 1e-10),
                                                    (0.0, (10, 10), (8, 8, 8), 4096, 1240, 1e-9)])
 def test_minsr_dual_deferred_equals_dual(shift, shape, ch, N, P, tol, stedc):
     """The deferred dual solve (E_loc produced on another stream, centred and
     projected on the library's auxiliary stream) gives the dual update.  At
     P = 1240 the auxiliary stream's GEMMs (explicit Q) run while the main
-    stream's tridiagonal eigensolver runs its own; pinvTol 1e-10 admits
-    condition numbers up to 1e10 there, and the explicit Q and the applied
-    reflectors round differently (5e-12 measured)."""
+    stream's tridiagonal eigensolver runs its own.  The two paths centre E_loc
+    in different places (library kernel / SampledObs) and, on the
+    eigendecomposition route, apply Q explicitly or by reflectors; the
+    rounding differences are amplified by the condition number (up to 1e10
+    with pinvTol 1e-10; 5e-12 measured at P = 1240)."""
     psi, opsi, H, Ho, s, lp, p = _cnn_case(shape, ch, N, 27)
     El = jv.stats.SampledObs(H.get_O_loc(s, psi, lp, 0.0), p)
     G = jv.stats.SampledObs(psi.gradients(s, layout=0), p)
@@ -615,6 +617,39 @@ def test_minsr_dual_tridiagonal(shape, ch, N, seed, stedc):
         a = to_np(jv.util.MinSR(None, pinvTol=tol, diagonalShift=shift, solver="T").solve(El, G, False))
         b = to_np(jv.util.MinSR(None, pinvTol=tol, diagonalShift=shift, solver="dual").solve(El, G, False))
         assert rel(b, a) < 1e-7, (shape, N, tol, shift, rel(b, a))
+
+
+def test_minsr_dual_cholesky_route(monkeypatch):
+    """Real dual MinSR: when S has no eigenvalue at or below the pseudo-inverse
+    cutoff the update comes from a Cholesky solve, and equals the
+    eigendecomposition route; a cutoff above the smallest eigenvalue, or a
+    singular S (N < P), must send the solve through the eigendecomposition."""
+    psi, opsi, H, Ho, s, lp, p = _cnn_case((10, 10), (8, 8, 8), 4096, 33)
+    El = jv.stats.SampledObs(H.get_O_loc(s, psi, lp, 0.0), p)
+    G = jv.stats.SampledObs(psi.gradients(s, layout=0), p)
+    D = to_np(G._data)
+    lam = np.linalg.eigvalsh(D.T @ D)
+    P = psi.numParameters
+    assert P == 1240 and lam[0] > 1e-10 * lam[-1]
+    eq = jv.util.MinSR(None, pinvTol=1e-10, solver="dual")
+    uc = to_np(eq.solve(El, G, False))
+    assert eq.last_info.iterations == 1 and eq.last_info.rank == P
+    monkeypatch.setenv("VMC_JAX_AMD_MINSR_CHOLESKY", "0")
+    ue = to_np(eq.solve(El, G, False))
+    assert eq.last_info.iterations == 0 and eq.last_info.rank == P
+    assert rel(uc, ue) < 1e-9
+    monkeypatch.delenv("VMC_JAX_AMD_MINSR_CHOLESKY")
+    # cutoff between the two smallest eigenvalues: one is cut, no Cholesky
+    tol = np.sqrt(lam[0] * lam[1]) / lam[-1]
+    eq2 = jv.util.MinSR(None, pinvTol=float(tol), solver="dual")
+    eq2.solve(El, G, False)
+    assert eq2.last_info.iterations == 0 and eq2.last_info.rank == P - 1
+    # singular S
+    psi, opsi, H, Ho, s, lp, p = _cnn_case((6, 6), (4, 4, 4), 128, 32)
+    El = jv.stats.SampledObs(H.get_O_loc(s, psi, lp, 0.0), p)
+    G = jv.stats.SampledObs(psi.gradients(s, layout=0), p)
+    eq.solve(El, G, False)
+    assert eq.last_info.iterations == 0 and eq.last_info.rank <= 128
 
 
 def test_minsr_dual_holomorphic():

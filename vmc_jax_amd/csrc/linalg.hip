@@ -547,6 +547,45 @@ __global__ void add_diag(double* __restrict__ T, int64_t n, double shift, int cp
   if (i < n) T[(i * n + i) * (cpx ? 2 : 1)] += shift;
 }
 
+// Partial sums of squares over a grid-stride slice; sum_finish adds them up.
+__global__ void __launch_bounds__(256) sum_sq_partial(const double* __restrict__ A, int64_t len,
+                                                      double* __restrict__ part) {
+  __shared__ double red[4];
+  double a = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += (int64_t)gridDim.x * blockDim.x)
+    a = fma(A[i], A[i], a);
+  a = block_sum(a, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = a;
+}
+
+__global__ void __launch_bounds__(256) sum_finish(const double* __restrict__ part, int m, double* __restrict__ out) {
+  __shared__ double red[4];
+  double a = 0.0;
+  for (int i = threadIdx.x; i < m; i += blockDim.x) a += part[i];
+  a = block_sum(a, red);
+  if (threadIdx.x == 0) *out = a;
+}
+
+// Sc = S - c I with c = ||S||_F (rcond + 8 p eps) (||S||_F bounds the largest
+// eigenvalue, 8 p eps ||S||_F the rounding of the Cholesky factorisation) and
+// S2 = S.
+__global__ void shifted_copy(const double* __restrict__ S, int64_t p, const double* __restrict__ fro2, double rcond,
+                             double* __restrict__ Sc, double* __restrict__ S2) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= p * p) return;
+  const double c = sqrt(*fro2) * (rcond + 8.0 * (double)p * 2.220446049250313e-16);
+  const double v = S[t];
+  Sc[t] = v - (t / p == t % p ? c : 0.0);
+  S2[t] = v;
+}
+
+__global__ void set_ints(int* __restrict__ v, int a, int b) {
+  if (threadIdx.x == 0) {
+    v[0] = a;
+    v[1] = b;
+  }
+}
+
 __global__ void set_identity(double* __restrict__ Q, int64_t n) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n * n) Q[t] = (t / n == t % n) ? 1.0 : 0.0;
@@ -1228,6 +1267,51 @@ static int trd_form_q(const double* V, int n, const double* Tbuf, double* Q, dou
   return JV_OK;
 }
 
+// Cholesky route of the real dual solve (VMC_JAX_AMD_MINSR_CHOLESKY=0 turns it
+// off).  With c = ||S||_F (rcond + 8 p eps) >= rcond lambda_max(S), a
+// successful Cholesky factorisation of S - c I shows (Sylvester's inertia) that
+// every eigenvalue of S exceeds the pseudo-inverse cutoff rcond lambda_max: the
+// pseudo-inverse is the inverse, and S x = b by Cholesky gives the same update
+// as the eigendecomposition.  S - c I and a copy of S are factored together
+// (S itself stays intact for the eigendecomposition); *ok reports whether the
+// first factorisation succeeded, *L is then the factor of S.  flags[2] and
+// flags[3] receive the two factorisations' info; reads flags[2] on the host.
+static bool minsr_cholesky_enabled() {
+  const char* s = getenv("VMC_JAX_AMD_MINSR_CHOLESKY");
+  return !(s && std::string(s) == "0");
+}
+
+int chol_factor2(double* A0, double* A1, int p, int* info, hipStream_t st);
+int chol_solve(const double* L, int p, const double* b, double* x, hipStream_t st);
+
+static int chol_route(const double* S, int64_t p, double rcond, int* flags, bool* ok, double** L, hipStream_t st) {
+  *ok = false;
+  const size_t pp = (size_t)p * p;
+  double* Sc = (double*)workspace(sizeof(double) * pp, 95);
+  double* S2 = (double*)workspace(sizeof(double) * pp, 97);
+  double* part = (double*)workspace(sizeof(double) * 257, 96);
+  if (!Sc || !S2 || !part) return fail(JV_E_NOMEM, "jv_minsr_solve_dual: workspace allocation failed");
+  {
+    ScopedTimer tm("cholesky", st);
+    hipLaunchKernelGGL(sum_sq_partial, dim3(256), dim3(256), 0, st, S, (int64_t)pp, part);
+    JV_LAUNCH_CHECK("sum_sq_partial");
+    hipLaunchKernelGGL(sum_finish, dim3(1), dim3(256), 0, st, part, 256, part + 256);
+    JV_LAUNCH_CHECK("sum_finish");
+    hipLaunchKernelGGL(shifted_copy, dim3((unsigned)((pp + 255) / 256)), dim3(256), 0, st, S, p, part + 256, rcond, Sc,
+                       S2);
+    JV_LAUNCH_CHECK("shifted_copy");
+    int rc = chol_factor2(Sc, S2, (int)p, flags + 2, st);
+    if (rc) return rc;
+  }
+  int hinfo = -1;
+  JV_HIP(hipMemcpyAsync(&hinfo, flags + 2, sizeof(int), hipMemcpyDeviceToHost, st));
+  JV_HIP(hipStreamSynchronize(st));
+  if (hinfo != 0) return JV_OK;  // an eigenvalue at or below the cutoff is possible
+  *L = S2;
+  *ok = true;
+  return JV_OK;
+}
+
 // Dual MinSR solve.  Either e (centred) is given, or e_raw with the weights w
 // and an event after which e_raw is complete (NULL: ready in stream order); e_raw
 // is then centred here (SampledObs, stats.py:197-205).  On the real tridiagonal
@@ -1275,6 +1359,37 @@ static int minsr_dual(const double* data, const double* e, const double* e_raw, 
     }
     return gemv(true, data, rhs, n, p, cpx, 1, 1.0, b, s, deferred ? 49 : 12);  // b = D^dagger e
   };
+  hipLaunchKernelGGL(set_ints, dim3(1), dim3(64), 0, st, rank, 0, 0);  // rank[1]: route taken
+  JV_LAUNCH_CHECK("set_ints");
+  if (!cpx && minsr_cholesky_enabled()) {
+    hipStream_t s2 = nullptr;
+    hipEvent_t ev_d = nullptr, ev_b = nullptr;
+    if (deferred) {
+      // b = D^T e on the auxiliary stream once E_loc is ready, beside the factorisations
+      if (!aux_stream(&s2, &ev_d, &ev_b)) return fail(JV_E_HIP, "jv_minsr_solve_dual: auxiliary stream creation failed");
+      JV_HIP(hipEventRecord(ev_d, st));
+      JV_HIP(hipStreamWaitEvent(s2, ev_d, 0));
+      if ((rc = rhs_b(s2))) return rc;
+      JV_HIP(hipEventRecord(ev_b, s2));
+    }
+    bool ok = false;
+    double* L = nullptr;
+    if ((rc = chol_route(S, p, rcond, rank, &ok, &L, st))) return rc;
+    if (ok) {
+      if (deferred) JV_HIP(hipStreamWaitEvent(st, ev_b, 0));
+      else if ((rc = rhs_b(st))) return rc;
+      {
+        ScopedTimer tm("chol_solve", st);
+        if ((rc = chol_solve(L, (int)p, b, x, st))) return rc;
+      }
+      hipLaunchKernelGGL(set_ints, dim3(1), dim3(64), 0, st, rank, (int)p, 1);
+      JV_LAUNCH_CHECK("set_ints");
+      hipLaunchKernelGGL(neg_copy, dim3((unsigned)((p + 255) / 256)), dim3(256), 0, st, x, p, update);
+      JV_LAUNCH_CHECK("neg_copy");
+      return JV_OK;
+    }
+    // otherwise the eigendecomposition below (the deferred path forms b again)
+  }
   if (deferred && !cpx && trd_supported(p)) {
     hipStream_t s2;
     hipEvent_t ev_red, ev_rhs;
@@ -1352,11 +1467,13 @@ static int minsr_dual_finish_info(int rc, jv_solve_info* info, int64_t p, double
   if (rc || !info) return rc;
   int* rank = (int*)workspace(64, 11);
   if (!rank) return fail(JV_E_NOMEM, "jv_minsr_solve_dual: workspace allocation failed");
-  int hr = 0;
-  JV_HIP(hipMemcpyAsync(&hr, rank, sizeof(int), hipMemcpyDeviceToHost, st));
+  int hr[4] = {0, 0, 0, 0};
+  JV_HIP(hipMemcpyAsync(hr, rank, sizeof(hr), hipMemcpyDeviceToHost, st));
   JV_HIP(hipStreamSynchronize(st));
-  info->rank = hr;
-  info->iterations = 0;
+  if (hr[1] == 1 && hr[3] != 0)
+    return fail(JV_E_SOLVER, "jv_minsr_solve_dual: Cholesky factorisation failed (info " + std::to_string(hr[3]) + ")");
+  info->rank = hr[0];
+  info->iterations = hr[1];
   info->residual = 0.0;
   info->pinv_cutoff = rcond;
   (void)p;
