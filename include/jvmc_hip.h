@@ -326,10 +326,16 @@ int jv_minsr_solve(const double* data, const double* e, int64_t n, int64_t p,
  * D^dagger D coincide, D^dagger annihilates the null space of T, so
  * -D^dagger pinv(T) e = -pinv(S) D^dagger e with the same relative cutoff.
  * Cheaper whenever N > P.  No separate reference function (exact rewrite of
- * minsr.py:53-80). */
+ * minsr.py:53-80).  Real data with try_cholesky != 0: when a Cholesky
+ * factorisation of S - c I, c = ||S||_F (rcond + 8 p eps) >= rcond
+ * lambda_max, succeeds, no eigenvalue lies at or below the cutoff, pinv(S) =
+ * S^-1 and the update comes from a Cholesky solve (info->iterations = 1);
+ * otherwise, or with try_cholesky = 0, from the eigendecomposition.  Both give
+ * the same update; try_cholesky only saves the factorisation when the caller
+ * expects a cut spectrum. */
 int jv_minsr_solve_dual(const double* data, const double* e, int64_t n, int64_t p,
-                        int32_t is_complex, double rcond, double diag_shift, double* update,
-                        jv_solve_info* info, void* stream);
+                        int32_t is_complex, double rcond, double diag_shift, int32_t try_cholesky,
+                        double* update, jv_solve_info* info, void* stream);
 
 /* jv_minsr_solve_dual with the local estimator still in flight: e_raw is the
  * uncentred E_loc (complex [n]) with the sample weights w [n], complete once the
@@ -342,7 +348,7 @@ int jv_minsr_solve_dual(const double* data, const double* e, int64_t n, int64_t 
  * E_loc before the solve; the order does not change the result). */
 int jv_minsr_solve_dual_deferred(const double* data, const double* e_raw, const double* w, void* e_ready,
                                  int64_t n, int64_t p, int32_t is_complex, double rcond, double diag_shift,
-                                 double* update, jv_solve_info* info, void* stream);
+                                 int32_t try_cholesky, double* update, jv_solve_info* info, void* stream);
 
 /* TDVP solve (tdvp.py:134-213).  data: centred O in jVMC's flat layout, complex
  * [n][pr]; e: centred E_loc, complex [n]; w: weights [n]; x = rhsPrefactor

@@ -22,7 +22,7 @@ def _solve(D, e, rcond, shift=0.0):
     ed = torch.as_tensor(e.astype(np.complex128), device="cuda").contiguous()
     upd = torch.empty(p, dtype=torch.float64, device="cuda")
     info = jv._lib.JvSolveInfo()
-    jv._lib.call("jv_minsr_solve_dual", jv._lib.ptr(Dd), jv._lib.ptr(ed), n, p, 0, float(rcond), float(shift),
+    jv._lib.call("jv_minsr_solve_dual", jv._lib.ptr(Dd), jv._lib.ptr(ed), n, p, 0, float(rcond), float(shift), 1,
                  jv._lib.ptr(upd), ctypes.byref(info), jv._lib.stream_ptr())
     return upd.cpu().numpy(), info
 

@@ -95,8 +95,9 @@ _SIGS = {
     "jv_eigh": [_i64, _i32, _vp, _vp, _vp],
     "jv_stedc": [_i64, _vp, _vp, _vp, _vp],
     "jv_minsr_solve": [_vp, _vp, _i64, _i64, _i32, _dbl, _dbl, _vp, ctypes.POINTER(JvSolveInfo), _vp],
-    "jv_minsr_solve_dual": [_vp, _vp, _i64, _i64, _i32, _dbl, _dbl, _vp, ctypes.POINTER(JvSolveInfo), _vp],
-    "jv_minsr_solve_dual_deferred": [_vp, _vp, _vp, _vp, _i64, _i64, _i32, _dbl, _dbl, _vp, ctypes.POINTER(JvSolveInfo),
+    "jv_minsr_solve_dual": [_vp, _vp, _i64, _i64, _i32, _dbl, _dbl, _i32, _vp, ctypes.POINTER(JvSolveInfo), _vp],
+    "jv_minsr_solve_dual_deferred": [_vp, _vp, _vp, _vp, _i64, _i64, _i32, _dbl, _dbl, _i32, _vp,
+                                     ctypes.POINTER(JvSolveInfo),
                                      _vp],
     "jv_tdvp_solve": [_vp, _vp, _vp, _i64, _i64, ctypes.POINTER(_dbl), _i32, _dbl, _i32, _dbl, _dbl, _dbl,
                       _dbl, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(JvSolveInfo), _vp],

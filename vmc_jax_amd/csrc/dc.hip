@@ -38,6 +38,8 @@ namespace {
 constexpr int DC_LEAF = 32;
 // the merge preparation keeps 20 bytes per row of the merged node in LDS
 constexpr int DC_MAX_N = 65536 / 20;
+// flags kept in the merged column table next to the column (< 2^29)
+constexpr int DC_NEG = 1 << 30, DC_ROT = 1 << 29, DC_COL = DC_ROT - 1;
 
 // Node i at depth k of the tree covers rows [i n / 2^k, (i + 1) n / 2^k)
 // (floors) and splits at (2 i + 1) n / 2^(k + 1): children are nodes 2i and
@@ -258,46 +260,84 @@ __global__ void __launch_bounds__(64) dc_prep(int lvk, const double* __restrict_
   dmax = wave_max(dmax);
   zmax = wave_max(zmax);
   __syncthreads();
-  if (lane != 0) return;
   const double rho = 2.0 * fabs(beta);
   const double tol = 8.0 * 2.220446049250313e-16 * fmax(dmax, zmax);
+  // Flags in the column table, computed by all lanes over contiguous chunks:
+  // DC_NEG for a negligible coupling, DC_ROT where the pair (previous
+  // non-negligible entry, t) passes the rotation test with the values before
+  // any rotation.  The sequential pass below reuses DC_ROT unless the
+  // previous entry was itself just rotated.
+  {
+    const int C = (nn + WAVE - 1) / WAVE;
+    const int c0 = min(nn, lane * C), c1 = min(nn, c0 + C);
+    int last = -1;
+    for (int t = c0; t < c1; ++t) {
+      if (rho * fabs(lzs[t]) <= tol) lct[t] |= DC_NEG;
+      else last = t;
+    }
+    int incl = last;
+    for (int o = 1; o < WAVE; o <<= 1) {
+      const int v = __shfl_up(incl, o, 64);
+      if (lane >= o) incl = max(incl, v);
+    }
+    int q = __shfl_up(incl, 1, 64);
+    if (lane == 0) q = -1;
+    for (int t = c0; t < c1; ++t) {
+      if (lct[t] & DC_NEG) continue;
+      if (q >= 0) {
+        const double zp = lzs[q], zi = lzs[t];
+        const double tz = hypot(zp, zi);
+        const double c = zi / tz, s = -zp / tz;
+        if (fabs((ldt[t] - ldt[q]) * c * s) <= tol) lct[t] |= DC_ROT;
+      }
+      q = t;
+    }
+  }
+  __syncthreads();
+  if (lane != 0) return;
   // Deflation in ascending order.  The kept poles stay ascending: a rotation
   // replaces the last kept pole d_p by d_t' in [d_p, d_t], and kept neighbours
   // differ by more than 2 tol, far above the rounding of d_t'.
   int k = 0, ndf = 0, nrot = 0, prev = -1;
+  bool prev_rotated = false;
   for (int t = 0; t < nn; ++t) {
-    const double zi = lzs[t];
-    if (rho * fabs(zi) <= tol) {  // negligible coupling
+    const int f = lct[t];
+    const int col = f & DC_COL;
+    if (f & DC_NEG) {  // negligible coupling
       b.dD[s0 + ndf] = ldt[t];
-      b.colD[s0 + ndf] = lct[t];
+      b.colD[s0 + ndf] = col;
       ++ndf;
       continue;
     }
-    if (prev >= 0) {
-      const double zp = lzs[prev];
+    bool rotated = false;
+    if (prev >= 0 && (prev_rotated || (f & DC_ROT))) {
+      const double zp = lzs[prev], zi = lzs[t];
       const double tz = hypot(zp, zi);
       const double c = zi / tz, s = -zp / tz;
       const double dp = ldt[prev], di = ldt[t];
       if (fabs((di - dp) * c * s) <= tol) {
         // rotate the pair so that z[prev] vanishes: prev deflates, t takes its place
-        b.rot[2 * (s0 + nrot)] = lct[prev];
-        b.rot[2 * (s0 + nrot) + 1] = lct[t];
+        const int pcol = lct[prev] & DC_COL;
+        b.rot[2 * (s0 + nrot)] = pcol;
+        b.rot[2 * (s0 + nrot) + 1] = col;
         b.rc[2 * (s0 + nrot)] = c;
         b.rc[2 * (s0 + nrot) + 1] = s;
         ++nrot;
         lzs[t] = tz;
         ldt[t] = dp * s * s + di * c * c;
         b.dD[s0 + ndf] = dp * c * c + di * s * s;
-        b.colD[s0 + ndf] = lct[prev];
+        b.colD[s0 + ndf] = pcol;
         ++ndf;
         --k;
+        rotated = true;
       }
     }
     b.dd[s0 + k] = ldt[t];
     b.zz[s0 + k] = lzs[t];
-    b.colN[s0 + k] = lct[t];
+    b.colN[s0 + k] = col;
     ++k;
     prev = t;
+    prev_rotated = rotated;
   }
   b.cnt[2 * s0] = k;
   b.cnt[2 * s0 + 1] = nrot;
@@ -327,19 +367,27 @@ __global__ void dc_rotate(int lvk, int n, const DcBufs b, double* __restrict__ Q
 // One wave per root j of node y: bisection on tau in the coordinate of the
 // nearer pole, f(tau) = 1 + rho sum z_i^2 / ((d_i - d_org) - tau).
 __global__ void __launch_bounds__(256) dc_secular(int lvk, int n, DcBufs b) {
+  extern __shared__ double sec[];  // poles [k], squared couplings [k]
   const DcNode nd = dc_node(n, lvk, blockIdx.y);
   const int s0 = nd.start;
+  const int k = b.cnt[2 * s0];
+  if ((int)(blockIdx.x * (blockDim.x / WAVE)) >= k) return;  // uniform over the workgroup
+  double* dd = sec;
+  double* z2 = sec + k;
+  for (int t = threadIdx.x; t < k; t += blockDim.x) {
+    dd[t] = b.dd[s0 + t];
+    const double z = b.zz[s0 + t];
+    z2[t] = z * z;
+  }
+  __syncthreads();
   const int j = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
   const int lane = threadIdx.x % WAVE;
-  const int k = b.cnt[2 * s0];
   if (j >= k) return;
   const double rho = b.rho[s0];
-  const double* dd = b.dd + s0;
-  const double* zz = b.zz + s0;
   auto fsum = [&](int org, double t) {
     double a = 0.0;
     const double dorg = dd[org];
-    for (int i = lane; i < k; i += WAVE) a += zz[i] * zz[i] / ((dd[i] - dorg) - t);
+    for (int i = lane; i < k; i += WAVE) a += z2[i] / ((dd[i] - dorg) - t);
     return 1.0 + rho * wave_sum(a);
   };
   const double lo = dd[j];
@@ -348,7 +396,7 @@ __global__ void __launch_bounds__(256) dc_secular(int lvk, int n, DcBufs b) {
     hi = dd[j + 1];
   } else {
     double a = 0.0;
-    for (int i = lane; i < k; i += WAVE) a += zz[i] * zz[i];
+    for (int i = lane; i < k; i += WAVE) a += z2[i];
     hi = lo + rho * wave_sum(a);
   }
   const double mid = 0.5 * (lo + hi);
@@ -377,21 +425,34 @@ __global__ void __launch_bounds__(256) dc_secular(int lvk, int n, DcBufs b) {
   }
 }
 
-// Gu-Eisenstat weights; thread = pole i.
-__global__ void dc_zhat(int lvk, int n, DcBufs b) {
+// Gu-Eisenstat weights; thread = pole i.  The roots' data (pole, origin
+// pole, tau) pass through LDS in chunks of 1024, in root order.
+__global__ void __launch_bounds__(256) dc_zhat(int lvk, int n, DcBufs b) {
+  __shared__ double sd[1024], so[1024], st[1024];
   const DcNode nd = dc_node(n, lvk, blockIdx.y);
   const int s0 = nd.start;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int k = b.cnt[2 * s0];
-  if (i >= k) return;
+  if ((int)(blockIdx.x * blockDim.x) >= k) return;  // uniform over the workgroup
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = i < k;
   const double rho = b.rho[s0];
   const double* dd = b.dd + s0;
-  const double di = dd[i];
-  auto delta = [&](int jj) { return (di - dd[b.org[s0 + jj]]) - b.tau[s0 + jj]; };  // d_i - lambda_j
-  double v = -delta(i) / rho;
-  for (int jj = 0; jj < k; ++jj)
-    if (jj != i) v *= -delta(jj) / (dd[jj] - di);
-  b.zh[s0 + i] = copysign(sqrt(fabs(v)), b.zz[s0 + i]);
+  const double di = live ? dd[i] : 0.0;
+  double v = live ? -((di - dd[b.org[s0 + i]]) - b.tau[s0 + i]) / rho : 0.0;  // (lambda_i - d_i) / rho
+  for (int c0 = 0; c0 < k; c0 += 1024) {
+    const int m = min(1024, k - c0);
+    for (int t = threadIdx.x; t < m; t += blockDim.x) {
+      sd[t] = dd[c0 + t];
+      so[t] = dd[b.org[s0 + c0 + t]];
+      st[t] = b.tau[s0 + c0 + t];
+    }
+    __syncthreads();
+    if (live)
+      for (int u = 0; u < m; ++u)
+        if (c0 + u != i) v *= -((di - so[u]) - st[u]) / (sd[u] - di);
+    __syncthreads();
+  }
+  if (live) b.zh[s0 + i] = copysign(sqrt(fabs(v)), b.zz[s0 + i]);
 }
 
 // Eigenvector j of the rank-one problem into column j of the node's U block
@@ -427,24 +488,45 @@ __global__ void __launch_bounds__(256) dc_vectors(int lvk, int n, const DcBufs b
   for (int i = lane; i < nn; i += WAVE) gc[i] = qs[i];
 }
 
-// Final order of a node: rank of each root (entries t < k) and deflated value
-// (entries t >= k) among all nn, ties by entry index.  Dynamic LDS: nn doubles.
+// Final order of a node: the roots (entries t < k) and deflated values
+// (entries t >= k) sorted by (NaN-safe key, entry index) with a bitonic
+// network in LDS, padded to a power of two with +inf keys and indices >= nn.
+// Dynamic LDS: 12 bytes per padded entry.
 __global__ void __launch_bounds__(256) dc_rank(int lvk, int n, DcBufs b, double* __restrict__ w) {
   extern __shared__ double lv[];
   const DcNode nd = dc_node(n, lvk, blockIdx.x);
   const int s0 = nd.start, nn = nd.n1 + nd.n2;
   const int k = b.cnt[2 * s0];
-  for (int t = threadIdx.x; t < nn; t += blockDim.x) lv[t] = t < k ? b.lam[s0 + t] : b.dD[s0 + (t - k)];
+  int P2 = 1;
+  while (P2 < nn) P2 <<= 1;
+  double* key = lv;
+  int* idx = (int*)(lv + P2);
+  for (int t = threadIdx.x; t < P2; t += blockDim.x) {
+    key[t] = t < nn ? dc_key(t < k ? b.lam[s0 + t] : b.dD[s0 + (t - k)]) : __builtin_inf();
+    idx[t] = t;
+  }
   __syncthreads();
-  for (int t = threadIdx.x; t < nn; t += blockDim.x) {
-    const double x = lv[t], kx = dc_key(x);
-    int r = 0;
-    for (int u = 0; u < nn; ++u) {
-      const double ky = dc_key(lv[u]);
-      r += (ky < kx || (ky == kx && u < t)) ? 1 : 0;
+  for (int size = 2; size <= P2; size <<= 1)
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int t = threadIdx.x; t < (P2 >> 1); t += blockDim.x) {
+        const int i = 2 * stride * (t / stride) + (t % stride), j = i + stride;
+        const bool asc = (i & size) == 0;
+        const double ki = key[i], kj = key[j];
+        const int ii = idx[i], ij = idx[j];
+        const bool gt = ki > kj || (ki == kj && ii > ij);
+        if (gt == asc) {
+          key[i] = kj;
+          key[j] = ki;
+          idx[i] = ij;
+          idx[j] = ii;
+        }
+      }
+      __syncthreads();
     }
+  for (int r = threadIdx.x; r < nn; r += blockDim.x) {
+    const int t = idx[r];
     b.rank[s0 + t] = r;
-    w[s0 + r] = x;
+    w[s0 + r] = t < k ? b.lam[s0 + t] : b.dD[s0 + (t - k)];
   }
 }
 
@@ -520,7 +602,7 @@ int dc_tridiag_eig(int n, double* d, const double* e, double* Z, hipStream_t st)
     JV_LAUNCH_CHECK("dc_prep");
     hipLaunchKernelGGL(dc_rotate, dim3((unsigned)((mx + 255) / 256), nodes), dim3(256), 0, st, k, n, b, Z);
     JV_LAUNCH_CHECK("dc_rotate");
-    hipLaunchKernelGGL(dc_secular, dim3((unsigned)((mx + 3) / 4), nodes), dim3(256), 0, st, k, n, b);
+    hipLaunchKernelGGL(dc_secular, dim3((unsigned)((mx + 3) / 4), nodes), dim3(256), (size_t)mx * 16, st, k, n, b);
     JV_LAUNCH_CHECK("dc_secular");
     hipLaunchKernelGGL(dc_zhat, dim3((unsigned)((mx + 255) / 256), nodes), dim3(256), 0, st, k, n, b);
     JV_LAUNCH_CHECK("dc_zhat");
@@ -535,7 +617,9 @@ int dc_tridiag_eig(int n, double* d, const double* e, double* Z, hipStream_t st)
       const int rc = gemm_real(m, m, m, U + o, n, 1, G + o, n, 1, C + o, n, st);
       if (rc) return rc;
     }
-    hipLaunchKernelGGL(dc_rank, dim3(nodes), dim3(256), (size_t)mx * 8, st, k, n, b, d);
+    int p2 = 1;
+    while (p2 < mx) p2 <<= 1;
+    hipLaunchKernelGGL(dc_rank, dim3(nodes), dim3(256), (size_t)p2 * 12, st, k, n, b, d);
     JV_LAUNCH_CHECK("dc_rank");
     hipLaunchKernelGGL(dc_scatter, dim3((unsigned)((mx + 255) / 256), (unsigned)mx, nodes), dim3(256), 0, st, k, n,
                        b, C, G, Z);

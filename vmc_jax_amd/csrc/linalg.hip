@@ -1209,28 +1209,31 @@ __global__ void neg_copy(const double* __restrict__ in, int64_t len, double* __r
 }
 static int minsr_dual(const double* data, const double* e, const double* e_raw, const double* w,
                       hipEvent_t e_ready, int64_t n, int64_t p, int32_t is_complex, double rcond, double diag_shift,
-                      double* update, hipStream_t st);
+                      bool try_chol, double* update, hipStream_t st);
 static int minsr_dual_finish_info(int rc, jv_solve_info* info, int64_t p, double rcond, hipStream_t st);
 }  // namespace jv
 
 extern "C" {
 
 int jv_minsr_solve_dual(const double* data, const double* e, int64_t n, int64_t p, int32_t is_complex,
-                        double rcond, double diag_shift, double* update, jv_solve_info* info, void* stream) {
+                        double rcond, double diag_shift, int32_t try_cholesky, double* update, jv_solve_info* info,
+                        void* stream) {
   if (!data || !e || !update || n < 1 || p < 1) return fail(JV_E_ARG, "jv_minsr_solve_dual: bad arguments");
   hipStream_t st = (hipStream_t)stream;
   return minsr_dual_finish_info(minsr_dual(data, e, nullptr, nullptr, nullptr, n, p, is_complex, rcond, diag_shift,
-                                           update, st), info, p, rcond, st);
+                                           try_cholesky != 0, update, st),
+                                info, p, rcond, st);
 }
 
 int jv_minsr_solve_dual_deferred(const double* data, const double* e_raw, const double* w, void* e_ready, int64_t n,
-                                 int64_t p, int32_t is_complex, double rcond, double diag_shift, double* update,
-                                 jv_solve_info* info, void* stream) {
+                                 int64_t p, int32_t is_complex, double rcond, double diag_shift, int32_t try_cholesky,
+                                 double* update, jv_solve_info* info, void* stream) {
   if (!data || !e_raw || !w || !update || n < 1 || p < 1)
     return fail(JV_E_ARG, "jv_minsr_solve_dual_deferred: bad arguments");
   hipStream_t st = (hipStream_t)stream;
   return minsr_dual_finish_info(minsr_dual(data, nullptr, e_raw, w, (hipEvent_t)e_ready, n, p, is_complex, rcond,
-                                           diag_shift, update, st), info, p, rcond, st);
+                                           diag_shift, try_cholesky != 0, update, st),
+                                info, p, rcond, st);
 }
 
 }  // extern "C"
@@ -1320,7 +1323,7 @@ static int chol_route(const double* S, int64_t p, double rcond, int* flags, bool
 // auxiliary stream during dstedc.
 static int minsr_dual(const double* data, const double* e, const double* e_raw, const double* w,
                       hipEvent_t e_ready, int64_t n, int64_t p, int32_t is_complex, double rcond, double diag_shift,
-                      double* update, hipStream_t st) {
+                      bool try_chol, double* update, hipStream_t st) {
   void* stream = (void*)st;
   const int cpx = is_complex;
   double* S = (double*)workspace(sizeof(double) * (size_t)p * p * (cpx ? 2 : 1), 6);
@@ -1361,7 +1364,7 @@ static int minsr_dual(const double* data, const double* e, const double* e_raw, 
   };
   hipLaunchKernelGGL(set_ints, dim3(1), dim3(64), 0, st, rank, 0, 0);  // rank[1]: route taken
   JV_LAUNCH_CHECK("set_ints");
-  if (!cpx && minsr_cholesky_enabled()) {
+  if (!cpx && try_chol && minsr_cholesky_enabled()) {
     hipStream_t s2 = nullptr;
     hipEvent_t ev_d = nullptr, ev_b = nullptr;
     if (deferred) {

@@ -29,6 +29,24 @@ class MinSR:
         self.diagonalizeOnDevice = diagonalizeOnDevice
         self.metaData = None
         self.last_info = None
+        self._chol_skip = 0
+
+    # Real dual solves try the Cholesky route first (exact when no eigenvalue
+    # falls below the cutoff).  After a solve whose spectrum was cut, the
+    # attempt is skipped for the next CHOLESKY_RETRY - 1 solves: the spectrum
+    # changes slowly from step to step, and a failed attempt costs a
+    # factorisation.  The update is the same either way.
+    CHOLESKY_RETRY = 8
+
+    def _cholesky_hint(self):
+        if self._chol_skip > 0:
+            self._chol_skip -= 1
+            return 0
+        return 1
+
+    def _note_route(self, tried):
+        if tried and self.last_info.iterations == 0:
+            self._chol_skip = self.CHOLESKY_RETRY - 1
 
     def set_pinv_tol(self, tol):
         self.pinvTol = tol
@@ -56,9 +74,16 @@ class MinSR:
         upd = torch.empty(p, dtype=torch.complex128 if cpx else torch.float64, device=D.device)
         info = _lib.JvSolveInfo()
         dual = self.solver == "dual" or (self.solver == "auto" and n > p)
-        _lib.call("jv_minsr_solve_dual" if dual else "jv_minsr_solve", _lib.ptr(D), _lib.ptr(e), n, p, int(cpx), float(self.pinvTol),
-                  float(self.diagonalShift), _lib.ptr(upd), ctypes.byref(info), _lib.stream_ptr())
+        if dual:
+            hint = self._cholesky_hint() if not cpx else 0
+            _lib.call("jv_minsr_solve_dual", _lib.ptr(D), _lib.ptr(e), n, p, int(cpx), float(self.pinvTol),
+                      float(self.diagonalShift), hint, _lib.ptr(upd), ctypes.byref(info), _lib.stream_ptr())
+        else:
+            hint = 0
+            _lib.call("jv_minsr_solve", _lib.ptr(D), _lib.ptr(e), n, p, int(cpx), float(self.pinvTol),
+                      float(self.diagonalShift), _lib.ptr(upd), ctypes.byref(info), _lib.stream_ptr())
         self.last_info = info
+        self._note_route(hint)
         if cpx:
             # T_ref = 2 G G^dagger: pinv scales by 1/2; [v, -i v] layout per leaf
             return psi.expand_compact(0.5 * upd)
@@ -80,10 +105,12 @@ class MinSR:
         n, P = D.shape
         upd = torch.empty(P, dtype=torch.float64, device=D.device)
         info = _lib.JvSolveInfo()
+        hint = self._cholesky_hint()
         _lib.call("jv_minsr_solve_dual_deferred", _lib.ptr(D), _lib.ptr(Eraw), _lib.ptr(p), ready._as_parameter_,
-                  n, P, 0, float(self.pinvTol), float(self.diagonalShift), _lib.ptr(upd), ctypes.byref(info),
+                  n, P, 0, float(self.pinvTol), float(self.diagonalShift), hint, _lib.ptr(upd), ctypes.byref(info),
                   _lib.stream_ptr())
         self.last_info = info
+        self._note_route(hint)
         return upd
 
     def __call__(self, netParameters, t, *, psi, hamiltonian, **rhsArgs):
