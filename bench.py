@@ -151,7 +151,7 @@ def timer(jv, name):
     return tot.value, cnt.value
 
 
-KERNELS = ("gram", "gemm", "eigh", "tridiag", "stedc", "ormtr", "trd_apply", "cholesky", "chol_solve", "eloc",
+KERNELS = ("gram", "gemm", "eigh", "tridiag", "stedc", "ormtr", "trd_apply", "cholesky", "chol_inverse", "eloc",
            "mc_sweep", "net_grad",
            "net_forward", "sym_mc", "sym_eloc", "sym_combine")
 

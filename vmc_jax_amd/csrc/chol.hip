@@ -1,5 +1,5 @@
 // Cholesky factorisation S = L L^T of real symmetric positive definite
-// matrices (row-major, lower triangle) and the solve L L^T x = b.
+// matrices (row-major, lower triangle) and the inverse of the factor.
 //
 // Right-looking, 64-column panels.  Per panel one launch factors the 64 x 64
 // diagonal block in LDS (every workgroup redundantly, so no workgroup waits
@@ -27,9 +27,10 @@ constexpr int CB = 64;  // panel width
 typedef double v4d __attribute__((ext_vector_type(4)));
 
 // panel k0: diagonal block (blockIdx.x == 0) and row blocks below it.  256
-// threads.  The diagonal block is factored right-looking in LDS (per column:
-// pivot and scaling, then the rank-one update of the trailing triangle spread
-// over all threads).  A row block is solved with four threads per row, thread
+// threads.  The diagonal block is factored right-looking by one wave, row t
+// in the registers of lane t (per column: pivot and scaling, then the
+// rank-one update of the trailing triangle).  A row block is solved with four
+// threads per row, thread
 // q of a row holding columns q, q + 4, ... in registers: per column the owner
 // finishes x_j, the row's four threads exchange it by shuffle and update
 // their later columns.
@@ -43,21 +44,50 @@ __global__ void __launch_bounds__(256) chol_panel(double* A0, double* A1, int p,
     if (r < b && c <= r) Ld[r][c] = A[(size_t)(k0 + r) * p + k0 + c];
   }
   __syncthreads();
-  for (int j = 0; j < b; ++j) {
-    const double dj = Ld[j][j];
-    const double sj = sqrt(dj);
-    if (blockIdx.x == 0 && tid == 0 && !(dj > 0.0) && info[blockIdx.y] == 0) info[blockIdx.y] = k0 + j + 1;
-    const int m = b - j - 1;
-    __syncthreads();
-    if (tid < m) Ld[j + 1 + tid][j] /= sj;
-    if (tid == 0) Ld[j][j] = sj;
-    __syncthreads();
-    for (int e = tid; e < m * m; e += 256) {
-      const int t = j + 1 + e / m, c = j + 1 + e % m;
-      if (c <= t) Ld[t][c] -= Ld[t][j] * Ld[c][j];
+  // The diagonal block is factored by wave 0 alone, lane t holding row t in
+  // registers (columns fully unrolled); per column the scaled column passes
+  // through LDS.  Lanes of one wave see each other's LDS writes in program
+  // order, so the steps need no workgroup barrier.
+  __shared__ double colj[CB];
+  if (tid < 64) {
+    const int t = tid;
+    double x[CB];
+#pragma unroll
+    for (int c = 0; c < CB; ++c) x[c] = (t < b && c <= t) ? Ld[t][c] : 0.0;
+#pragma unroll
+    for (int j = 0; j < CB; ++j) {
+      if (j < b) {
+        const double dj = __shfl(x[j], j, 64);
+        const double sj = sqrt(dj);
+        if (blockIdx.x == 0 && t == 0 && !(dj > 0.0) && info[blockIdx.y] == 0) info[blockIdx.y] = k0 + j + 1;
+        double l = 0.0;
+        if (t > j) {
+          l = x[j] / sj;
+          x[j] = l;
+        } else if (t == j) {
+          x[j] = sj;
+        }
+        colj[t] = l;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+        for (int c = j + 1; c < CB; ++c) {
+          const double lc = colj[c];
+          if (c <= t) x[c] -= l * lc;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
     }
-    __syncthreads();
+    if (t < b) {
+#pragma unroll
+      for (int c = 0; c < CB; ++c)
+        if (c <= t) Ld[t][c] = x[c];
+    }
   }
+  __syncthreads();
   if (blockIdx.x == 0) {
     for (int e = tid; e < CB * CB; e += 256) {
       const int r = e / CB, c = e % CB;
@@ -142,95 +172,90 @@ __global__ void chol_zero_info(int* info) {
   if (threadIdx.x < 2) info[threadIdx.x] = 0;
 }
 
-// x = (L L^T)^{-1} b with L the lower triangle of the row-major p x p matrix.
-// One workgroup of 1024 threads, left-looking over blocks of 64 unknowns: all
-// threads form the block's right-hand side from the unknowns already solved
-// (independent loads, several accumulators per thread), then wave 0 solves
-// the diagonal block from LDS.  The unknowns live in LDS (XL) or in x.
-constexpr int SOLVE_LDS_MAX = 3000;
+// W = L^{-1} for the lower-triangular L, right-looking and in place on W = I
+// (row-major; the strict upper triangle stays zero).  Block row i: every tile
+// W_ij (j <= i) becomes L_ii^{-1} W_ij (inv_row: forward substitution with
+// L_ii in LDS, four threads per column, thread q owning rows q, q + 4, ...),
+// then every tile below takes W_kj -= L_ki W_ij (inv_update: FP64 MFMA
+// tiles, k > i, j <= i).
+__global__ void inv_identity(double* __restrict__ W, int64_t p) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < p * p) W[t] = (t / p == t % p) ? 1.0 : 0.0;
+}
 
-template <bool XL>
-__global__ void __launch_bounds__(1024) chol_solve_kernel(const double* __restrict__ L, int p,
-                                                         const double* __restrict__ b, double* x) {
-  __shared__ double Ld[CB][CB + 1];
-  __shared__ double red[16][CB];
-  __shared__ double xl[XL ? SOLVE_LDS_MAX : 1];
-  double* xv = XL ? xl : x;
+__global__ void __launch_bounds__(256) inv_row(const double* __restrict__ L, int p, int i0, double* W) {
+  __shared__ double Ls[CB][CB + 1];
+  const int tid = threadIdx.x, lane = tid % 64;
+  const int c0 = CB * blockIdx.x;  // tile column block j <= i
+  for (int e = tid; e < CB * CB; e += 256) {
+    const int r = e / CB, c = e % CB;
+    double v = r == c ? 1.0 : 0.0;
+    if (i0 + r < p && c <= r) v = L[(size_t)(i0 + r) * p + i0 + c];
+    Ls[r][c] = v;
+  }
+  __syncthreads();
+  const int col = tid / 4, q = tid % 4;
+  const int gc = c0 + col;
+  const bool live = gc < p;
+  double x[CB / 4];
+#pragma unroll
+  for (int u = 0; u < CB / 4; ++u) {
+    const int gi = i0 + q + 4 * u;
+    x[u] = (live && gi < p) ? W[(size_t)gi * p + gc] : 0.0;
+  }
+  const int base = lane & ~3;
+#pragma unroll
+  for (int r = 0; r < CB; ++r) {
+    if (q == (r & 3)) x[r >> 2] /= Ls[r][r];
+    const double xr = __shfl(x[r >> 2], base + (r & 3), 64);
+#pragma unroll
+    for (int u = 0; u < CB / 4; ++u)
+      if (q + 4 * u > r) x[u] -= Ls[q + 4 * u][r] * xr;
+  }
+#pragma unroll
+  for (int u = 0; u < CB / 4; ++u) {
+    const int gi = i0 + q + 4 * u;
+    if (live && gi < p) W[(size_t)gi * p + gc] = x[u];
+  }
+}
+
+// tiles (k, j): k over the block rows below i0 (blockIdx.x / (i + 1)), j <= i
+__global__ void __launch_bounds__(256) inv_update(const double* __restrict__ L, int p, int i0, double* W) {
+  const int nj = i0 / CB + 1;
+  const int kt = blockIdx.x / nj, jt = blockIdx.x % nj;
+  const int r0 = i0 + CB * (kt + 1), c0 = CB * jt;
+  __shared__ double As[32][CB + 1], Bs[32][CB + 1];
   const int tid = threadIdx.x, lane = tid % 64, wv = tid / 64;
-  const int nblk = (p + CB - 1) / CB;
-  // forward: L y = b
-  for (int bi = 0; bi < nblk; ++bi) {
-    const int kb = bi * CB, nb = min(CB, p - kb);
-    for (int e = tid; e < CB * CB; e += 1024) {
-      const int r = e / CB, c = e % CB;
-      if (r < nb && c <= r) Ld[r][c] = L[(size_t)(kb + r) * p + kb + c];
-    }
-    // row r = tid / 16 of the block, 16 threads per row over the columns < kb
-    {
-      const int r = tid / 16, sg = tid % 16;
-      double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-      if (r < nb) {
-        const double* Lr = L + (size_t)(kb + r) * p;
-        int l = sg;
-        for (; l + 112 < kb; l += 128) {
-#pragma unroll
-          for (int u = 0; u < 8; ++u) s[u] += Lr[l + 16 * u] * xv[l + 16 * u];
-        }
-        for (; l < kb; l += 16) s[0] += Lr[l] * xv[l];
-      }
-      double t = ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
-      for (int o = 8; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
-      if (sg == 0) red[0][r] = t;
+  const int wm = (wv / 2) * 32, wn = (wv % 2) * 32;
+  v4d acc[2][2];
+  for (int a = 0; a < 2; ++a)
+    for (int c = 0; c < 2; ++c) acc[a][c] = v4d{0, 0, 0, 0};
+  for (int kc = 0; kc < CB; kc += 32) {
+    for (int u = 0; u < (CB * 32) / 256; ++u) {
+      const int e = tid + 256 * u;
+      const int m = e / 32, k = e % 32;  // As[k][m] = L[r0 + m][i0 + kc + k]
+      As[k][m] = r0 + m < p ? L[(size_t)(r0 + m) * p + i0 + kc + k] : 0.0;
+      const int n = e % CB, kb = e / CB;  // Bs[kb][n] = W[i0 + kc + kb][c0 + n]
+      Bs[kb][n] = (i0 + kc + kb < p && c0 + n < p) ? W[(size_t)(i0 + kc + kb) * p + c0 + n] : 0.0;
     }
     __syncthreads();
-    if (wv == 0) {
-      double r = lane < nb ? b[kb + lane] - red[0][lane] : 0.0;
-      for (int j = 0; j < nb; ++j) {
-        const double yj = __shfl(r, j, 64) / Ld[j][j];
-        if (lane == j) r = yj;
-        else if (lane > j && lane < nb) r -= Ld[lane][j] * yj;
-      }
-      if (lane < nb) xv[kb + lane] = r;
+#pragma unroll
+    for (int ks = 0; ks < 32; ks += 4) {
+      double ar[2], br[2];
+      for (int a = 0; a < 2; ++a) ar[a] = As[ks + lane / 16][wm + 16 * a + lane % 16];
+      for (int c = 0; c < 2; ++c) br[c] = Bs[ks + lane / 16][wn + 16 * c + lane % 16];
+      for (int a = 0; a < 2; ++a)
+        for (int c = 0; c < 2; ++c) acc[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], br[c], acc[a][c], 0, 0, 0);
     }
     __syncthreads();
   }
-  // backward: L^T x = y, blocks from the last; column c of the block gets
-  // sum over the solved rows below of L[i][c] x[i] (lanes over c, waves over i)
-  for (int bi = nblk - 1; bi >= 0; --bi) {
-    const int kb = bi * CB, nb = min(CB, p - kb);
-    for (int e = tid; e < CB * CB; e += 1024) {
-      const int r = e / CB, c = e % CB;
-      if (r < nb && c <= r) Ld[r][c] = L[(size_t)(kb + r) * p + kb + c];
-    }
-    {
-      double s[4] = {0, 0, 0, 0};
-      if (lane < nb) {
-        const double* Lc = L + kb + lane;
-        int i = kb + nb + wv;
-        for (; i + 48 < p; i += 64) {
-#pragma unroll
-          for (int u = 0; u < 4; ++u) s[u] += Lc[(size_t)(i + 16 * u) * p] * xv[i + 16 * u];
-        }
-        for (; i < p; i += 16) s[0] += Lc[(size_t)i * p] * xv[i];
+  for (int a = 0; a < 2; ++a)
+    for (int c = 0; c < 2; ++c)
+      for (int r = 0; r < 4; ++r) {
+        const int gi = r0 + wm + 16 * a + 4 * r + lane / 16;
+        const int gj = c0 + wn + 16 * c + lane % 16;
+        if (gi < p && gj < p) W[(size_t)gi * p + gj] -= acc[a][c][r];
       }
-      red[wv][lane] = (s[0] + s[1]) + (s[2] + s[3]);
-    }
-    __syncthreads();
-    if (wv == 0) {
-      double s = 0.0;
-      for (int w = 0; w < 16; ++w) s += red[w][lane];
-      double r = lane < nb ? xv[kb + lane] - s : 0.0;
-      for (int j = nb - 1; j >= 0; --j) {
-        const double xj = __shfl(r, j, 64) / Ld[j][j];
-        if (lane == j) r = xj;
-        else if (lane < j) r -= Ld[j][lane] * xj;
-      }
-      if (lane < nb) xv[kb + lane] = r;
-    }
-    __syncthreads();
-  }
-  if (XL)
-    for (int i = tid; i < p; i += 1024) x[i] = xl[i];
 }
 
 }  // namespace
@@ -257,11 +282,22 @@ int chol_factor2(double* A0, double* A1, int p, int* info, hipStream_t st) {
   return JV_OK;
 }
 
-// x = (L L^T)^{-1} b (x and b may not alias).
-int chol_solve(const double* L, int p, const double* b, double* x, hipStream_t st) {
-  if (p <= SOLVE_LDS_MAX) hipLaunchKernelGGL(chol_solve_kernel<true>, dim3(1), dim3(1024), 0, st, L, p, b, x);
-  else hipLaunchKernelGGL(chol_solve_kernel<false>, dim3(1), dim3(1024), 0, st, L, p, b, x);
-  JV_LAUNCH_CHECK("chol_solve_kernel");
+// W = L^{-1} of the factor L (W and L may not alias).
+int chol_inverse_lower(const double* L, int p, double* W, hipStream_t st) {
+  const int64_t pp = (int64_t)p * p;
+  hipLaunchKernelGGL(inv_identity, dim3((unsigned)((pp + 255) / 256)), dim3(256), 0, st, W, (int64_t)p);
+  JV_LAUNCH_CHECK("inv_identity");
+  const int nblk = (p + CB - 1) / CB;
+  for (int bi = 0; bi < nblk; ++bi) {
+    const int i0 = CB * bi;
+    hipLaunchKernelGGL(inv_row, dim3((unsigned)(bi + 1)), dim3(256), 0, st, L, p, i0, W);
+    JV_LAUNCH_CHECK("inv_row");
+    const int below = nblk - bi - 1;
+    if (below > 0) {
+      hipLaunchKernelGGL(inv_update, dim3((unsigned)(below * (bi + 1))), dim3(256), 0, st, L, p, i0, W);
+      JV_LAUNCH_CHECK("inv_update");
+    }
+  }
   return JV_OK;
 }
 

@@ -1285,7 +1285,7 @@ static bool minsr_cholesky_enabled() {
 }
 
 int chol_factor2(double* A0, double* A1, int p, int* info, hipStream_t st);
-int chol_solve(const double* L, int p, const double* b, double* x, hipStream_t st);
+int chol_inverse_lower(const double* L, int p, double* W, hipStream_t st);
 
 static int chol_route(const double* S, int64_t p, double rcond, int* flags, bool* ok, double** L, hipStream_t st) {
   *ok = false;
@@ -1379,12 +1379,19 @@ static int minsr_dual(const double* data, const double* e, const double* e_raw, 
     double* L = nullptr;
     if ((rc = chol_route(S, p, rcond, rank, &ok, &L, st))) return rc;
     if (ok) {
+      // S^-1 = W^T W with W = L^-1 (into S), formed while E_loc may still run;
+      // after b only a GEMV remains
+      {
+        ScopedTimer tm("chol_inverse", st);
+        double* Wi = (double*)workspace(sizeof(double) * (size_t)p * p, 95);  // S - c I is no longer needed
+        if (!Wi) return fail(JV_E_NOMEM, "jv_minsr_solve_dual: workspace allocation failed");
+        if ((rc = chol_inverse_lower(L, (int)p, Wi, st))) return rc;
+        GemmArgs g{p, p, p, Wi, 1, p, 0, Wi, p, 1, 0, S, p, 0, 1, 1.0};
+        if ((rc = gemm(g, st))) return rc;
+      }
       if (deferred) JV_HIP(hipStreamWaitEvent(st, ev_b, 0));
       else if ((rc = rhs_b(st))) return rc;
-      {
-        ScopedTimer tm("chol_solve", st);
-        if ((rc = chol_solve(L, (int)p, b, x, st))) return rc;
-      }
+      if ((rc = gemv(false, S, b, p, p, 0, 0, 1.0, x, st, 12))) return rc;
       hipLaunchKernelGGL(set_ints, dim3(1), dim3(64), 0, st, rank, (int)p, 1);
       JV_LAUNCH_CHECK("set_ints");
       hipLaunchKernelGGL(neg_copy, dim3((unsigned)((p + 255) / 256)), dim3(256), 0, st, x, p, update);
