@@ -13,9 +13,13 @@ region starts.
 The MinSR pseudo-inverse is computed in its dual form by default: the nonzero
 spectrum of the 8192 x 8192 T = O O^T equals that of the 1240 x 1240 O^T O, and
 -O^T pinv(T) E = -pinv(O^T O) O^T E with the same relative cutoff, so the
-update is the reference's (the parity tests compare both forms).  The
-reference's T form (8192^2 Gram + FP64 eigendecomposition of T) is timed for a
-few steps and reported beside it.
+update is the reference's (the parity tests compare both forms).  While no
+eigenvalue of O^T O lies at or below the cutoff (the first ~16 steps from the
+random initial network), the pseudo-inverse is the inverse and the solve is a
+Cholesky solve; later steps diagonalise O^T O.  ``minsr_cholesky_steps``
+reports how many timed steps took the Cholesky route.  The reference's T form
+(8192^2 Gram + FP64 eigendecomposition of T) is timed for a few steps and
+reported beside it.
 
 ``--workload c1``: 1D TFIM L=10, h=0.7, complex RBM alpha=2 with visible bias,
 exact sampler, imaginary-time TDVP (Euler dt=1e-2, snrTol=2, pinvTol=1e-8).
@@ -91,7 +95,8 @@ class Workload:
             self.stepper = jv.util.stepper.Euler(timeStep=5e-3)
             self.desc = ("TFIM 10x10 periodic, h=3; CNN 3x(8ch,3x3) log-cosh, P=1240; 512 chains, 8192 samples/step; "
                          "MinSR pinvTol=1e-10 (%s); Euler dt=5e-3" %
-                         ("dual form: eigendecomposition of the 1240x1240 O^T O, same update as the 8192x8192 T solve"
+                         ("dual form on the 1240x1240 O^T O, same update as the 8192x8192 T solve: Cholesky while no "
+                          "eigenvalue falls below the cutoff, else eigendecomposition"
                           if mode == "dual" else "T form: 8192x8192 T Gram + FP64 eigendecomposition"))
         elif name == "c2":
             self.shape, self.samples = (32,), 16384
