@@ -39,9 +39,19 @@ __global__ void __launch_bounds__(256) chol_panel(double* A0, double* A1, int p,
   __shared__ double Ld[CB][CB + 1];
   const int tid = threadIdx.x;
   const int b = min(CB, p - k0);
-  for (int e = tid; e < CB * CB; e += 256) {
-    const int r = e / CB, c = e % CB;
-    if (r < b && c <= r) Ld[r][c] = A[(size_t)(k0 + r) * p + k0 + c];
+  {  // all 16 loads in flight together (clamped, always valid addresses), then LDS
+    double v[(CB * CB) / 256];
+#pragma unroll
+    for (int u = 0; u < (CB * CB) / 256; ++u) {
+      const int e = tid + 256 * u, r = e / CB, c = e % CB;
+      const int rr = min(r, b - 1), cc = min(c, rr);
+      v[u] = A[(size_t)(k0 + rr) * p + k0 + cc];
+    }
+#pragma unroll
+    for (int u = 0; u < (CB * CB) / 256; ++u) {
+      const int e = tid + 256 * u, r = e / CB, c = e % CB;
+      Ld[r][c] = (r < b && c <= r) ? v[u] : 0.0;
+    }
   }
   __syncthreads();
   // The diagonal block is factored by wave 0 alone, lane t holding row t in
@@ -51,6 +61,7 @@ __global__ void __launch_bounds__(256) chol_panel(double* A0, double* A1, int p,
   __shared__ double colj[CB];
   if (tid < 64) {
     const int t = tid;
+    int first_bad = 0;  // 1-based column of the first non-positive pivot
     double x[CB];
 #pragma unroll
     for (int c = 0; c < CB; ++c) x[c] = (t < b && c <= t) ? Ld[t][c] : 0.0;
@@ -59,7 +70,7 @@ __global__ void __launch_bounds__(256) chol_panel(double* A0, double* A1, int p,
       if (j < b) {
         const double dj = __shfl(x[j], j, 64);
         const double sj = sqrt(dj);
-        if (blockIdx.x == 0 && t == 0 && !(dj > 0.0) && info[blockIdx.y] == 0) info[blockIdx.y] = k0 + j + 1;
+        if (first_bad == 0 && !(dj > 0.0)) first_bad = j + 1;
         double l = 0.0;
         if (t > j) {
           l = x[j] / sj;
@@ -86,6 +97,7 @@ __global__ void __launch_bounds__(256) chol_panel(double* A0, double* A1, int p,
       for (int c = 0; c < CB; ++c)
         if (c <= t) Ld[t][c] = x[c];
     }
+    if (blockIdx.x == 0 && t == 0 && first_bad != 0 && info[blockIdx.y] == 0) info[blockIdx.y] = k0 + first_bad;
   }
   __syncthreads();
   if (blockIdx.x == 0) {
@@ -102,7 +114,10 @@ __global__ void __launch_bounds__(256) chol_panel(double* A0, double* A1, int p,
   double* row = A + (size_t)(live ? i : k0) * p + k0;
   double x[CB / 4];
 #pragma unroll
-  for (int k = 0; k < CB / 4; ++k) x[k] = live ? row[q + 4 * k] : 0.0;
+  for (int k = 0; k < CB / 4; ++k) {
+    const double v = row[q + 4 * k];  // row points at a valid row even when !live
+    x[k] = live ? v : 0.0;
+  }
   const int base = (threadIdx.x % 64) & ~3;
 #pragma unroll
   for (int j = 0; j < CB; ++j) {
@@ -141,12 +156,20 @@ __global__ void __launch_bounds__(256) chol_update(double* A0, double* A1, int p
   for (int a = 0; a < 2; ++a)
     for (int c = 0; c < 2; ++c) acc[a][c] = v4d{0, 0, 0, 0};
   for (int kc = 0; kc < CB; kc += 32) {
+    double va[(CB * 32) / 256], vb[(CB * 32) / 256];
+#pragma unroll
     for (int r = 0; r < (CB * 32) / 256; ++r) {
       const int e = tid + 256 * r;
       const int m = e / 32, k = e % 32;
-      const int gi = i0 + m, gj = j0 + m;
-      As[k][m] = gi < p ? A[(size_t)gi * p + k0 + kc + k] : 0.0;
-      Bs[k][m] = gj < p ? A[(size_t)gj * p + k0 + kc + k] : 0.0;
+      va[r] = A[(size_t)min(i0 + m, p - 1) * p + k0 + kc + k];
+      vb[r] = A[(size_t)min(j0 + m, p - 1) * p + k0 + kc + k];
+    }
+#pragma unroll
+    for (int r = 0; r < (CB * 32) / 256; ++r) {
+      const int e = tid + 256 * r;
+      const int m = e / 32, k = e % 32;
+      As[k][m] = i0 + m < p ? va[r] : 0.0;
+      Bs[k][m] = j0 + m < p ? vb[r] : 0.0;
     }
     __syncthreads();
 #pragma unroll
@@ -187,11 +210,20 @@ __global__ void __launch_bounds__(256) inv_row(const double* __restrict__ L, int
   __shared__ double Ls[CB][CB + 1];
   const int tid = threadIdx.x, lane = tid % 64;
   const int c0 = CB * blockIdx.x;  // tile column block j <= i
-  for (int e = tid; e < CB * CB; e += 256) {
-    const int r = e / CB, c = e % CB;
-    double v = r == c ? 1.0 : 0.0;
-    if (i0 + r < p && c <= r) v = L[(size_t)(i0 + r) * p + i0 + c];
-    Ls[r][c] = v;
+  const int nb = min(CB, p - i0);
+  {
+    double v[(CB * CB) / 256];
+#pragma unroll
+    for (int u = 0; u < (CB * CB) / 256; ++u) {
+      const int e = tid + 256 * u, r = e / CB, c = e % CB;
+      const int rr = min(r, nb - 1), cc = min(c, rr);
+      v[u] = L[(size_t)(i0 + rr) * p + i0 + cc];
+    }
+#pragma unroll
+    for (int u = 0; u < (CB * CB) / 256; ++u) {
+      const int e = tid + 256 * u, r = e / CB, c = e % CB;
+      Ls[r][c] = (r < nb && c <= r) ? v[u] : (r == c ? 1.0 : 0.0);
+    }
   }
   __syncthreads();
   const int col = tid / 4, q = tid % 4;
@@ -201,7 +233,8 @@ __global__ void __launch_bounds__(256) inv_row(const double* __restrict__ L, int
 #pragma unroll
   for (int u = 0; u < CB / 4; ++u) {
     const int gi = i0 + q + 4 * u;
-    x[u] = (live && gi < p) ? W[(size_t)gi * p + gc] : 0.0;
+    const double v = W[(size_t)min(gi, p - 1) * p + min(gc, p - 1)];
+    x[u] = (live && gi < p) ? v : 0.0;
   }
   const int base = lane & ~3;
 #pragma unroll
@@ -231,12 +264,22 @@ __global__ void __launch_bounds__(256) inv_update(const double* __restrict__ L, 
   for (int a = 0; a < 2; ++a)
     for (int c = 0; c < 2; ++c) acc[a][c] = v4d{0, 0, 0, 0};
   for (int kc = 0; kc < CB; kc += 32) {
+    double va[(CB * 32) / 256], vb[(CB * 32) / 256];
+#pragma unroll
     for (int u = 0; u < (CB * 32) / 256; ++u) {
       const int e = tid + 256 * u;
       const int m = e / 32, k = e % 32;  // As[k][m] = L[r0 + m][i0 + kc + k]
-      As[k][m] = r0 + m < p ? L[(size_t)(r0 + m) * p + i0 + kc + k] : 0.0;
+      va[u] = L[(size_t)min(r0 + m, p - 1) * p + i0 + kc + k];
       const int n = e % CB, kb = e / CB;  // Bs[kb][n] = W[i0 + kc + kb][c0 + n]
-      Bs[kb][n] = (i0 + kc + kb < p && c0 + n < p) ? W[(size_t)(i0 + kc + kb) * p + c0 + n] : 0.0;
+      vb[u] = W[(size_t)min(i0 + kc + kb, p - 1) * p + min(c0 + n, p - 1)];
+    }
+#pragma unroll
+    for (int u = 0; u < (CB * 32) / 256; ++u) {
+      const int e = tid + 256 * u;
+      const int m = e / 32, k = e % 32;
+      As[k][m] = r0 + m < p ? va[u] : 0.0;
+      const int n = e % CB, kb = e / CB;
+      Bs[kb][n] = (i0 + kc + kb < p && c0 + n < p) ? vb[u] : 0.0;
     }
     __syncthreads();
 #pragma unroll
