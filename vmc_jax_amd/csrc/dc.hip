@@ -16,7 +16,7 @@
 //  * eigenvectors from recomputed weights z^_i (Gu and Eisenstat), which keeps
 //    them orthogonal: z^_i^2 = (lambda_i - d_i) / rho * prod_{j != i}
 //    (lambda_j - d_i) / (d_j - d_i), u_j = (z^_i / (d_i - lambda_j))_i / norm;
-//  * the merged eigenvectors [Q1 0; 0 Q2] P G U as one GEMM per merge.
+//  * the merged eigenvectors [Q1 0; 0 Q2] P G U, one batched GEMM per level.
 // Merges run level by level, all merges of a level in the same launches; the
 // eigenvector matrix is updated in place.
 #include <hip/hip_runtime.h>
@@ -30,8 +30,6 @@
 
 namespace jv {
 
-int gemm_real(int64_t M, int64_t N, int64_t K, const double* A, int64_t sam, int64_t sak, const double* B,
-              int64_t sbk, int64_t sbn, double* C, int64_t ldc, hipStream_t st);
 
 namespace {
 
@@ -488,6 +486,68 @@ __global__ void __launch_bounds__(256) dc_vectors(int lvk, int n, const DcBufs b
   for (int i = lane; i < nn; i += WAVE) gc[i] = qs[i];
 }
 
+// Merged eigenvectors of every node of a level in one launch (blockIdx.y =
+// node): C(r, j) = sum_{i < k} G(r, i) U(i, j) for the k non-deflated columns
+// (column-major blocks at offset start (n + 1) inside the n x n buffers;
+// columns j >= k are the deflated ones, taken from G as they are).  Tiles of
+// C^T (64 j x 64 r) with FP64 MFMA 16x16x4 over K chunks of 32, four waves
+// of 32 x 32 (A operand A[row = lane % 16][k = lane / 16], B operand
+// B[k = lane / 16][col = lane % 16], result element q of lane l at
+// C[4 q + l / 16][l % 16]); tiles and chunks beyond k are skipped.
+__global__ void __launch_bounds__(256) dc_merge_gemm(int lvk, int n, DcBufs b, const double* __restrict__ G,
+                                                    const double* __restrict__ U, double* __restrict__ C) {
+  const DcNode nd = dc_node(n, lvk, blockIdx.y);
+  const int s0 = nd.start, m = nd.n1 + nd.n2;
+  const int k = b.cnt[2 * s0];
+  const int T = (m + 63) / 64;
+  const int j0 = 64 * (blockIdx.x / T), r0 = 64 * (blockIdx.x % T);
+  if (blockIdx.x >= (unsigned)(T * T) || j0 >= k) return;  // uniform over the workgroup
+  __shared__ double As[32][65], Bs[32][65];
+  const int tid = threadIdx.x, lane = tid % 64, wv = tid / 64;
+  const int wm = (wv / 2) * 32, wn = (wv % 2) * 32;
+  typedef double v4d __attribute__((ext_vector_type(4)));
+  v4d acc[2][2];
+  for (int a = 0; a < 2; ++a)
+    for (int c = 0; c < 2; ++c) acc[a][c] = v4d{0, 0, 0, 0};
+  const size_t base = (size_t)s0 * n + s0;
+  for (int i0 = 0; i0 < k; i0 += 32) {
+    double va[8], vb[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = tid + 256 * u;
+      const int jj = e / 32, kk = e % 32;  // As[kk][jj] = U(i0 + kk, j0 + jj)
+      va[u] = U[base + (size_t)min(j0 + jj, m - 1) * n + min(i0 + kk, m - 1)];
+      const int kb = e / 64, rr = e % 64;  // Bs[kb][rr] = G(r0 + rr, i0 + kb)
+      vb[u] = G[base + (size_t)min(i0 + kb, m - 1) * n + min(r0 + rr, m - 1)];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = tid + 256 * u;
+      const int jj = e / 32, kk = e % 32;
+      As[kk][jj] = (j0 + jj < k && i0 + kk < k) ? va[u] : 0.0;
+      const int kb = e / 64, rr = e % 64;
+      Bs[kb][rr] = (i0 + kb < k && r0 + rr < m) ? vb[u] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < 32; ks += 4) {
+      double ar[2], br[2];
+      for (int a = 0; a < 2; ++a) ar[a] = As[ks + lane / 16][wm + 16 * a + lane % 16];
+      for (int c = 0; c < 2; ++c) br[c] = Bs[ks + lane / 16][wn + 16 * c + lane % 16];
+      for (int a = 0; a < 2; ++a)
+        for (int c = 0; c < 2; ++c) acc[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], br[c], acc[a][c], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  for (int a = 0; a < 2; ++a)
+    for (int c = 0; c < 2; ++c)
+      for (int q = 0; q < 4; ++q) {
+        const int j = j0 + wm + 16 * a + 4 * q + lane / 16;
+        const int r = r0 + wn + 16 * c + lane % 16;
+        if (j < k && r < m) C[base + (size_t)j * n + r] = acc[a][c][q];
+      }
+}
+
 // Final order of a node: the roots (entries t < k) and deflated values
 // (entries t >= k) sorted by (NaN-safe key, entry index) with a bitonic
 // network in LDS, padded to a power of two with +inf keys and indices >= nn.
@@ -608,14 +668,11 @@ int dc_tridiag_eig(int n, double* d, const double* e, double* Z, hipStream_t st)
     JV_LAUNCH_CHECK("dc_zhat");
     hipLaunchKernelGGL(dc_vectors, dim3((unsigned)((mx + 3) / 4), nodes), dim3(256), 0, st, k, n, b, Z, U, G);
     JV_LAUNCH_CHECK("dc_vectors");
-    // C block = G block U block (column-major blocks inside n x n buffers):
-    // C^T = U^T G^T computed row-major
-    for (unsigned i = 0; i < nodes; ++i) {
-      const DcNode x = dc_node(n, k, (int)i);
-      const int64_t m = x.n1 + x.n2;
-      const size_t o = (size_t)x.start * n + x.start;
-      const int rc = gemm_real(m, m, m, U + o, n, 1, G + o, n, 1, C + o, n, st);
-      if (rc) return rc;
+    // C block = G block U block for every node of the level
+    {
+      const unsigned T = (unsigned)((mx + 63) / 64);
+      hipLaunchKernelGGL(dc_merge_gemm, dim3(T * T, nodes), dim3(256), 0, st, k, n, b, G, U, C);
+      JV_LAUNCH_CHECK("dc_merge_gemm");
     }
     int p2 = 1;
     while (p2 < mx) p2 <<= 1;

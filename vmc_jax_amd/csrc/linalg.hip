@@ -432,13 +432,6 @@ __global__ void scale_vec(double* __restrict__ v, int64_t len, double alpha) {
   if (i < len) v[i] *= alpha;
 }
 
-// Real C[M][N] = opA opB with the strides of GemmArgs (no conjugation, alpha 1).
-int gemm_real(int64_t M, int64_t N, int64_t K, const double* A, int64_t sam, int64_t sak, const double* B,
-              int64_t sbk, int64_t sbn, double* C, int64_t ldc, hipStream_t st) {
-  GemmArgs g{M, N, K, A, sam, sak, 0, B, sbk, sbn, 0, C, ldc, 0, 0, 1.0};
-  return gemm(g, st);
-}
-
 int gemv(bool trans, const double* M, const double* x, int64_t rows, int64_t cols, int cpx, int conjM,
          double alpha, double* y, hipStream_t st, int ws_slot) {
   if (!trans) {
