@@ -133,8 +133,11 @@ typedef struct jv_solve_info {
                                 below the cutoff and the update came from a
                                 Cholesky solve, 0 from the eigendecomposition */
   int32_t rank;              /* MinSR: eigenvalues kept by the pseudo-inverse */
-  double e_var;              /* TDVP: sum_n |e_n|^2 = Var E_loc (stats.py:248-252) */
+  double e_var;              /* TDVP and MinSR deferred dual: sum_n |e_n|^2 of the centred,
+                                weighted E_loc = Var E_loc (stats.py:248-252) */
   double tdvp_error;         /* TDVP: tdvp_error metadata (tdvp.py:102-104) */
+  double e_mean[2];          /* MinSR deferred dual: weighted mean of E_loc (stats.py:197-205),
+                                complex (re, im) */
 } jv_solve_info;
 
 /* ------------------------------------------------------------------ misc */

@@ -17,7 +17,7 @@ int main(void) {
          sizeof(jv_optable), sizeof(jv_mc_args), sizeof(jv_solve_info), sizeof(jv_orbit));
   P(jv_net, num_sites); P(jv_net, num_params); P(jv_net, off_conv_kernel); P(jv_net, off_conv_bias);
   P(jv_optable, max_sites_changed); P(jv_optable, idx); P(jv_optable, diag); P(jv_optable, fermi); P(jv_mc_args, mu); P(jv_mc_args, seed); P(jv_mc_args, stream);
-  P(jv_solve_info, rank); P(jv_solve_info, e_var); P(jv_solve_info, tdvp_error);
+  P(jv_solve_info, rank); P(jv_solve_info, e_var); P(jv_solve_info, tdvp_error); P(jv_solve_info, e_mean);
   P(jv_orbit, avg); P(jv_orbit, complex_factor); P(jv_orbit, map); P(jv_orbit, factor);
   return 0;
 }

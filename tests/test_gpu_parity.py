@@ -602,6 +602,7 @@ def test_minsr_dual_overlapped_call_matches_oracle(stedc):
     ref = eqo(opsi.get_parameters(), 0.0, psi=opsi, hamiltonian=Ho, intStep=0)
     assert rel(upd, ref) < 1e-7
     assert abs(eq.get_energy_mean() - eqo.get_energy_mean()) < 1e-11 * abs(eqo.get_energy_mean())
+    assert abs(eq.get_energy_variance() - eqo.get_energy_variance()) < 1e-10 * abs(eqo.get_energy_variance())
 
 
 @pytest.mark.parametrize("shape,ch,N,seed", [((10, 10), (8, 8, 8), 2048, 31), ((6, 6), (4, 4, 4), 128, 32)])

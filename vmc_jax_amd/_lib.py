@@ -61,7 +61,7 @@ class JvMcArgs(ctypes.Structure):
 
 class JvSolveInfo(ctypes.Structure):
     _fields_ = [("residual", _dbl), ("pinv_cutoff", _dbl), ("iterations", _i32), ("rank", _i32),
-                ("e_var", _dbl), ("tdvp_error", _dbl)]
+                ("e_var", _dbl), ("tdvp_error", _dbl), ("e_mean", _dbl * 2)]
 
 
 # name -> argtypes; every function returns int status

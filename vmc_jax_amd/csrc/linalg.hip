@@ -1203,7 +1203,8 @@ __global__ void neg_copy(const double* __restrict__ in, int64_t len, double* __r
 static int minsr_dual(const double* data, const double* e, const double* e_raw, const double* w,
                       hipEvent_t e_ready, int64_t n, int64_t p, int32_t is_complex, double rcond, double diag_shift,
                       bool try_chol, double* update, hipStream_t st);
-static int minsr_dual_finish_info(int rc, jv_solve_info* info, int64_t p, double rcond, hipStream_t st);
+static int minsr_dual_finish_info(int rc, jv_solve_info* info, int64_t p, double rcond, bool deferred,
+                                  hipStream_t st);
 }  // namespace jv
 
 extern "C" {
@@ -1215,7 +1216,7 @@ int jv_minsr_solve_dual(const double* data, const double* e, int64_t n, int64_t 
   hipStream_t st = (hipStream_t)stream;
   return minsr_dual_finish_info(minsr_dual(data, e, nullptr, nullptr, nullptr, n, p, is_complex, rcond, diag_shift,
                                            try_cholesky != 0, update, st),
-                                info, p, rcond, st);
+                                info, p, rcond, false, st);
 }
 
 int jv_minsr_solve_dual_deferred(const double* data, const double* e_raw, const double* w, void* e_ready, int64_t n,
@@ -1226,7 +1227,7 @@ int jv_minsr_solve_dual_deferred(const double* data, const double* e_raw, const 
   hipStream_t st = (hipStream_t)stream;
   return minsr_dual_finish_info(minsr_dual(data, nullptr, e_raw, w, (hipEvent_t)e_ready, n, p, is_complex, rcond,
                                            diag_shift, try_cholesky != 0, update, st),
-                                info, p, rcond, st);
+                                info, p, rcond, true, st);
 }
 
 }  // extern "C"
@@ -1335,7 +1336,8 @@ static int minsr_dual(const double* data, const double* e, const double* e_raw, 
     JV_LAUNCH_CHECK("add_diag");
   }
   const bool deferred = e == nullptr;
-  double* emean = deferred ? (double*)workspace(sizeof(double) * 2, 50) : nullptr;
+  // deferred: E_loc mean (2 doubles) and variance, read back with the solve info
+  double* emean = deferred ? (double*)workspace(sizeof(double) * 4, 50) : nullptr;
   double* ec = deferred ? (double*)workspace(sizeof(double) * 2 * (size_t)n, 51) : nullptr;
   if (deferred && (!emean || !ec)) return fail(JV_E_NOMEM, "jv_minsr_solve_dual: workspace allocation failed");
   // centred e (when deferred) and b = D^dagger e on stream s
@@ -1345,6 +1347,7 @@ static int minsr_dual(const double* data, const double* e, const double* e_raw, 
       if (e_ready) JV_HIP(hipStreamWaitEvent(s, e_ready, 0));
       int r = center(e_raw, w, n, 1, 1, emean, ec, s, 48);
       if (r) return r;
+      if ((r = col_norm2(ec, n, 1, 1, emean + 2, s, 98))) return r;  // Var E_loc (stats.py:248-252)
       ee = ec;
     }
     const double* rhs = ee;
@@ -1466,13 +1469,25 @@ static int minsr_dual(const double* data, const double* e, const double* e_raw, 
 }
 
 // Reads the pseudo-inverse rank back into info (after a successful solve).
-static int minsr_dual_finish_info(int rc, jv_solve_info* info, int64_t p, double rcond, hipStream_t st) {
+// With deferred, also the E_loc mean and variance formed on the auxiliary
+// stream (the main stream has waited for it).
+static int minsr_dual_finish_info(int rc, jv_solve_info* info, int64_t p, double rcond, bool deferred,
+                                  hipStream_t st) {
   if (rc || !info) return rc;
   int* rank = (int*)workspace(64, 11);
   if (!rank) return fail(JV_E_NOMEM, "jv_minsr_solve_dual: workspace allocation failed");
   int hr[4] = {0, 0, 0, 0};
+  double em[3] = {0.0, 0.0, 0.0};
   JV_HIP(hipMemcpyAsync(hr, rank, sizeof(hr), hipMemcpyDeviceToHost, st));
+  if (deferred) {
+    const double* emean = (const double*)workspace(sizeof(double) * 4, 50);
+    if (!emean) return fail(JV_E_NOMEM, "jv_minsr_solve_dual: workspace allocation failed");
+    JV_HIP(hipMemcpyAsync(em, emean, sizeof(em), hipMemcpyDeviceToHost, st));
+  }
   JV_HIP(hipStreamSynchronize(st));
+  info->e_mean[0] = em[0];
+  info->e_mean[1] = em[1];
+  info->e_var = em[2];
   if (hr[1] == 1 && hr[3] != 0)
     return fail(JV_E_SOLVER, "jv_minsr_solve_dual: Cholesky factorisation failed (info " + std::to_string(hr[3]) + ")");
   info->rank = hr[0];

@@ -141,7 +141,7 @@ class MinSR:
             update = self._solve_deferred(Eraw, ready, sampleGradients, w)
             main.wait_stream(side)
             Eraw.record_stream(main)
-            Eloc = SampledObs(Eraw.reshape(1, -1), p)
+            Eloc = None  # mean and variance come with the solve info
         else:
             timer.start("compute Eloc")
             Eloc = hamiltonian.get_O_loc(sampleConfigs, psi, sampleLogPsi, t)
@@ -155,7 +155,11 @@ class MinSR:
             timer.stop("solve MinSR eqn.")
         psi.set_parameters(tmpParameters)
         if rhsArgs.get("intStep", None) == 0:
-            self.ElocMean0 = complex(Eloc.mean()[0].item())
-            self.ElocVar0 = float(Eloc.var()[0].item())
+            if Eloc is None:
+                self.ElocMean0 = complex(self.last_info.e_mean[0], self.last_info.e_mean[1])
+                self.ElocVar0 = float(self.last_info.e_var)
+            else:
+                self.ElocMean0 = complex(Eloc.mean()[0].item())
+                self.ElocVar0 = float(Eloc.var()[0].item())
             self.metaData = {}
         return update
