@@ -129,9 +129,9 @@ typedef struct jv_solve_info {
   double residual;           /* tdvp_residual (tdvp.py:209) */
   double pinv_cutoff;        /* max(cutoff, pinvCutoff) (tdvp.py:213) */
   int32_t iterations;        /* TDVP: regulariser iterations taken (tdvp.py:198);
-                                MinSR dual, real: 1 when no eigenvalue lies at or
-                                below the cutoff and the update came from a
-                                Cholesky solve, 0 from the eigendecomposition */
+                                MinSR: 1 when no eigenvalue lies at or below the
+                                cutoff and the update came from a Cholesky
+                                solve, 0 from the eigendecomposition */
   int32_t rank;              /* MinSR: eigenvalues kept by the pseudo-inverse */
   double e_var;              /* TDVP and MinSR deferred dual: sum_n |e_n|^2 of the centred,
                                 weighted E_loc = Var E_loc (stats.py:248-252) */
@@ -319,10 +319,15 @@ int jv_stedc(int64_t n, double* d, const double* e, double* z, void* stream);
  * layout handled by the caller.  Real-parameter nets (is_complex 0): D real
  * [n][p], e complex [n]; T = D D^T + shift I (the imaginary stacked half of
  * minsr.py:68-78 is identically zero and drops out exactly), update real [p].
- * pinv keeps |lambda| > rcond * max|lambda|.  info (host) may be NULL. */
+ * pinv keeps |lambda| > rcond * max|lambda|.  try_cholesky != 0: when a
+ * Cholesky factorisation of T - c I (complex T: of its real 2n x 2n form
+ * [[Re T, -Im T], [Im T, Re T]], same eigenvalues) with c = ||T||_F (rcond +
+ * 8 n eps) succeeds, no eigenvalue is cut and x = T^-1 e comes from L^-1
+ * (info->iterations = 1); otherwise the eigendecomposition runs.  info (host)
+ * may be NULL. */
 int jv_minsr_solve(const double* data, const double* e, int64_t n, int64_t p,
-                   int32_t is_complex, double rcond, double diag_shift, double* update,
-                   jv_solve_info* info, void* stream);
+                   int32_t is_complex, double rcond, double diag_shift, int32_t try_cholesky,
+                   double* update, jv_solve_info* info, void* stream);
 
 /* Same update as jv_minsr_solve from the P x P Gram S = D^dagger D (+ shift I)
  * instead of the N x N tangent kernel: the nonzero spectra of D D^dagger and

@@ -95,3 +95,53 @@ def test_cut_spectrum_uses_eigendecomposition():
     ref, rank = _reference(D, e, 1e-12)
     assert info.iterations == 1 and rank == p
     assert _rel(u, ref) < 1e-5  # condition number 1e10: both accurate to ~ cond * eps
+
+
+def _solve_t(D, e, rcond, shift=0.0):
+    n, p = D.shape
+    cpx = np.iscomplexobj(D)
+    Dd = torch.as_tensor(D, device="cuda").contiguous()
+    ed = torch.as_tensor(e.astype(np.complex128), device="cuda").contiguous()
+    upd = torch.empty(p, dtype=torch.complex128 if cpx else torch.float64, device="cuda")
+    info = jv._lib.JvSolveInfo()
+    jv._lib.call("jv_minsr_solve", jv._lib.ptr(Dd), jv._lib.ptr(ed), n, p, int(cpx), float(rcond), float(shift), 1,
+                 jv._lib.ptr(upd), ctypes.byref(info), jv._lib.stream_ptr())
+    return upd.cpu().numpy(), info
+
+
+def _reference_t(D, e, rcond, shift=0.0):
+    """minsr.py:53-80 in the T form: -D^dagger pinv(T) e, T = D D^dagger (+ shift
+    for real data, where only Re e couples)."""
+    cpx = np.iscomplexobj(D)
+    T = D @ D.conj().T + (0 if cpx else shift) * np.eye(D.shape[0])
+    w, V = np.linalg.eigh(T)
+    keep = np.abs(w) > rcond * np.abs(w).max()
+    inv = np.where(keep, 1.0 / np.where(keep, w, 1.0), 0.0)
+    rhs = e if cpx else e.real
+    return -D.conj().T @ ((V * inv) @ (V.conj().T @ rhs)), int(keep.sum())
+
+
+@pytest.mark.parametrize("cpx,n,p", [(False, 300, 700), (False, 64, 65), (True, 200, 500), (True, 1100, 2000)])
+def test_t_form_takes_cholesky(cpx, n, p):
+    """The reference's own MinSR form with fewer samples than parameters: T is
+    full rank, the route (complex T through its real 2n x 2n form) is taken."""
+    rng = np.random.default_rng(n + p)
+    D = rng.normal(size=(n, p)) + (1j * rng.normal(size=(n, p)) if cpx else 0)
+    D /= np.sqrt(p)
+    e = rng.normal(size=n) + 1j * rng.normal(size=n)
+    print("T-form cholesky route", cpx, n, p, flush=True)
+    u, info = _solve_t(D, e, 1e-10)
+    ref, rank = _reference_t(D, e, 1e-10)
+    assert info.iterations == 1 and info.rank == n == rank
+    assert _rel(u, ref) < 1e-10
+
+
+def test_t_form_singular_uses_eigendecomposition():
+    rng = np.random.default_rng(5)
+    n, p = 400, 150  # T = D D^T has rank 150
+    D = rng.normal(size=(n, p)) / np.sqrt(p)
+    e = rng.normal(size=n) + 0j
+    u, info = _solve_t(D, e, 1e-10)
+    ref, rank = _reference_t(D, e, 1e-10)
+    assert info.iterations == 0 and info.rank == rank == p
+    assert _rel(u, ref) < 1e-8

@@ -94,7 +94,7 @@ _SIGS = {
     "jv_gemv": [_i32, _vp, _vp, _i64, _i64, _i32, _i32, _dbl, _vp, _vp],
     "jv_eigh": [_i64, _i32, _vp, _vp, _vp],
     "jv_stedc": [_i64, _vp, _vp, _vp, _vp],
-    "jv_minsr_solve": [_vp, _vp, _i64, _i64, _i32, _dbl, _dbl, _vp, ctypes.POINTER(JvSolveInfo), _vp],
+    "jv_minsr_solve": [_vp, _vp, _i64, _i64, _i32, _dbl, _dbl, _i32, _vp, ctypes.POINTER(JvSolveInfo), _vp],
     "jv_minsr_solve_dual": [_vp, _vp, _i64, _i64, _i32, _dbl, _dbl, _i32, _vp, ctypes.POINTER(JvSolveInfo), _vp],
     "jv_minsr_solve_dual_deferred": [_vp, _vp, _vp, _vp, _i64, _i64, _i32, _dbl, _dbl, _i32, _vp,
                                      ctypes.POINTER(JvSolveInfo),

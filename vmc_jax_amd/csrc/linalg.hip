@@ -638,8 +638,47 @@ int jv_eigh(int64_t n, int32_t is_complex, double* A, double* w, void* stream) {
   return JV_OK;
 }
 
+}  // extern "C"
+namespace jv {
+static bool minsr_cholesky_enabled();
+static int chol_route(const double* S, int64_t p, double rcond, int* flags, bool* ok, double** L, hipStream_t st);
+int chol_inverse_lower(const double* L, int p, double* W, hipStream_t st);
+static int minsr_dual_finish_info(int rc, jv_solve_info* info, int64_t p, double rcond, bool deferred,
+                                  hipStream_t st);
+
+// R = [[Re T, -Im T], [Im T, Re T]] (2n x 2n real symmetric) of the complex
+// Hermitian T (n x n, interleaved): same eigenvalues as T, each twice.
+__global__ void realify_hermitian(const double* __restrict__ T, int64_t n, double* __restrict__ R) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t m = 2 * n;
+  if (t >= m * m) return;
+  const int64_t i = t / m, j = t % m;
+  const int64_t a = i % n, c = j % n;
+  const double re = T[2 * (a * n + c)], im = T[2 * (a * n + c) + 1];
+  R[t] = (i < n) == (j < n) ? re : (i < n ? -im : im);
+}
+
+// b = [Re e; Im e] (cpx) or Re e
+__global__ void split_complex(const double* __restrict__ e, int64_t n, int cpx, double* __restrict__ b) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  b[t] = e[2 * t];
+  if (cpx) b[n + t] = e[2 * t + 1];
+}
+
+// x (interleaved complex) = xr + i xi from [xr; xi]
+__global__ void join_complex(const double* __restrict__ y, int64_t n, double* __restrict__ x) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  x[2 * t] = y[t];
+  x[2 * t + 1] = y[n + t];
+}
+}  // namespace jv
+
+extern "C" {
+
 int jv_minsr_solve(const double* data, const double* e, int64_t n, int64_t p, int32_t is_complex, double rcond,
-                   double diag_shift, double* update, jv_solve_info* info, void* stream) {
+                   double diag_shift, int32_t try_cholesky, double* update, jv_solve_info* info, void* stream) {
   if (!data || !e || !update || n < 1 || p < 1) return fail(JV_E_ARG, "jv_minsr_solve: bad arguments");
   hipStream_t st = (hipStream_t)stream;
   const int cpx = is_complex;
@@ -656,6 +695,48 @@ int jv_minsr_solve(const double* data, const double* e, int64_t n, int64_t p, in
   if (!cpx && diag_shift != 0.0) {
     hipLaunchKernelGGL(add_diag, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, T, n, diag_shift, 0);
     JV_LAUNCH_CHECK("add_diag");
+  }
+  hipLaunchKernelGGL(set_ints, dim3(1), dim3(64), 0, st, rank, 0, 0);  // rank[1]: route taken
+  JV_LAUNCH_CHECK("set_ints");
+  if (try_cholesky && minsr_cholesky_enabled()) {
+    // Cholesky route (as in jv_minsr_solve_dual): T, or its real form R for
+    // complex T, has no eigenvalue at or below the cutoff; then
+    // x = W^T W b with W = L^-1 (two GEMVs, no n^3 product)
+    const int64_t m = cpx ? 2 * n : n;
+    double* R = T;
+    if (cpx) {
+      R = (double*)workspace(sizeof(double) * (size_t)m * m, 99);
+      if (!R) return fail(JV_E_NOMEM, "jv_minsr_solve: workspace allocation failed");
+      hipLaunchKernelGGL(realify_hermitian, dim3((unsigned)((m * m + 255) / 256)), dim3(256), 0, st, T, n, R);
+      JV_LAUNCH_CHECK("realify_hermitian");
+    }
+    bool ok = false;
+    double* L = nullptr;
+    if ((rc = chol_route(R, m, rcond, rank, &ok, &L, st))) return rc;
+    if (ok) {
+      double* W = (double*)workspace(sizeof(double) * (size_t)m * m, 95);  // R - c I is no longer needed
+      double* b = (double*)workspace(sizeof(double) * 2 * (size_t)m, 100);
+      if (!W || !b) return fail(JV_E_NOMEM, "jv_minsr_solve: workspace allocation failed");
+      {
+        ScopedTimer tm("chol_inverse", st);
+        if ((rc = chol_inverse_lower(L, (int)m, W, st))) return rc;
+      }
+      hipLaunchKernelGGL(split_complex, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, e, n, cpx, b);
+      JV_LAUNCH_CHECK("split_complex");
+      if ((rc = gemv(false, W, b, m, m, 0, 0, 1.0, b + m, st, 12))) return rc;  // W b
+      if ((rc = gemv(true, W, b + m, m, m, 0, 0, 1.0, b, st, 12))) return rc;   // W^T W b
+      const double* xs = b;
+      if (cpx) {
+        hipLaunchKernelGGL(join_complex, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, b, n, x);
+        JV_LAUNCH_CHECK("join_complex");
+        xs = x;
+      }
+      // update = -D^dagger x
+      if ((rc = gemv(true, data, xs, n, p, cpx, 1, -1.0, update, st, 12))) return rc;
+      hipLaunchKernelGGL(set_ints, dim3(1), dim3(64), 0, st, rank, (int)n, 1);
+      JV_LAUNCH_CHECK("set_ints");
+      return minsr_dual_finish_info(JV_OK, info, n, rcond, false, st);
+    }
   }
   rc = eigh_raw(n, cpx, T, ev, st);
   if (rc) return rc;
@@ -675,16 +756,7 @@ int jv_minsr_solve(const double* data, const double* e, int64_t n, int64_t p, in
   // update = -D^dagger x (holomorphic: the caller applies the layout factor)
   rc = gemv(true, data, x, n, p, cpx, 1, -1.0, update, st, 12);
   if (rc) return rc;
-  if (info) {
-    int hr = 0;
-    JV_HIP(hipMemcpyAsync(&hr, rank, sizeof(int), hipMemcpyDeviceToHost, st));
-    JV_HIP(hipStreamSynchronize(st));
-    info->rank = hr;
-    info->iterations = 0;
-    info->residual = 0.0;
-    info->pinv_cutoff = rcond;
-  }
-  return JV_OK;
+  return minsr_dual_finish_info(JV_OK, info, n, rcond, false, st);
 }
 
 }  // extern "C"
@@ -1297,13 +1369,21 @@ static int chol_route(const double* S, int64_t p, double rcond, int* flags, bool
     hipLaunchKernelGGL(shifted_copy, dim3((unsigned)((pp + 255) / 256)), dim3(256), 0, st, S, p, part + 256, rcond, Sc,
                        S2);
     JV_LAUNCH_CHECK("shifted_copy");
-    int rc = chol_factor2(Sc, S2, (int)p, flags + 2, st);
+    // small matrices: both factorisations in the same latency-bound launches;
+    // large ones: the proof first, so that a refused attempt costs one
+    const bool together = p < 2048;
+    int rc = chol_factor2(Sc, together ? S2 : nullptr, (int)p, flags + 2, st);
     if (rc) return rc;
   }
   int hinfo = -1;
   JV_HIP(hipMemcpyAsync(&hinfo, flags + 2, sizeof(int), hipMemcpyDeviceToHost, st));
   JV_HIP(hipStreamSynchronize(st));
   if (hinfo != 0) return JV_OK;  // an eigenvalue at or below the cutoff is possible
+  if (p >= 2048) {
+    ScopedTimer tm("cholesky", st);
+    int rc = chol_factor2(S2, nullptr, (int)p, flags + 3, st);
+    if (rc) return rc;
+  }
   *L = S2;
   *ok = true;
   return JV_OK;

@@ -37,6 +37,10 @@ class MinSR:
     # changes slowly from step to step, and a failed attempt costs a
     # factorisation.  The update is the same either way.
     CHOLESKY_RETRY = 8
+    # The holomorphic T form (complex T = G G^dagger, N < P) tries the route
+    # only when asked: with RBM gradients its T is usually rank deficient, and
+    # a refused attempt costs a factorisation of the 2N x 2N real form.
+    try_cholesky_holomorphic = False
 
     def _cholesky_hint(self):
         if self._chol_skip > 0:
@@ -79,9 +83,9 @@ class MinSR:
             _lib.call("jv_minsr_solve_dual", _lib.ptr(D), _lib.ptr(e), n, p, int(cpx), float(self.pinvTol),
                       float(self.diagonalShift), hint, _lib.ptr(upd), ctypes.byref(info), _lib.stream_ptr())
         else:
-            hint = 0
+            hint = self._cholesky_hint() if (not cpx or self.try_cholesky_holomorphic) else 0
             _lib.call("jv_minsr_solve", _lib.ptr(D), _lib.ptr(e), n, p, int(cpx), float(self.pinvTol),
-                      float(self.diagonalShift), _lib.ptr(upd), ctypes.byref(info), _lib.stream_ptr())
+                      float(self.diagonalShift), hint, _lib.ptr(upd), ctypes.byref(info), _lib.stream_ptr())
         self.last_info = info
         self._note_route(hint)
         if cpx:
