@@ -347,11 +347,13 @@ int jv_minsr_solve_dual(const double* data, const double* e, int64_t n, int64_t 
 
 /* jv_minsr_solve_dual with the local estimator still in flight: e_raw is the
  * uncentred E_loc (complex [n]) with the sample weights w [n], complete once the
- * event e_ready (a hipEvent_t; NULL = ready in stream order) has fired.  The
- * centring of e_raw (stats.py:197-205), b = D^dagger e and, on the real
- * tridiagonal path, Q^T b run on the library's auxiliary stream, so the
- * producer of e_raw overlaps the Gram and the Householder reduction, and Q^T b
- * overlaps the tridiagonal eigensolver.  Same update as jv_minsr_solve_dual on
+ * event e_ready (a hipEvent_t; NULL = ready in stream order) has fired, so
+ * the producer of e_raw overlaps the Gram and its factorisation or reduction.
+ * On the Cholesky route the centring of e_raw (stats.py:197-205) and
+ * b = D^dagger e follow e_ready on the calling stream; on the real tridiagonal
+ * path they run with Q^T b on the library's auxiliary stream, overlapping the
+ * tridiagonal eigensolver.  info->e_mean / e_var return the statistics of
+ * E_loc.  Same update as jv_minsr_solve_dual on
  * the centred data.  No separate reference function (minsr.py:82-165 computes
  * E_loc before the solve; the order does not change the result). */
 int jv_minsr_solve_dual_deferred(const double* data, const double* e_raw, const double* w, void* e_ready,

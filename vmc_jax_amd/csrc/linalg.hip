@@ -1441,16 +1441,6 @@ static int minsr_dual(const double* data, const double* e, const double* e_raw, 
   hipLaunchKernelGGL(set_ints, dim3(1), dim3(64), 0, st, rank, 0, 0);  // rank[1]: route taken
   JV_LAUNCH_CHECK("set_ints");
   if (!cpx && try_chol && minsr_cholesky_enabled()) {
-    hipStream_t s2 = nullptr;
-    hipEvent_t ev_d = nullptr, ev_b = nullptr;
-    if (deferred) {
-      // b = D^T e on the auxiliary stream once E_loc is ready, beside the factorisations
-      if (!aux_stream(&s2, &ev_d, &ev_b)) return fail(JV_E_HIP, "jv_minsr_solve_dual: auxiliary stream creation failed");
-      JV_HIP(hipEventRecord(ev_d, st));
-      JV_HIP(hipStreamWaitEvent(s2, ev_d, 0));
-      if ((rc = rhs_b(s2))) return rc;
-      JV_HIP(hipEventRecord(ev_b, s2));
-    }
     bool ok = false;
     double* L = nullptr;
     if ((rc = chol_route(S, p, rcond, rank, &ok, &L, st))) return rc;
@@ -1465,8 +1455,9 @@ static int minsr_dual(const double* data, const double* e, const double* e_raw, 
         GemmArgs g{p, p, p, Wi, 1, p, 0, Wi, p, 1, 0, S, p, 0, 1, 1.0};
         if ((rc = gemm(g, st))) return rc;
       }
-      if (deferred) JV_HIP(hipStreamWaitEvent(st, ev_b, 0));
-      else if ((rc = rhs_b(st))) return rc;
+      // b = D^T e on this stream (deferred: after E_loc's event; the
+      // factorisation and inverse are done long before E_loc in practice)
+      if ((rc = rhs_b(st))) return rc;
       if ((rc = gemv(false, S, b, p, p, 0, 0, 1.0, x, st, 12))) return rc;
       hipLaunchKernelGGL(set_ints, dim3(1), dim3(64), 0, st, rank, (int)p, 1);
       JV_LAUNCH_CHECK("set_ints");
@@ -1474,7 +1465,7 @@ static int minsr_dual(const double* data, const double* e, const double* e_raw, 
       JV_LAUNCH_CHECK("neg_copy");
       return JV_OK;
     }
-    // otherwise the eigendecomposition below (the deferred path forms b again)
+    // otherwise the eigendecomposition below
   }
   if (deferred && !cpx && trd_supported(p)) {
     hipStream_t s2;
