@@ -265,6 +265,12 @@ struct TrdcArgs {
 __device__ inline cd cmul(cd a, cd b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
 __device__ inline cd cmulc(cd a, cd b) { return {a.re * b.re + a.im * b.im, a.im * b.re - a.re * b.im}; }  // a conj(b)
 
+// R: complex n-vector elements per thread (n <= TRD_T * R).  VG = false keeps
+// v_{k-1}, w_{k-1} and the column in LDS next to v_k (64 n bytes); VG = true
+// (large n) keeps only v_k in LDS (16 n bytes: the column is built in its
+// place) and the column pass rebuilds v_{k-1} (row k-1 of M) and
+// w_{k-1} = tau_{k-1} y_{k-1} - alpha' v_{k-1} from global memory.
+template <int R, bool VG>
 __global__ void __launch_bounds__(TRD_T) trd_step_c(TrdcArgs a, int k) {
   extern __shared__ double sh[];
   const int n = a.n, G = a.G, tid = threadIdx.x, g = blockIdx.x;
@@ -278,25 +284,29 @@ __global__ void __launch_bounds__(TRD_T) trd_step_c(TrdcArgs a, int k) {
   const int w = tid / WAVE, lane = tid % WAVE;
   const int jw = j0 + w * G;
   cd* vk = (cd*)sh;
-  cd* vp = vk + n;
-  cd* wp = vp + n;
-  cd* x = wp + n;
-  double* red = (double*)(x + n);  // 2 * TRD_W
+  cd* vp = VG ? nullptr : vk + n;
+  cd* wp = VG ? nullptr : vk + 2 * n;
+  cd* x = VG ? vk : vk + 3 * n;
+  double* red = (double*)(VG ? vk + n : vk + 4 * n);  // 2 * TRD_W
   const cd* Mc = (const cd*)a.M;
+  cd tp = {0.0, 0.0};  // tau_{k-1}
+  double ap = 0.0;     // alpha'_{k-1}
   if (k == 0) {
-    cd xr[TRDC_R];
+    cd xr[R];
 #pragma unroll
-    for (int r = 0; r < TRDC_R; ++r) {
+    for (int r = 0; r < R; ++r) {
       const int i = tid + r * TRD_T;
       xr[r] = i < n ? Mc[i] : cd{0.0, 0.0};
     }
 #pragma unroll
-    for (int r = 0; r < TRDC_R; ++r) {
+    for (int r = 0; r < R; ++r) {
       const int i = tid + r * TRD_T;
       if (i < n) {
         x[i] = cconj(xr[r]);
-        vp[i] = {0.0, 0.0};
-        wp[i] = {0.0, 0.0};
+        if (!VG) {
+          vp[i] = {0.0, 0.0};
+          wp[i] = {0.0, 0.0};
+        }
       }
     }
   } else {
@@ -304,9 +314,9 @@ __global__ void __launch_bounds__(TRD_T) trd_step_c(TrdcArgs a, int k) {
     const cd* yv = (const cd*)a.y + (size_t)pb * n;
     const cd* uv = (const cd*)a.u + (size_t)pb * n;
     const cd* vrow = (const cd*)(k == 1 ? a.v0 : a.M + 2 * (size_t)(k - 1) * n);
-    cd yr[TRDC_R], ur[TRDC_R], vr[TRDC_R];
+    cd yr[R], ur[R], vr[R];
 #pragma unroll
-    for (int r = 0; r < TRDC_R; ++r) {
+    for (int r = 0; r < R; ++r) {
       const int i = tid + r * TRD_T;
       const bool in = i >= k && i < n;
       yr[r] = in ? yv[i] : cd{0.0, 0.0};
@@ -315,19 +325,21 @@ __global__ void __launch_bounds__(TRD_T) trd_step_c(TrdcArgs a, int k) {
     }
     double pdv = 0.0;
     for (int q = tid; q < G; q += TRD_T) pdv += a.pd[pb * G + q];
-    const cd tp = {a.tau[2 * (k - 1)], a.tau[2 * (k - 1) + 1]};
+    tp = {a.tau[2 * (k - 1)], a.tau[2 * (k - 1) + 1]};
     const cd yk = yv[k];
     const double sdot = trd_block_sum(pdv, red);
-    const double ap = 0.5 * cabs2(tp) * sdot;
+    ap = 0.5 * cabs2(tp) * sdot;
     const cd ty = cmul(tp, yk);
     const cd gam = {2.0 * ap - ty.re, ty.im};  // 2 alpha' - conj(tau y_k)
 #pragma unroll
-    for (int r = 0; r < TRDC_R; ++r) {
+    for (int r = 0; r < R; ++r) {
       const int i = tid + r * TRD_T;
       if (i < n) {
-        vp[i] = vr[r];
-        const cd t = cmul(tp, yr[r]);
-        wp[i] = {t.re - ap * vr[r].re, t.im - ap * vr[r].im};
+        if (!VG) {
+          vp[i] = vr[r];
+          const cd t = cmul(tp, yr[r]);
+          wp[i] = {t.re - ap * vr[r].re, t.im - ap * vr[r].im};
+        }
         const cd gv = cmul(gam, vr[r]);
         x[i] = {ur[r].re + gv.re, ur[r].im + gv.im};
       }
@@ -335,15 +347,21 @@ __global__ void __launch_bounds__(TRD_T) trd_step_c(TrdcArgs a, int k) {
   }
   __syncthreads();
   cd tk = {0.0, 0.0};
+  const double xkk = x[k].re;
+  double sigma = 0.0;
+  cd alpha = {0.0, 0.0};
   if (k <= n - 2) {
     double part = 0.0;
 #pragma unroll
-    for (int r = 0; r < TRDC_R; ++r) {
+    for (int r = 0; r < R; ++r) {
       const int i = tid + r * TRD_T;
       if (i >= k + 2 && i < n) part += cabs2(x[i]);
     }
-    const double sigma = trd_block_sum(part, red + TRD_W);
-    const cd alpha = x[k + 1];
+    sigma = trd_block_sum(part, red + TRD_W);
+    alpha = x[k + 1];
+  }
+  if (VG) __syncthreads();  // every read of x above precedes v_k overwriting it
+  if (k <= n - 2) {
     double beta = alpha.re;
     cd scale = {0.0, 0.0};
     if (sigma != 0.0 || alpha.im != 0.0) {
@@ -363,7 +381,7 @@ __global__ void __launch_bounds__(TRD_T) trd_step_c(TrdcArgs a, int k) {
   } else {
     for (int i = tid; i < n; i += TRD_T) vk[i] = {0.0, 0.0};
   }
-  if (g == 0 && tid == 0) a.d[k] = x[k].re;
+  if (g == 0 && tid == 0) a.d[k] = xkk;
   __syncthreads();
   if (k > n - 2) return;
   if (g == 0) {
@@ -378,17 +396,37 @@ __global__ void __launch_bounds__(TRD_T) trd_step_c(TrdcArgs a, int k) {
   }
   const int cb = k & 1;
   double ppart = 0.0;
+  // VG: v_{k-1} is row k-1 of M (reflector 0 is still in v0 at k = 1; no row
+  // read here is written in this step) and y_{k-1} is the other y buffer
+  const cd* vg = (const cd*)(k == 1 ? a.v0 : a.M + 2 * (size_t)(k >= 1 ? k - 1 : 0) * n);
+  const cd* yg = (const cd*)a.y + (size_t)((k + 1) & 1) * n;
+  auto w_of = [&](cd v, cd y) {
+    const cd t = cmul(tp, y);
+    return cd{t.re - ap * v.re, t.im - ap * v.im};
+  };
   for (int j = jw; j < n; j += TRD_W * G) {
     cd* row = (cd*)(a.M + 2 * (size_t)j * n);
-    const cd vj = vp[j], wj = wp[j];
+    cd vj, wj;
+    if (VG) {
+      vj = k >= 1 ? vg[j] : cd{0.0, 0.0};
+      wj = k >= 1 ? w_of(vj, yg[j]) : cd{0.0, 0.0};
+    } else {
+      vj = vp[j];
+      wj = wp[j];
+    }
     double accr = 0.0, acci = 0.0;
     cd cj = {0.0, 0.0};
     for (int base = k + 1; base < n; base += TRDC_CP * WAVE) {
-      cd cv[TRDC_CP];
+      cd cv[TRDC_CP], gv[VG ? TRDC_CP : 1], gy[VG ? TRDC_CP : 1];
 #pragma unroll
       for (int t = 0; t < TRDC_CP; ++t) {
         const int i = base + t * WAVE + lane;
         cv[t] = i < n ? row[i] : cd{0.0, 0.0};
+        if (VG && k >= 1) {
+          const int ic = min(i, n - 1);
+          gv[VG ? t : 0] = vg[ic];
+          gy[VG ? t : 0] = yg[ic];
+        }
       }
 #pragma unroll
       for (int t = 0; t < TRDC_CP; ++t) {
@@ -396,7 +434,15 @@ __global__ void __launch_bounds__(TRD_T) trd_step_c(TrdcArgs a, int k) {
         if (i < n) {
           cd v = cv[t];
           if (k >= 1) {
-            const cd p1 = cmulc(vj, wp[i]), p2 = cmulc(wj, vp[i]);
+            cd vpi, wpi;
+            if (VG) {
+              vpi = gv[VG ? t : 0];
+              wpi = w_of(vpi, gy[VG ? t : 0]);
+            } else {
+              vpi = vp[i];
+              wpi = wp[i];
+            }
+            const cd p1 = cmulc(vj, wpi), p2 = cmulc(wj, vpi);
             v = {v.re - p1.re - p2.re, v.im - p1.im - p2.im};
           }
           row[i] = v;
@@ -622,8 +668,15 @@ int trd_min_n() {
 bool trd_supported(int64_t n) { return n >= trd_min_n() && n >= 3 && n <= TRD_MAX_N; }
 
 // Complex: 4 complex n-vectors in LDS and TRDC_R elements per thread.
-constexpr int TRDC_MAX_N = 2496;
-bool trdc_supported(int64_t n) { return n >= trd_min_n() && n >= 3 && n <= TRDC_MAX_N && n <= TRD_T * TRDC_R; }
+constexpr int TRDC_MAX_N = 2496;   // four complex n-vectors in LDS
+constexpr int TRDC_R_BIG = 7;      // large-n variant: v_k alone in LDS
+// Up to the largest tridiagonal of the own divide and conquer.  At n = 2600
+// the whole eigendecomposition takes 76 ms against rocSOLVER zheevd's 98 ms;
+// at n = 4096 the 4096 launches (37 us each) make it 229 ms against 212 ms, so
+// larger matrices stay with zheevd.
+constexpr int TRDC_BIG_MAX_N = 3276;
+static_assert(TRDC_BIG_MAX_N <= TRD_T * TRDC_R_BIG, "n-vector elements per thread");
+bool trdc_supported(int64_t n) { return n >= trd_min_n() && n >= 3 && n <= TRDC_BIG_MAX_N; }
 
 // Eigen-solve of the real symmetric n x n matrix S (destroyed) restricted to
 // what the pseudo-inverse needs: ev = eigenvalues (ascending), c (length n,
@@ -747,14 +800,21 @@ int trdc_eigh_full(double* S, int n, double* ev, hipStream_t st) {
   a.pd = work + 13 * (size_t)n;
   a.n = n;
   a.G = G;
-  const size_t lds = sizeof(double) * (8 * (size_t)n + 2 * TRD_W);
+  const bool big = n > TRDC_MAX_N || n > TRD_T * TRDC_R;
+  const void* fn = big ? (const void*)trd_step_c<TRDC_R_BIG, true> : (const void*)trd_step_c<TRDC_R, false>;
+  const size_t lds = sizeof(double) * ((big ? 2 : 8) * (size_t)n + 2 * TRD_W);
   if (lds > 64 * 1024) {
-    hipError_t e = hipFuncSetAttribute((const void*)trd_step_c, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute(trd_step_c)");
   }
   {
     ScopedTimer tm("tridiag", st);
-    for (int k = 0; k < n; ++k) hipLaunchKernelGGL(trd_step_c, dim3((unsigned)G), dim3(TRD_T), lds, st, a, k);
+    for (int k = 0; k < n; ++k) {
+      if (big)
+        hipLaunchKernelGGL((trd_step_c<TRDC_R_BIG, true>), dim3((unsigned)G), dim3(TRD_T), lds, st, a, k);
+      else
+        hipLaunchKernelGGL((trd_step_c<TRDC_R, false>), dim3((unsigned)G), dim3(TRD_T), lds, st, a, k);
+    }
     JV_LAUNCH_CHECK("trd_step_c");
   }
   int rc = stedc(n, ev, a.e, Z, (rocblas_int*)(work + 13 * (size_t)n + 2 * G + 8), st);
