@@ -7,6 +7,10 @@
 // one launch subtracts the panel's outer product from the trailing lower
 // triangle with FP64 MFMA tiles.  Two matrices go through the same launches
 // (blockIdx.y), which costs next to nothing for these latency-bound sizes.
+// The other workgroups of the panel launch read the diagonal block's input
+// values, so workgroup 0 leaves the factored block in a scratch buffer and the
+// trailing update (the next launch, which never reads that block) copies it
+// into place; a panel launch of one workgroup writes it directly.
 // A pivot that is not positive (or NaN) records its 1-based index in info;
 // the remaining arithmetic then runs on NaN without any data-dependent
 // addressing.  Only the lower triangle is read; the strict upper triangle
@@ -34,7 +38,8 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 // q of a row holding columns q, q + 4, ... in registers: per column the owner
 // finishes x_j, the row's four threads exchange it by shuffle and update
 // their later columns.
-__global__ void __launch_bounds__(256) chol_panel(double* A0, double* A1, int p, int k0, int* info) {
+__global__ void __launch_bounds__(256) chol_panel(double* A0, double* A1, int p, int k0, int* info,
+                                                  double* __restrict__ dscr) {
   double* A = blockIdx.y ? A1 : A0;
   __shared__ double Ld[CB][CB + 1];
   const int tid = threadIdx.x;
@@ -101,9 +106,12 @@ __global__ void __launch_bounds__(256) chol_panel(double* A0, double* A1, int p,
   }
   __syncthreads();
   if (blockIdx.x == 0) {
+    const bool alone = gridDim.x == 1;
+    double* dst = alone ? A + (size_t)k0 * p + k0 : dscr + (size_t)blockIdx.y * CB * CB;
+    const int ld = alone ? p : CB;
     for (int e = tid; e < CB * CB; e += 256) {
       const int r = e / CB, c = e % CB;
-      if (r < b && c <= r) A[(size_t)(k0 + r) * p + k0 + c] = Ld[r][c];
+      if (r < b && c <= r) dst[(size_t)r * ld + c] = Ld[r][c];
     }
     return;
   }
@@ -138,8 +146,16 @@ __global__ void __launch_bounds__(256) chol_panel(double* A0, double* A1, int p,
 // 32 x 32, FP64 MFMA 16x16x4 (A operand A[row = lane % 16][k = lane / 16],
 // B operand B[k = lane / 16][col = lane % 16], result element r of lane l at
 // C[4 r + l / 16][l % 16]).
-__global__ void __launch_bounds__(256) chol_update(double* A0, double* A1, int p, int k0) {
+__global__ void __launch_bounds__(256) chol_update(double* A0, double* A1, int p, int k0,
+                                                   const double* __restrict__ dscr) {
   double* A = blockIdx.y ? A1 : A0;
+  if (blockIdx.x == 0) {  // the factored diagonal block of panel k0 (a full CB x CB block here)
+    const double* src = dscr + (size_t)blockIdx.y * CB * CB;
+    for (int e = threadIdx.x; e < CB * CB; e += 256) {
+      const int r = e / CB, c = e % CB;
+      if (c <= r) A[(size_t)(k0 + r) * p + k0 + c] = src[e];
+    }
+  }
   const int k1 = k0 + CB;
   // linear lower-tile index -> (ti, tj), ti >= tj
   int bidx = blockIdx.x, ti = 0;
@@ -307,18 +323,20 @@ __global__ void __launch_bounds__(256) inv_update(const double* __restrict__ L, 
 // the first non-positive pivot (1-based) or 0.
 int chol_factor2(double* A0, double* A1, int p, int* info, hipStream_t st) {
   const unsigned nmat = A1 ? 2 : 1;
+  double* dscr = (double*)workspace(sizeof(double) * 2 * CB * CB, 103);
+  if (!dscr) return fail(JV_E_NOMEM, "cholesky: workspace allocation failed");
   hipLaunchKernelGGL(chol_zero_info, dim3(1), dim3(64), 0, st, info);
   JV_LAUNCH_CHECK("chol_zero_info");
   for (int k0 = 0; k0 < p; k0 += CB) {
     const int b = std::min(CB, p - k0);
     const int below = p - k0 - b;
     const unsigned gx = 1 + (unsigned)((below + CB - 1) / CB);
-    hipLaunchKernelGGL(chol_panel, dim3(gx, nmat), dim3(256), 0, st, A0, A1 ? A1 : A0, p, k0, info);
+    hipLaunchKernelGGL(chol_panel, dim3(gx, nmat), dim3(256), 0, st, A0, A1 ? A1 : A0, p, k0, info, dscr);
     JV_LAUNCH_CHECK("chol_panel");
     if (below > 0) {
       const int T = (below + CB - 1) / CB;
       hipLaunchKernelGGL(chol_update, dim3((unsigned)(T * (T + 1) / 2), nmat), dim3(256), 0, st, A0, A1 ? A1 : A0, p,
-                         k0);
+                         k0, dscr);
       JV_LAUNCH_CHECK("chol_update");
     }
   }
