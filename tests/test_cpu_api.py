@@ -210,3 +210,33 @@ def test_fermionic_table_host_side():
     Ho.add(oop.scal_opstr(0.5, (oop.number(1),)))
     Ho.compile()
     np.testing.assert_array_equal(Ho.fermiC, H.fermionicH)
+
+
+def test_minsr_cholesky_retry_policy(lib):
+    """Host-side route policy of MinSR: a refused Cholesky attempt skips the
+    next CHOLESKY_RETRY - 1 attempts; a skipped solve whose spectrum kept the
+    rank the route accepts re-enables the attempt; a taken route changes
+    nothing."""
+    from vmc_jax_amd.util.minsr import MinSR
+    eq = MinSR(None, pinvTol=1e-8)
+
+    def solved(tried, iterations, rank, full):
+        eq.last_info = lib.JvSolveInfo()
+        eq.last_info.iterations = iterations
+        eq.last_info.rank = rank
+        eq._note_route(tried, full)
+
+    assert eq._cholesky_hint() == 1
+    solved(1, 0, 2870, 4095)          # refused: the spectrum was cut
+    hints = [eq._cholesky_hint() for _ in range(MinSR.CHOLESKY_RETRY - 1)]
+    assert hints == [0] * (MinSR.CHOLESKY_RETRY - 1) and eq._cholesky_hint() == 1
+    solved(1, 0, 2870, 4095)
+    assert eq._cholesky_hint() == 0
+    solved(0, 0, 4095, 4095)          # skipped, nothing cut beyond the centring vector
+    assert eq._cholesky_hint() == 1
+    solved(1, 1, 4095, 4095)          # route taken
+    assert eq._cholesky_hint() == 1
+    solved(1, 0, 1237, 1240)          # refused again
+    assert eq._cholesky_hint() == 0
+    solved(0, 0, 1230, 1240)          # skipped and still cut: keep skipping
+    assert eq._chol_skip == MinSR.CHOLESKY_RETRY - 2

@@ -145,3 +145,77 @@ def test_t_form_singular_uses_eigendecomposition():
     ref, rank = _reference_t(D, e, 1e-10)
     assert info.iterations == 0 and info.rank == rank == p
     assert _rel(u, ref) < 1e-8
+
+
+def _centred(O, w):
+    """jv_center's layout: sqrt(w_n) (O_n - sum_m w_m O_m) (stats.py:197-205)."""
+    return np.sqrt(w)[:, None] * (O - (w[:, None] * O).sum(0))
+
+
+@pytest.mark.parametrize("cpx,n,p", [(False, 300, 700), (True, 200, 500), (True, 1100, 2000)])
+def test_t_form_centred_deflates_null_vector(cpx, n, p):
+    """Centred data with uniform weights: T = D D^dagger has the null vector
+    1/sqrt(n), whose eigenvalue the pseudo-inverse cuts.  The route deflates it
+    (rank n - 1) and gives the eigendecomposition's update."""
+    rng = np.random.default_rng(n + p + 1)
+    O = (rng.normal(size=(n, p)) + (1j * rng.normal(size=(n, p)) if cpx else 0)) / np.sqrt(p)
+    D = _centred(O, np.full(n, 1.0 / n))
+    e = rng.normal(size=n) + 1j * rng.normal(size=n)
+    print("centred T form", cpx, n, p, flush=True)
+    u, info = _solve_t(D, e, 1e-10)
+    ref, rank = _reference_t(D, e, 1e-10)
+    assert rank == n - 1
+    assert info.iterations == 1 and info.rank == n - 1
+    assert _rel(u, ref) < 1e-10
+
+
+@pytest.mark.parametrize("case", ["weights", "rank"])
+def test_t_form_centred_not_deflatable(case):
+    """Non-uniform weights (the null vector is sqrt(w), not the constant: no
+    deflation), or fewer parameters than samples (more than one eigenvalue
+    cut): the eigendecomposition runs and gives the pseudo-inverse update."""
+    rng = np.random.default_rng(9)
+    n, p = (300, 700) if case == "weights" else (400, 150)
+    O = (rng.normal(size=(n, p)) + 1j * rng.normal(size=(n, p))) / np.sqrt(p)
+    w = rng.uniform(0.5, 1.5, n) if case == "weights" else np.ones(n)
+    D = _centred(O, w / w.sum())
+    e = rng.normal(size=n) + 1j * rng.normal(size=n)
+    u, info = _solve_t(D, e, 1e-10)
+    ref, rank = _reference_t(D, e, 1e-10)
+    assert info.iterations == 0 and info.rank == rank == (n - 1 if case == "weights" else p)
+    assert _rel(u, ref) < 1e-8
+
+
+def test_cholesky_route_concurrent_streams():
+    """Eight host threads, each on its own stream, run the route at once (the
+    pattern of the batched field sweep): every workgroup of a panel launch must
+    read the diagonal block's input values however late it starts, so each
+    result is bit-identical to the same solve run alone."""
+    import threading
+    rng = np.random.default_rng(13)
+    n, p = 1100, 2000
+    cases = []
+    for _ in range(8):
+        O = (rng.normal(size=(n, p)) + 1j * rng.normal(size=(n, p))) / np.sqrt(p)
+        cases.append((_centred(O, np.full(n, 1.0 / n)), rng.normal(size=n) + 1j * rng.normal(size=n)))
+    alone = [_solve_t(D, e, 1e-10) for D, e in cases]
+    results, errors = [None] * len(cases), []
+
+    def work(k):
+        try:
+            s = torch.cuda.Stream()
+            with torch.cuda.stream(s):
+                results[k] = _solve_t(*cases[k], 1e-10)
+            s.synchronize()
+        except Exception as exc:  # reported below
+            errors.append(repr(exc))
+
+    threads = [threading.Thread(target=work, args=(k,)) for k in range(len(cases))]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not errors, errors
+    for (u, info), (ua, infoa) in zip(results, alone):
+        assert info.iterations == infoa.iterations == 1 and info.rank == n - 1
+        assert np.array_equal(u, ua)

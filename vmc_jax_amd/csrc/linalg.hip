@@ -562,14 +562,111 @@ __global__ void __launch_bounds__(256) sum_finish(const double* __restrict__ par
 // Sc = S - c I with c = ||S||_F (rcond + 8 p eps) (||S||_F bounds the largest
 // eigenvalue, 8 p eps ||S||_F the rounding of the Cholesky factorisation) and
 // S2 = S.
+// Sc = S + a E - c I and S2 = S + a E, where E has ones in the blk x blk
+// diagonal blocks and a = *add (no add: a = 0), the deflation of the centring
+// null vector found by null_probe / null_decide.
 __global__ void shifted_copy(const double* __restrict__ S, int64_t p, const double* __restrict__ fro2, double rcond,
-                             double* __restrict__ Sc, double* __restrict__ S2) {
+                             const double* __restrict__ add, int64_t blk, double* __restrict__ Sc,
+                             double* __restrict__ S2) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= p * p) return;
   const double c = sqrt(*fro2) * (rcond + 8.0 * (double)p * 2.220446049250313e-16);
-  const double v = S[t];
-  Sc[t] = v - (t / p == t % p ? c : 0.0);
+  const int64_t i = t / p, j = t % p;
+  double v = S[t];
+  if (add && i / blk == j / blk) v += *add;
+  Sc[t] = v - (i == j ? c : 0.0);
   S2[t] = v;
+}
+
+// Centred data (row n scaled by sqrt(w_n), sum_n sqrt(w_n) D_n = 0) make
+// u0 = sqrt(w) a null vector of T = D D^dagger, so the pseudo-inverse of T
+// always cuts one eigenvalue and T itself never passes the Cholesky test.  For
+// uniform weights u0 = 1/sqrt(n): part[i] = |(T u0)_i|^2 and part[n + i] =
+// Re T_ii (one wave per row).
+__global__ void __launch_bounds__(256) null_probe(const double* __restrict__ T, int64_t n, int cpx,
+                                                  double* __restrict__ part) {
+  const int lane = threadIdx.x % WAVE;
+  const int64_t i = (int64_t)blockIdx.x * 4 + threadIdx.x / WAVE;
+  if (i >= n) return;
+  double sr = 0.0, si = 0.0;
+  if (cpx) {
+    const double* row = T + 2 * i * n;
+    for (int64_t j = lane; j < n; j += WAVE) {
+      sr += row[2 * j];
+      si += row[2 * j + 1];
+    }
+  } else {
+    const double* row = T + i * n;
+    for (int64_t j = lane; j < n; j += WAVE) sr += row[j];
+  }
+  sr = wave_sum(sr);
+  si = wave_sum(si);
+  if (lane == 0) {
+    part[i] = (sr * sr + si * si) / (double)n;
+    part[n + i] = cpx ? T[2 * (i * n + i)] : T[i * n + i];
+  }
+}
+
+// addv[0] = mu / n (mu = max Re T_ii <= lambda_max(T)) and addv[1] = 1 when
+// ||T u0|| <= rcond mu / 2, else both 0.  Then T has an eigenvalue within
+// ||T u0|| of zero, below the cutoff rcond lambda_max; T + mu u0 u0^T moves it
+// up, and by interlacing a successful Cholesky test of the deflated matrix
+// (c from ||T||_F: mu adds at most ||T||_F to the norm, which the 8 p eps
+// margin covers) shows every other eigenvalue of T above the cutoff.
+__global__ void __launch_bounds__(1024) null_decide(const double* __restrict__ part, int64_t n, double rcond,
+                                                    double* __restrict__ addv) {
+  __shared__ double rs[1024], rm[1024];
+  const int tid = threadIdx.x;
+  double s = 0.0, m = 0.0;
+  for (int64_t i = tid; i < n; i += 1024) {
+    s += part[i];
+    m = fmax(m, part[n + i]);
+  }
+  rs[tid] = s;
+  rm[tid] = m;
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if (tid < w) {
+      rs[tid] += rs[tid + w];
+      rm[tid] = fmax(rm[tid], rm[tid + w]);
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    const bool defl = rm[0] > 0.0 && sqrt(rs[0]) <= 0.5 * rcond * rm[0];
+    addv[0] = defl ? rm[0] / (double)n : 0.0;
+    addv[1] = defl ? 1.0 : 0.0;
+  }
+}
+
+// b <- b - u0 (u0^T b) for each of the `halves` length-n parts of b (u0 =
+// 1/sqrt(n): the mean is removed) when addv[1] is set; rank[0] = n - addv[1],
+// rank[1] = 1 (the Cholesky route).
+__global__ void __launch_bounds__(1024) project_null(double* __restrict__ b, int64_t n, int halves,
+                                                     const double* __restrict__ addv, int* __restrict__ rank) {
+  __shared__ double red[1024];
+  const int tid = threadIdx.x;
+  const bool defl = addv[1] != 0.0;
+  if (tid == 0) {
+    rank[0] = (int)n - (defl ? 1 : 0);
+    rank[1] = 1;
+  }
+  if (!defl) return;
+  for (int h = 0; h < halves; ++h) {
+    double* v = b + (int64_t)h * n;
+    double sum = 0.0;
+    for (int64_t i = tid; i < n; i += 1024) sum += v[i];
+    red[tid] = sum;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+      if (tid < w) red[tid] += red[tid + w];
+      __syncthreads();
+    }
+    const double mean = red[0] / (double)n;
+    __syncthreads();
+    for (int64_t i = tid; i < n; i += 1024) v[i] -= mean;
+    __syncthreads();
+  }
 }
 
 __global__ void set_ints(int* __restrict__ v, int a, int b) {
@@ -641,7 +738,8 @@ int jv_eigh(int64_t n, int32_t is_complex, double* A, double* w, void* stream) {
 }  // extern "C"
 namespace jv {
 static bool minsr_cholesky_enabled();
-static int chol_route(const double* S, int64_t p, double rcond, int* flags, bool* ok, double** L, hipStream_t st);
+static int chol_route(const double* S, int64_t p, double rcond, int* flags, bool* ok, double** L, hipStream_t st,
+                      const double* add = nullptr, int64_t blk = 1);
 int chol_inverse_lower(const double* L, int p, double* W, hipStream_t st);
 static int minsr_dual_finish_info(int rc, jv_solve_info* info, int64_t p, double rcond, bool deferred,
                                   hipStream_t st);
@@ -703,6 +801,15 @@ int jv_minsr_solve(const double* data, const double* e, int64_t n, int64_t p, in
     // complex T, has no eigenvalue at or below the cutoff; then
     // x = W^T W b with W = L^-1 (two GEMVs, no n^3 product)
     const int64_t m = cpx ? 2 * n : n;
+    // the centring null vector 1/sqrt(n) of uniformly weighted data is deflated
+    // (its eigenvalue is cut by the pseudo-inverse, and b is projected off it)
+    double* probe = (double*)workspace(sizeof(double) * 2 * (size_t)n, 101);
+    double* addv = (double*)workspace(sizeof(double) * 2, 102);
+    if (!probe || !addv) return fail(JV_E_NOMEM, "jv_minsr_solve: workspace allocation failed");
+    hipLaunchKernelGGL(null_probe, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, T, n, cpx, probe);
+    JV_LAUNCH_CHECK("null_probe");
+    hipLaunchKernelGGL(null_decide, dim3(1), dim3(1024), 0, st, probe, n, rcond, addv);
+    JV_LAUNCH_CHECK("null_decide");
     double* R = T;
     if (cpx) {
       R = (double*)workspace(sizeof(double) * (size_t)m * m, 99);
@@ -712,7 +819,7 @@ int jv_minsr_solve(const double* data, const double* e, int64_t n, int64_t p, in
     }
     bool ok = false;
     double* L = nullptr;
-    if ((rc = chol_route(R, m, rcond, rank, &ok, &L, st))) return rc;
+    if ((rc = chol_route(R, m, rcond, rank, &ok, &L, st, addv, n))) return rc;
     if (ok) {
       double* W = (double*)workspace(sizeof(double) * (size_t)m * m, 95);  // R - c I is no longer needed
       double* b = (double*)workspace(sizeof(double) * 2 * (size_t)m, 100);
@@ -723,6 +830,8 @@ int jv_minsr_solve(const double* data, const double* e, int64_t n, int64_t p, in
       }
       hipLaunchKernelGGL(split_complex, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, e, n, cpx, b);
       JV_LAUNCH_CHECK("split_complex");
+      hipLaunchKernelGGL(project_null, dim3(1), dim3(1024), 0, st, b, n, cpx ? 2 : 1, addv, rank);
+      JV_LAUNCH_CHECK("project_null");
       if ((rc = gemv(false, W, b, m, m, 0, 0, 1.0, b + m, st, 12))) return rc;  // W b
       if ((rc = gemv(true, W, b + m, m, m, 0, 0, 1.0, b, st, 12))) return rc;   // W^T W b
       const double* xs = b;
@@ -731,10 +840,8 @@ int jv_minsr_solve(const double* data, const double* e, int64_t n, int64_t p, in
         JV_LAUNCH_CHECK("join_complex");
         xs = x;
       }
-      // update = -D^dagger x
+      // update = -D^dagger x (project_null has set the rank)
       if ((rc = gemv(true, data, xs, n, p, cpx, 1, -1.0, update, st, 12))) return rc;
-      hipLaunchKernelGGL(set_ints, dim3(1), dim3(64), 0, st, rank, (int)n, 1);
-      JV_LAUNCH_CHECK("set_ints");
       return minsr_dual_finish_info(JV_OK, info, n, rcond, false, st);
     }
   }
@@ -1353,7 +1460,8 @@ static bool minsr_cholesky_enabled() {
 int chol_factor2(double* A0, double* A1, int p, int* info, hipStream_t st);
 int chol_inverse_lower(const double* L, int p, double* W, hipStream_t st);
 
-static int chol_route(const double* S, int64_t p, double rcond, int* flags, bool* ok, double** L, hipStream_t st) {
+static int chol_route(const double* S, int64_t p, double rcond, int* flags, bool* ok, double** L, hipStream_t st,
+                      const double* add, int64_t blk) {
   *ok = false;
   const size_t pp = (size_t)p * p;
   double* Sc = (double*)workspace(sizeof(double) * pp, 95);
@@ -1366,8 +1474,8 @@ static int chol_route(const double* S, int64_t p, double rcond, int* flags, bool
     JV_LAUNCH_CHECK("sum_sq_partial");
     hipLaunchKernelGGL(sum_finish, dim3(1), dim3(256), 0, st, part, 256, part + 256);
     JV_LAUNCH_CHECK("sum_finish");
-    hipLaunchKernelGGL(shifted_copy, dim3((unsigned)((pp + 255) / 256)), dim3(256), 0, st, S, p, part + 256, rcond, Sc,
-                       S2);
+    hipLaunchKernelGGL(shifted_copy, dim3((unsigned)((pp + 255) / 256)), dim3(256), 0, st, S, p, part + 256, rcond, add,
+                       blk, Sc, S2);
     JV_LAUNCH_CHECK("shifted_copy");
     // small matrices: both factorisations in the same latency-bound launches;
     // large ones: the proof first, so that a refused attempt costs one

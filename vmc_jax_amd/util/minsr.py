@@ -32,15 +32,18 @@ class MinSR:
         self._chol_skip = 0
 
     # Real dual solves try the Cholesky route first (exact when no eigenvalue
-    # falls below the cutoff).  After a solve whose spectrum was cut, the
-    # attempt is skipped for the next CHOLESKY_RETRY - 1 solves: the spectrum
-    # changes slowly from step to step, and a failed attempt costs a
-    # factorisation.  The update is the same either way.
+    # falls below the cutoff).  After a refused attempt, the attempt is skipped
+    # for the next CHOLESKY_RETRY - 1 solves: the spectrum changes slowly from
+    # step to step, and a refused attempt costs a factorisation.  A skipped
+    # solve whose eigendecomposition cut nothing the route cannot handle makes
+    # the next solve try again.  The update is the same either way.
     CHOLESKY_RETRY = 8
     # The holomorphic T form (complex T = G G^dagger, N < P) tries the route
-    # only when asked: with RBM gradients its T is usually rank deficient, and
-    # a refused attempt costs a factorisation of the 2N x 2N real form.
-    try_cholesky_holomorphic = False
+    # too: centred, uniformly weighted samples leave T one null vector, the
+    # constant 1/sqrt(N), which the device deflates (the pseudo-inverse cuts
+    # its eigenvalue); the attempt succeeds when nothing else is cut.  Set to
+    # False to go straight to the eigendecomposition.
+    try_cholesky_holomorphic = True
 
     def _cholesky_hint(self):
         if self._chol_skip > 0:
@@ -48,9 +51,15 @@ class MinSR:
             return 0
         return 1
 
-    def _note_route(self, tried):
-        if tried and self.last_info.iterations == 0:
+    def _note_route(self, tried, full):
+        """`full`: the rank of a spectrum the route accepts (P for the dual
+        Gram; N - 1 for T, whose centring null vector the route deflates)."""
+        if self.last_info.iterations != 0:
+            return
+        if tried:
             self._chol_skip = self.CHOLESKY_RETRY - 1
+        elif self.last_info.rank >= full:
+            self._chol_skip = 0
 
     def set_pinv_tol(self, tol):
         self.pinvTol = tol
@@ -87,7 +96,7 @@ class MinSR:
             _lib.call("jv_minsr_solve", _lib.ptr(D), _lib.ptr(e), n, p, int(cpx), float(self.pinvTol),
                       float(self.diagonalShift), hint, _lib.ptr(upd), ctypes.byref(info), _lib.stream_ptr())
         self.last_info = info
-        self._note_route(hint)
+        self._note_route(hint, p if dual else n - 1)
         if cpx:
             # T_ref = 2 G G^dagger: pinv scales by 1/2; [v, -i v] layout per leaf
             return psi.expand_compact(0.5 * upd)
@@ -114,7 +123,7 @@ class MinSR:
                   n, P, 0, float(self.pinvTol), float(self.diagonalShift), hint, _lib.ptr(upd), ctypes.byref(info),
                   _lib.stream_ptr())
         self.last_info = info
-        self._note_route(hint)
+        self._note_route(hint, P)
         return upd
 
     def __call__(self, netParameters, t, *, psi, hamiltonian, **rhsArgs):
