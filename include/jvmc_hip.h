@@ -323,8 +323,11 @@ int jv_stedc(int64_t n, double* d, const double* e, double* z, void* stream);
  * Cholesky factorisation of T - c I (complex T: of its real 2n x 2n form
  * [[Re T, -Im T], [Im T, Re T]], same eigenvalues) with c = ||T||_F (rcond +
  * 8 n eps) succeeds, no eigenvalue is cut and x = T^-1 e comes from L^-1
- * (info->iterations = 1); otherwise the eigendecomposition runs.  info (host)
- * may be NULL. */
+ * (info->iterations = 1); otherwise the eigendecomposition runs.  Centred data
+ * with uniform weights leave T the null vector u0 = 1/sqrt(n): when
+ * ||T u0|| <= rcond max Re T_ii / 2 the test runs on T + (max Re T_ii) u0 u0^T
+ * and, if it passes, x = T'^-1 (e - u0 u0^T e) (the pseudo-inverse with u0's
+ * eigenvalue cut; info->rank = n - 1).  info (host) may be NULL. */
 int jv_minsr_solve(const double* data, const double* e, int64_t n, int64_t p,
                    int32_t is_complex, double rcond, double diag_shift, int32_t try_cholesky,
                    double* update, jv_solve_info* info, void* stream);
