@@ -213,9 +213,9 @@ def test_fermionic_table_host_side():
 
 
 def test_minsr_cholesky_retry_policy(lib):
-    """Host-side route policy of MinSR: a refused Cholesky attempt skips the
-    next CHOLESKY_RETRY - 1 attempts; a skipped solve whose spectrum kept the
-    rank the route accepts re-enables the attempt; a taken route changes
+    """Host-side route policy of the dual MinSR solve: a refused Cholesky
+    attempt skips the next CHOLESKY_RETRY - 1 attempts; a skipped solve whose
+    spectrum kept full rank re-enables the attempt; a taken route changes
     nothing."""
     from vmc_jax_amd.util.minsr import MinSR
     eq = MinSR(None, pinvTol=1e-8)
@@ -227,14 +227,14 @@ def test_minsr_cholesky_retry_policy(lib):
         eq._note_route(tried, full)
 
     assert eq._cholesky_hint() == 1
-    solved(1, 0, 2870, 4095)          # refused: the spectrum was cut
+    solved(1, 0, 1100, 1240)          # refused: the spectrum was cut
     hints = [eq._cholesky_hint() for _ in range(MinSR.CHOLESKY_RETRY - 1)]
     assert hints == [0] * (MinSR.CHOLESKY_RETRY - 1) and eq._cholesky_hint() == 1
-    solved(1, 0, 2870, 4095)
+    solved(1, 0, 1100, 1240)
     assert eq._cholesky_hint() == 0
-    solved(0, 0, 4095, 4095)          # skipped, nothing cut beyond the centring vector
+    solved(0, 0, 1240, 1240)          # skipped, nothing cut
     assert eq._cholesky_hint() == 1
-    solved(1, 1, 4095, 4095)          # route taken
+    solved(1, 1, 1240, 1240)          # route taken
     assert eq._cholesky_hint() == 1
     solved(1, 0, 1237, 1240)          # refused again
     assert eq._cholesky_hint() == 0

@@ -35,8 +35,9 @@ class MinSR:
     # falls below the cutoff).  After a refused attempt, the attempt is skipped
     # for the next CHOLESKY_RETRY - 1 solves: the spectrum changes slowly from
     # step to step, and a refused attempt costs a factorisation.  A skipped
-    # solve whose eigendecomposition cut nothing the route cannot handle makes
-    # the next solve try again.  The update is the same either way.
+    # solve whose eigendecomposition cut nothing makes the next solve try
+    # again.  The T form tries on every solve that can pass.  The update is the
+    # same either way.
     CHOLESKY_RETRY = 8
     # The holomorphic T form (complex T = G G^dagger, N < P) tries the route
     # too: centred, uniformly weighted samples leave T one null vector, the
@@ -52,8 +53,8 @@ class MinSR:
         return 1
 
     def _note_route(self, tried, full):
-        """`full`: the rank of a spectrum the route accepts (P for the dual
-        Gram; N - 1 for T, whose centring null vector the route deflates)."""
+        """Back-off of the dual solve; `full`: the rank of a spectrum the
+        route accepts (P for the P x P Gram)."""
         if self.last_info.iterations != 0:
             return
         if tried:
@@ -92,11 +93,15 @@ class MinSR:
             _lib.call("jv_minsr_solve_dual", _lib.ptr(D), _lib.ptr(e), n, p, int(cpx), float(self.pinvTol),
                       float(self.diagonalShift), hint, _lib.ptr(upd), ctypes.byref(info), _lib.stream_ptr())
         else:
-            hint = self._cholesky_hint() if (not cpx or self.try_cholesky_holomorphic) else 0
+            # T form: a refused attempt costs one factorisation beside an
+            # eigendecomposition many times its price, so every solve that can
+            # pass tries (T has rank <= P: with N - 1 > P it is always cut)
+            hint = int((not cpx or self.try_cholesky_holomorphic) and n - 1 <= p)
             _lib.call("jv_minsr_solve", _lib.ptr(D), _lib.ptr(e), n, p, int(cpx), float(self.pinvTol),
                       float(self.diagonalShift), hint, _lib.ptr(upd), ctypes.byref(info), _lib.stream_ptr())
         self.last_info = info
-        self._note_route(hint, p if dual else n - 1)
+        if dual:
+            self._note_route(hint, p)
         if cpx:
             # T_ref = 2 G G^dagger: pinv scales by 1/2; [v, -i v] layout per leaf
             return psi.expand_compact(0.5 * upd)
