@@ -53,10 +53,13 @@ __host__ __device__ inline DcNode dc_node(int n, int k, int i) {
   return DcNode{(int)a, (int)(m - a), (int)(z - m)};
 }
 
-// leaf depth: the smallest D with ceil(n / 2^D) <= DC_LEAF
+// leaf depth: the smallest D with ceil(n / 2^D) <= leaf, leaf = DC_LEAF, or
+// DC_LEAF / 2 below n = 512 (the sequential QL chain of a leaf grows as its
+// size squared; at small n an extra merge level costs less than it saves)
 inline int dc_depth(int n) {
+  const int leaf = n < 512 ? DC_LEAF / 2 : DC_LEAF;
   int D = 0;
-  while (((int64_t)n + (int64_t(1) << D) - 1) >> D > DC_LEAF) ++D;
+  while (((int64_t)n + (int64_t(1) << D) - 1) >> D > leaf) ++D;
   return D;
 }
 
