@@ -146,7 +146,14 @@ const char* jv_last_error(void);
 /* Selects the device for the calling thread.  Replaces the rank-to-device pick
  * of global_defs.py:15-19. */
 int jv_set_device(int device);
-/* Releases the calling thread's workspaces and solver handles. */
+/* Loads the solver libraries' kernels on `device` once per process and makes it
+ * current (no reference counterpart: jax initialises its backends on import).
+ * Every first solver call does the same, serialised; a server calls it before
+ * it starts job threads so no job pays the load. */
+int jv_init(int device);
+/* Releases the calling thread's workspaces and solver handles.  Worker threads
+ * release theirs when they end; the main thread's are kept until this call or
+ * process exit (no HIP call is made from exit handlers). */
 int jv_shutdown(void);
 /* Kernel timing (no reference counterpart; measurement support): when enabled,
  * HIP events are recorded on the launch stream around the named kernels
@@ -391,6 +398,25 @@ int jv_tdvp_solve_holomorphic(const double* data, const double* e, const double*
                               int32_t make_real, double diag_shift, int32_t use_snr, double glob_num_samples,
                               double snr_tol, double pinv_tol, double pinv_cutoff, double* update, double* S,
                               double* F, double* ev, double* snr, jv_solve_info* info, void* stream);
+
+/* SampledObs.subset (stats.py:310-329): rows r_k = start + k step, k < count, of
+ * centred data [n][p] (real or complex) with weights w [n] and mean [p]:
+ * out_w = w[r] / nrm (nrm = sum_k w[r_k]), out_mean = sum_k sqrt(out_w_k)
+ * data[r_k] / sqrt(nrm) + mean, out_data[k] = data[r_k] / sqrt(nrm) +
+ * sqrt(out_w_k) (mean - out_mean).  Device pointers. */
+int jv_subset(const double* data, const double* w, const double* mean, int64_t n, int64_t p, int32_t is_complex,
+              int64_t start, int64_t step, int64_t count, double* out_data, double* out_w, double* out_mean,
+              void* stream);
+
+/* Cross-validation check of a TDVP update (tdvp.py:306-324) on a second sample
+ * set with centred data D [n][pr] (complex, jVMC layout) and centred E_loc e [n]:
+ * with S0 = D^dagger D, F0 = -x D^dagger e, S = makeReal(S0) (+ diag shift as
+ * tdvp.py:145-146) and F = makeReal(F0), out (host, 2 doubles) receives
+ * out[0] = ||S u - F|| / ||F|| and out[1] = Re(u^T S0 u) - 2 Re(u^T F0), the
+ * numerator of the tdvp_error (tdvp.py:102-104).  S0 is applied as D^dagger (D u):
+ * no pr x pr matrix is formed.  update u: real [pr] (device). */
+int jv_tdvp_cross_validate(const double* D, const double* e, int64_t n, int64_t pr, const double* x,
+                           int32_t make_real, double diag_shift, const double* update, double* out, void* stream);
 
 #ifdef __cplusplus
 }

@@ -93,7 +93,7 @@ class TDVP:
         return update, residual, max(cutoff, self.pinvCutoff)
 
     def __call__(self, netParameters, t, *, psi, hamiltonian, **rhsArgs):
-        """tdvp.py:219-326 (without cross-validation)."""
+        """tdvp.py:219-326."""
         tmp = psi.get_parameters()
         psi.set_parameters(netParameters)
         numSamples = rhsArgs.get("numSamples", None)
@@ -107,7 +107,30 @@ class TDVP:
             self.ElocVar0 = self.ElocVar
             self.metaData = {"tdvp_error": self._get_tdvp_error(update), "tdvp_residual": res,
                              "pinv_cutoff": cut, "SNR": self.snr, "spectrum": self.ev}
+            if self.crossValidation:
+                update, res = self.cross_validate(Eloc, grads)
         return update
+
+    def cross_validate(self, Eloc, grads):
+        """tdvp.py:306-324: solve on the even samples, validate on the odd ones.
+
+        The tdvp_error of the even-sample update is taken with S0, F0 of the
+        odd samples (the last get_tdvp_equation call) and the full sample's
+        ElocVar0; the residual ratio divides by the full solve's residual.  The
+        full solve is repeated, so the returned update is the full one."""
+        E1, G1 = Eloc.subset(start=0, step=2), grads.subset(start=0, step=2)
+        E2, G2 = Eloc.subset(start=1, step=2), grads.subset(start=1, step=2)
+        update_1, _, _ = self.solve(E1, G1)
+        S2, F2 = self.get_tdvp_equation(E2, G2)
+        validation_tdvpErr = self._get_tdvp_error(update_1)
+        update, res, _ = self.solve(Eloc, grads)
+        validation_residual = (np.linalg.norm(S2.dot(update_1) - F2) / np.linalg.norm(F2)) / res
+        self.crossValidationFactor_residual = validation_residual
+        self.crossValidationFactor_tdvpErr = validation_tdvpErr / self.metaData["tdvp_error"]
+        self.metaData["tdvp_residual_cross_validation_ratio"] = self.crossValidationFactor_residual
+        self.metaData["tdvp_error_cross_validation_ratio"] = self.crossValidationFactor_tdvpErr
+        self.S, _ = self.get_tdvp_equation(Eloc, grads)
+        return update, res
 
     def get_energy_variance(self):
         return self.ElocVar0

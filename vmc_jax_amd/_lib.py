@@ -68,6 +68,7 @@ class JvSolveInfo(ctypes.Structure):
 _SIGS = {
     "jv_version": [],
     "jv_set_device": [ctypes.c_int],
+    "jv_init": [ctypes.c_int],
     "jv_shutdown": [],
     "jv_timer_enable": [ctypes.c_int],
     "jv_timer_reset": [],
@@ -103,6 +104,9 @@ _SIGS = {
                       _dbl, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(JvSolveInfo), _vp],
     "jv_tdvp_solve_holomorphic": [_vp, _vp, _vp, _i64, _i64, _vp, _vp, _i64, ctypes.POINTER(_dbl), _i32, _dbl, _i32,
                                   _dbl, _dbl, _dbl, _dbl, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(JvSolveInfo), _vp],
+    "jv_subset": [_vp, _vp, _vp, _i64, _i64, _i32, _i64, _i64, _i64, _vp, _vp, _vp, _vp],
+    "jv_tdvp_cross_validate": [_vp, _vp, _i64, _i64, ctypes.POINTER(_dbl), _i32, _dbl, _vp, ctypes.POINTER(_dbl),
+                               _vp],
 }
 
 

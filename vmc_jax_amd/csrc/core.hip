@@ -1,5 +1,8 @@
 // Error plumbing, per-device workspace and solver handles for libjvmc_hip.
 #include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+#include <sys/syscall.h>
+#include <unistd.h>
 
 #include <mutex>
 #include <vector>
@@ -59,12 +62,65 @@ void release(DeviceState& s, int d) {
   (void)hipSetDevice(cur);
 }
 
+// A worker thread that ends returns its workspaces and handles.  The main
+// thread's thread_local state is destroyed only inside exit(), when the HIP
+// runtime and the libraries it loaded (a profiler's included) may already be
+// finalising: it makes no HIP call then and leaves the memory to the process
+// teardown (jv_shutdown releases it explicitly while the runtime is up).
 ThreadState::~ThreadState() {
   std::lock_guard<std::mutex> lk(g_mu);
-  for (size_t d = 0; d < dev.size(); ++d)
-    if (!dev[d].ptr.empty() || dev[d].blas || dev[d].aux) release(dev[d], (int)d);
+  const bool main_thread = (pid_t)syscall(SYS_gettid) == getpid();
+  if (!main_thread)
+    for (size_t d = 0; d < dev.size(); ++d)
+      if (!dev[d].ptr.empty() || dev[d].blas || dev[d].aux) release(dev[d], (int)d);
   for (size_t i = 0; i < g_threads.size(); ++i)
     if (g_threads[i] == this) { g_threads.erase(g_threads.begin() + i); break; }
+}
+
+// rocBLAS loads its kernel libraries, and the HIP runtime rocSOLVER's code
+// objects, on first use.  Threads that make their first solver calls at the
+// same moment raced inside those loads; they now happen once per device,
+// before any handle is handed out: rocblas_initialize() plus one tiny real and
+// one tiny complex eigendecomposition on a private stream.
+constexpr int MAX_DEVICES = 64;
+std::once_flag g_warm_once[MAX_DEVICES];
+int g_warm_rc[MAX_DEVICES];
+
+void warm_libraries(int d) {
+  rocblas_initialize();
+  int rc = 0;
+  rocblas_handle h = nullptr;
+  hipStream_t st = nullptr;
+  double* buf = nullptr;
+  constexpr int n = 16;
+  const size_t bytes = sizeof(double) * (2 * n * n + 4 * n) + sizeof(rocblas_int);
+  if (rocblas_create_handle(&h) != rocblas_status_success) rc = JV_E_SOLVER;
+  if (!rc && hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) rc = JV_E_HIP;
+  if (!rc && hipMalloc((void**)&buf, bytes) != hipSuccess) rc = JV_E_NOMEM;
+  if (!rc) {
+    rocblas_set_stream(h, st);
+    (void)hipMemsetAsync(buf, 0, bytes, st);  // A = 0: a valid symmetric / Hermitian input
+    double* A = buf;
+    double* w = buf + 2 * n * n;
+    double* E = w + n;
+    rocblas_int* info = (rocblas_int*)(E + 2 * n);
+    if (rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, n, A, n, w, E, info) !=
+            rocblas_status_success ||
+        rocsolver_zheevd(h, rocblas_evect_original, rocblas_fill_upper, n, (rocblas_double_complex*)A, n, w, E,
+                         info) != rocblas_status_success)
+      rc = JV_E_SOLVER;
+    if (hipStreamSynchronize(st) != hipSuccess && !rc) rc = JV_E_HIP;
+  }
+  if (buf) (void)hipFree(buf);
+  if (st) (void)hipStreamDestroy(st);
+  if (h) rocblas_destroy_handle(h);
+  g_warm_rc[d] = rc;
+}
+
+int warm(int d) {
+  if (d < 0 || d >= MAX_DEVICES) return JV_E_ARG;
+  std::call_once(g_warm_once[d], warm_libraries, d);
+  return g_warm_rc[d];
 }
 
 DeviceState& state() {
@@ -134,6 +190,10 @@ bool aux_stream(hipStream_t* st, hipEvent_t* e0, hipEvent_t* e1) {
 rocblas_handle blas_handle(hipStream_t stream) {
   DeviceState& s = state();
   if (!s.blas) {
+    int d = 0;
+    (void)hipGetDevice(&d);
+    if (warm(d) != JV_OK) return nullptr;
+    std::lock_guard<std::mutex> lk(g_mu);
     if (rocblas_create_handle(&s.blas) != rocblas_status_success) return nullptr;
     // deterministic kernels only (no atomic split-K): results must not depend on scheduling
     rocblas_set_atomics_mode(s.blas, rocblas_atomics_not_allowed);
@@ -152,6 +212,13 @@ const char* jv_last_error(void) { return jv::g_last_error.c_str(); }
 
 int jv_set_device(int device) {
   JV_HIP(hipSetDevice(device));
+  return JV_OK;
+}
+
+int jv_init(int device) {
+  JV_HIP(hipSetDevice(device));
+  int rc = jv::warm(device);
+  if (rc) return jv::fail(rc, "jv_init: loading the rocBLAS / rocSOLVER kernels failed");
   return JV_OK;
 }
 

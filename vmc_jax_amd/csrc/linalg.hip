@@ -1371,6 +1371,152 @@ int jv_tdvp_solve_holomorphic(const double* D, const double* e, const double* w,
                          pinv_cutoff, update, S, F, ev, snr, info, col_g, col_ig, pc, stream);
 }
 
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ subsets and cross-validation
+namespace jv {
+// nrm = sum_k w[start + k step]; wout[k] = w[start + k step] / nrm; scal[0] = nrm
+// (one workgroup, fixed summation order).
+__global__ void subset_weights(const double* __restrict__ w, int64_t start, int64_t step, int64_t count,
+                               double* __restrict__ wout, double* __restrict__ scal) {
+  __shared__ double red[16];
+  __shared__ double tot;
+  double a = 0.0;
+  for (int64_t k = threadIdx.x; k < count; k += blockDim.x) a += w[start + k * step];
+  double s = block_sum(a, red);
+  if (threadIdx.x == 0) { tot = s; scal[0] = s; }
+  __syncthreads();
+  for (int64_t k = threadIdx.x; k < count; k += blockDim.x) wout[k] = w[start + k * step] / tot;
+}
+
+// part[chunk][j] = sum over the chunk's rows of sqrt(w'_k) data[r_k][j] / sqrt(nrm)
+__global__ void subset_mean_partial(const double* __restrict__ D, const double* __restrict__ wn,
+                                    const double* __restrict__ scal, int64_t start, int64_t step, int64_t count,
+                                    int64_t len, double* __restrict__ part) {
+  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= len) return;
+  const double rs = 1.0 / sqrt(scal[0]);
+  int64_t k0 = (int64_t)blockIdx.y * CHUNK_ROWS, k1 = std::min<int64_t>(count, k0 + CHUNK_ROWS);
+  double a = 0.0;
+  for (int64_t k = k0; k < k1; ++k) a = fma(sqrt(wn[k]), D[(start + k * step) * len + j] * rs, a);
+  part[blockIdx.y * len + j] = a;
+}
+
+__global__ void subset_apply(const double* __restrict__ D, const double* __restrict__ wn,
+                             const double* __restrict__ scal, const double* __restrict__ mean0,
+                             double* __restrict__ mean1, int64_t start, int64_t step, int64_t count, int64_t len,
+                             double* __restrict__ out) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count * len) return;
+  int64_t k = t / len, j = t - k * len;
+  const double d = D[(start + k * step) * len + j] / sqrt(scal[0]);
+  out[t] = d + sqrt(wn[k]) * (mean0[j] - (mean1[j] + mean0[j]));
+}
+
+__global__ void add_vec(double* __restrict__ y, const double* __restrict__ x, int64_t len) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < len) y[i] += x[i];
+}
+
+// out[0] = ||S2 u - F2||^2, out[1] = ||F2||^2, out[2] = Re(u^T S0 u) - 2 Re(u^T F0) with
+// S0 u given (complex), S2 u = makeReal(S0 u) + shift' diag(S2) u, F2 = makeReal(F0).
+__global__ void cross_validate_k(const double* __restrict__ u, const double* __restrict__ S0u,
+                                 const double* __restrict__ F0, const double* __restrict__ dg, int64_t pr, int imag,
+                                 double shift, double* __restrict__ out) {
+  __shared__ double red[16];
+  double rr = 0.0, ff = 0.0, q = 0.0;
+  for (int64_t i = threadIdx.x; i < pr; i += blockDim.x) {
+    const double sr = S0u[2 * i], si = S0u[2 * i + 1], fr = F0[2 * i], fi = F0[2 * i + 1];
+    q += u[i] * (sr - 2.0 * fr);
+    if (imag) {  // i Im(S0 u) - i Im(F0): diag(S) = i Im(diag S0) = 0
+      const double d = si - fi;
+      rr = fma(d, d, rr);
+      ff = fma(fi, fi, ff);
+    } else {
+      const double d = sr + (shift > 1e-10 ? shift * dg[i] * u[i] : 0.0) - fr;
+      rr = fma(d, d, rr);
+      ff = fma(fr, fr, ff);
+    }
+  }
+  rr = block_sum(rr, red);
+  ff = block_sum(ff, red);
+  q = block_sum(q, red);
+  if (threadIdx.x == 0) { out[0] = rr; out[1] = ff; out[2] = q; }
+}
+}  // namespace jv
+
+extern "C" {
+
+int jv_subset(const double* data, const double* w, const double* mean, int64_t n, int64_t p, int32_t is_complex,
+              int64_t start, int64_t step, int64_t count, double* out_data, double* out_w, double* out_mean,
+              void* stream) {
+  using namespace jv;
+  if (!data || !w || !mean || !out_data || !out_w || !out_mean || p < 1 || step < 1 || start < 0 || count < 1 ||
+      start + (count - 1) * step >= n)
+    return fail(JV_E_ARG, "jv_subset: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t len = p * (is_complex ? 2 : 1);
+  const int nchunks = (int)((count + CHUNK_ROWS - 1) / CHUNK_ROWS);
+  double* scal = (double*)workspace(sizeof(double) * 8, 50);
+  double* part = (double*)workspace(sizeof(double) * (size_t)nchunks * len, 51);
+  if (!scal || !part) return fail(JV_E_NOMEM, "jv_subset: workspace allocation failed");
+  hipLaunchKernelGGL(subset_weights, dim3(1), dim3(1024), 0, st, w, start, step, count, out_w, scal);
+  JV_LAUNCH_CHECK("subset_weights");
+  hipLaunchKernelGGL(subset_mean_partial, dim3((unsigned)((len + 255) / 256), nchunks), dim3(256), 0, st, data,
+                     out_w, scal, start, step, count, len, part);
+  JV_LAUNCH_CHECK("subset_mean_partial");
+  // out_mean = sum of the partials (the shift from the old mean); the old mean is added after the data pass
+  hipLaunchKernelGGL(col_reduce, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, st, part, nchunks, len, out_mean);
+  JV_LAUNCH_CHECK("col_reduce");
+  hipLaunchKernelGGL(subset_apply, dim3((unsigned)((count * len + 255) / 256)), dim3(256), 0, st, data, out_w, scal,
+                     mean, out_mean, start, step, count, len, out_data);
+  JV_LAUNCH_CHECK("subset_apply");
+  hipLaunchKernelGGL(add_vec, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, st, out_mean, mean, len);
+  JV_LAUNCH_CHECK("add_vec");
+  return JV_OK;
+}
+
+int jv_tdvp_cross_validate(const double* D, const double* e, int64_t n, int64_t pr, const double* x,
+                           int32_t make_real, double diag_shift, const double* update, double* out, void* stream) {
+  using namespace jv;
+  if (!D || !e || !x || !update || !out || n < 1 || pr < 1)
+    return fail(JV_E_ARG, "jv_tdvp_cross_validate: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  const int imag = make_real != 0;
+  double* ucpx = (double*)workspace(sizeof(double) * 2 * pr, 52);
+  double* y = (double*)workspace(sizeof(double) * 2 * n, 53);
+  double* S0u = (double*)workspace(sizeof(double) * 2 * pr, 54);
+  double* raw = (double*)workspace(sizeof(double) * 2 * pr, 55);
+  double* F0 = (double*)workspace(sizeof(double) * 2 * pr, 56);
+  double* F = (double*)workspace(sizeof(double) * 2 * pr, 57);
+  double* dg = (double*)workspace(sizeof(double) * pr, 58);
+  double* scal = (double*)workspace(sizeof(double) * 4, 59);
+  if (!ucpx || !y || !S0u || !raw || !F0 || !F || !dg || !scal)
+    return fail(JV_E_NOMEM, "jv_tdvp_cross_validate: workspace allocation failed");
+  const unsigned bp = (unsigned)((pr + 255) / 256);
+  int rc;
+  hipLaunchKernelGGL(to_complex, dim3(bp), dim3(256), 0, st, update, pr, ucpx);
+  JV_LAUNCH_CHECK("to_complex");
+  // S0 u = D^H (D u) with S0 = D^H D (stats.py:235-245), F0 = -x D^H e (tdvp.py:139-140)
+  if ((rc = gemv(false, D, ucpx, n, pr, 1, 0, 1.0, y, st, 60))) return rc;
+  if ((rc = gemv(true, D, y, n, pr, 1, 1, 1.0, S0u, st, 60))) return rc;
+  if ((rc = gemv(true, D, e, n, pr, 1, 1, 1.0, raw, st, 60))) return rc;
+  hipLaunchKernelGGL(tdvp_force, dim3(bp), dim3(256), 0, st, raw, pr, x[0], x[1], imag, F0, F);
+  JV_LAUNCH_CHECK("tdvp_force");
+  if (!imag && diag_shift > 1e-10) {  // diag(Re S0) = column norms of D (tdvp.py:145-146)
+    if ((rc = col_norm2(D, n, pr, 1, dg, st, 61))) return rc;
+  }
+  hipLaunchKernelGGL(cross_validate_k, dim3(1), dim3(1024), 0, st, update, S0u, F0, dg, pr, imag, diag_shift, scal);
+  JV_LAUNCH_CHECK("cross_validate_k");
+  double h[3];
+  JV_HIP(hipMemcpyAsync(h, scal, sizeof(h), hipMemcpyDeviceToHost, st));
+  JV_HIP(hipStreamSynchronize(st));
+  out[0] = sqrt(h[0]) / sqrt(h[1]);
+  out[1] = h[2];
+  return JV_OK;
+}
+
 }  // extern "C"
 
 // ====================================================================== MinSR, dual form

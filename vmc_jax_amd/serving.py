@@ -157,6 +157,10 @@ class Server:
     def run(self, poll=0.002, time_scale=1.0):
         """Runs until every submitted job has finished.  Arrival times are in
         seconds after the start of ``run`` (scaled by ``time_scale``)."""
+        if self.stream_factory is torch.cuda.Stream:
+            # solver kernels loaded once on this thread before job threads start
+            from . import _lib
+            _lib.call("jv_init", torch.cuda.current_device())
         t_start = time.perf_counter()
         while self.queue or self.active:
             now = (time.perf_counter() - t_start) / time_scale

@@ -82,16 +82,20 @@ class SampledObs:
         return SampledObs(t.reshape(1, t.shape[0], -1), self._weights)
 
     def subset(self, start=None, end=None, step=None):
-        """stats.py:310-329."""
-        sl = slice(start, end, step)
-        w = self._weights[sl]
-        nrm = w.sum()
+        """stats.py:310-329 (jv_subset: renormalised weights, re-centred data)."""
+        n, m = self._data.shape
+        rows = range(n)[slice(start, end, step)]
+        if len(rows) == 0:
+            raise ValueError("subset selects no samples")
+        cnt = len(rows)
         new = SampledObs()
-        new._weights = (w / nrm).contiguous()
-        d = self._data[sl] / torch.sqrt(nrm)
-        new._mean = torch.sum(torch.sqrt(new._weights)[:, None] * d, dim=0) + self._mean
-        new._data = (d + torch.sqrt(new._weights)[:, None] * (self._mean - new._mean)[None, :]).contiguous()
-        new._cpx = torch.is_complex(new._data)
+        new._weights = torch.empty(cnt, dtype=torch.float64, device=self._data.device)
+        new._mean = torch.empty_like(self._mean)
+        new._data = torch.empty((cnt, m), dtype=self._data.dtype, device=self._data.device)
+        new._cpx = torch.is_complex(self._data)
+        _lib.call("jv_subset", _lib.ptr(self._data), _lib.ptr(self._weights), _lib.ptr(self._mean), n, m,
+                  int(new._cpx), rows.start, rows.step, cnt, _lib.ptr(new._data), _lib.ptr(new._weights),
+                  _lib.ptr(new._mean), _lib.stream_ptr())
         return new
 
     def tangent_kernel(self):

@@ -16,10 +16,11 @@ from .tdvp import _Timer
 
 
 class MinSR:
-    def __init__(self, sampler, pinvTol=1e-14, diagonalShift=0., diagonalizeOnDevice=True, solver="T"):
+    def __init__(self, sampler, pinvTol=1e-14, diagonalShift=0., diagonalizeOnDevice=True, solver="auto"):
         """``solver``: "T" diagonalises the N x N tangent kernel as the reference
         does; "dual" diagonalises the P x P Gram D^dagger D, which gives the same
-        update (identical nonzero spectrum); "auto" picks the smaller."""
+        update (identical nonzero spectrum); "auto" (default) picks the smaller,
+        so a jVMC script gets the dual form whenever N > P."""
         if solver not in ("T", "dual", "auto"):
             raise ValueError("solver must be 'T', 'dual' or 'auto'")
         self.solver = solver

@@ -61,7 +61,8 @@ class AdaptiveHeun:
             k11 = f(y, t + dt, **rhsArgs, intStep=4)
             dy1 = dy1 + 0.25 * dt * (k01 + k11)
             updateDiff = float(normFunction(dy1 - dy0))
-            fe = self.tolerance / updateDiff
+            # jnp division: a zero difference gives inf, i.e. the x2 step growth
+            fe = self.tolerance / updateDiff if updateDiff != 0.0 else float("inf")
             tmp = 0.2 if 0.2 > 0.9 * fe ** 0.33333 else 0.9 * fe ** 0.33333
             if tmp > 2.:
                 tmp = 2.

@@ -51,8 +51,6 @@ HOLOMORPHIC_SOLVE = {"real": 128, "imag": 1}
 class TDVP:
     def __init__(self, sampler, snrTol=2, pinvTol=1e-14, pinvCutoff=1e-8, makeReal='imag', rhsPrefactor=1.j,
                  diagonalShift=0., crossValidation=False, diagonalizeOnDevice=True):
-        if crossValidation:
-            raise NotImplementedError("crossValidation is not implemented")
         self.sampler = sampler
         self.snrTol = snrTol
         self.pinvTol = pinvTol
@@ -69,8 +67,7 @@ class TDVP:
         self.diagonalShift = delta
 
     def set_cross_validation(self, crossValidation=True):
-        if crossValidation:
-            raise NotImplementedError("crossValidation is not implemented")
+        self.crossValidation = crossValidation
 
     def get_residuals(self):
         return self.metaData["tdvp_error"], self.metaData["tdvp_residual"]
@@ -162,4 +159,34 @@ class TDVP:
             self.ElocVar0 = self.ElocVar
             self.metaData = {"tdvp_error": self._tdvp_error, "tdvp_residual": solverResidual,
                              "pinv_cutoff": pinvCutoff, "SNR": self.snr, "spectrum": self.ev}
+            if self.crossValidation:
+                update = self._cross_validate(Eloc, sampleGradients)
+        return update
+
+    def _cross_validate(self, Eloc, sampleGradients):
+        """tdvp.py:306-324: the update of the even samples, validated on the odd
+        ones.  Its tdvp_error uses the odd samples' S0, F0 with the full sample's
+        ElocVar0 (tdvp.py:102-104 after the odd-sample get_tdvp_equation); the
+        residual ratio divides by the full solve's residual.  The full solve is
+        repeated, as in the reference, so the solver state ends on the full
+        sample."""
+        Eloc1, grads1 = Eloc.subset(start=0, step=2), sampleGradients.subset(start=0, step=2)
+        Eloc2, grads2 = Eloc.subset(start=1, step=2), sampleGradients.subset(start=1, step=2)
+        update_1, _, _ = self.solve(Eloc1, grads1)
+        D2 = grads2._data
+        if not torch.is_complex(D2):
+            D2 = D2.to(torch.complex128)
+        D2 = D2.contiguous()
+        e2 = Eloc2._data.reshape(-1).to(torch.complex128).contiguous()
+        x = (ctypes.c_double * 2)(complex(self.rhsPrefactor).real, complex(self.rhsPrefactor).imag)
+        out = (ctypes.c_double * 2)()
+        _lib.call("jv_tdvp_cross_validate", _lib.ptr(D2), _lib.ptr(e2), D2.shape[0], D2.shape[1], x,
+                  int(self._imag), float(self.diagonalShift), _lib.ptr(update_1), out, _lib.stream_ptr())
+        validation_tdvpErr = abs(1.0 + out[1] / self.ElocVar0)
+        update, solverResidual, _ = self.solve(Eloc, sampleGradients)
+        with np.errstate(divide="ignore", invalid="ignore"):  # jnp semantics: x / 0 -> inf
+            self.crossValidationFactor_residual = np.float64(out[0]) / np.float64(solverResidual)
+            self.crossValidationFactor_tdvpErr = np.float64(validation_tdvpErr) / np.float64(self.metaData["tdvp_error"])
+        self.metaData["tdvp_residual_cross_validation_ratio"] = self.crossValidationFactor_residual
+        self.metaData["tdvp_error_cross_validation_ratio"] = self.crossValidationFactor_tdvpErr
         return update
