@@ -76,7 +76,7 @@ def tfim_strings(op, shape, J=-1.0, h=-3.0):
 
 
 class Workload:
-    def __init__(self, name, mode):
+    def __init__(self, name, mode, precision="fp64"):
         import torch
         import vmc_jax_amd as jv
         self.jv = jv
@@ -85,7 +85,7 @@ class Workload:
         if name == "c3":
             self.shape, self.samples = (10, 10), 8192
             net = jv.nets.CNN(F=(3, 3), channels=(8, 8, 8), strides=(1, 1))
-            psi = jv.NQS(net)
+            psi = jv.NQS(net, precision=precision)
             psi(torch.zeros((1, 1) + self.shape, dtype=torch.int32))
             psi.set_parameters(net.he_normal(psi.desc, rng))
             self.H = tfim_strings(jv.operator, self.shape, -1.0, -3.0)
@@ -161,9 +161,9 @@ KERNELS = ("gram", "gemm", "eigh", "tridiag", "stedc", "ormtr", "trd_apply", "ch
            "net_forward", "sym_mc", "sym_eloc", "sym_combine")
 
 
-def run_gpu(workload, steps, warmup, mode, overlap=True):
+def run_gpu(workload, steps, warmup, mode, overlap=True, precision="fp64"):
     import torch
-    w = Workload(workload, mode)
+    w = Workload(workload, mode, precision)
     if hasattr(w.eq, "overlap"):
         w.eq.overlap = overlap
     jv = w.jv
@@ -189,32 +189,55 @@ def run_gpu(workload, steps, warmup, mode, overlap=True):
     return w, elapsed, kt, energies
 
 
-# HBM bytes per launch (read + written) of the hand-written kernels on this
-# workload, from rocprofv3 FETCH_SIZE / WRITE_SIZE passes scaled by a copy
-# kernel of known size (x2.000 read, x1.000 written); the summary with the
-# commands is profiles/eloc_mc_gram_tfim10x10_cnn_hbm_traffic.md.
-MEASURED_TRAFFIC = {"eloc": 6.203e6 + 0.262e6, "mc_sweep": 0.767e6 + 3.654e6,
-                    "gram": 673.600e6 + 51.710e6 + 50.189e6 + 32.478e6}
+def measured_traffic():
+    """HBM bytes per launch (read + written) of the hand-written kernels on this
+    workload: the `traffic_bytes_per_launch` table of the committed counter
+    summary (rocprofv3 FETCH_SIZE / WRITE_SIZE passes, scaled by a copy kernel
+    of known size), or None when it is absent."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_tfim10x10_cnn.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["traffic_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
 
 
-def roofline_c3(kt, steps, mode):
-    """Roofline of the dominant hand-written kernel, algorithmic counts per step:
-    E_loc 2.03e11 flop (8192 samples x 101 strings x a 244,800-flop forward),
-    Metropolis 3.3e10 flop (133k proposals x 2.45e5), Gram 2 N^2 P (T form) or
-    2 N P^2 (dual) graded on the computed triangle of tiles."""
-    per_step = {"eloc": 2.03e11, "mc_sweep": 3.3e10,
-                "gram": (8192.0 * 8256.0 * 1240.0) if mode == "T" else (1240.0 * 1280.0 * 8192.0),
-                "net_grad": 3 * 244800.0 * 8192}
+# Per step of the 10x10 CNN workload (8192 samples, 101 strings of which 100
+# single flips, 512 chains x 260 proposals):
+#   counted: the reference algorithm's flops (a full 244,800-flop forward per
+#            s' or proposal; Gram 2 N P^2 on the computed triangle of tiles);
+#   executed: the multiply-adds the window kernels actually issue (per sample a
+#            full forward, 244,800 flop, plus per flip the 9 + 25 + 49 window
+#            outputs of 8 channels: 2 x 43,272 flop; per proposal the same
+#            window), without the log-cosh evaluations (~45 FP64 operations
+#            each, 664 per flip) and address arithmetic.
+FLIP_FLOP = 2.0 * (9 * 9 * 8 + 25 * 72 * 8 + 49 * 72 * 8)
+COUNTED = {"eloc": 2.03e11, "mc_sweep": 3.3e10, "net_grad": 3 * 244800.0 * 8192}
+EXECUTED = {"eloc": 8192 * (244800.0 + 100 * FLIP_FLOP), "mc_sweep": 512 * 260 * FLIP_FLOP,
+            "net_grad": 3 * 244800.0 * 8192}
+BOUND = {"eloc": "valu", "mc_sweep": "valu", "net_grad": "valu", "gram": "mfma"}
+
+
+def roofline_c3(kt, steps, mode, traffic=None):
+    """Roofline of the dominant hand-written kernel (largest time per step).
+    `achieved` uses the counted flops (the metric's algorithmic count);
+    `executed_frac` the flops the kernel issues.  The window kernels run on the
+    FP64 VALU (bound "valu"), whose peak equals the FP64 MFMA peak on MI355X
+    (78.6 TFLOP/s).  `traffic`: HBM bytes per launch from the committed
+    FETCH_SIZE / WRITE_SIZE counter passes of this workload."""
+    per_step = dict(COUNTED)
+    per_step["gram"] = (8192.0 * 8256.0 * 1240.0) if mode == "T" else (1240.0 * 1280.0 * 8192.0)
     cands = {k: v for k, v in kt.items() if k in per_step and v[1] > 0}
     dom = max(cands, key=lambda k: cands[k][0])
     total_ms, launches = cands[dom]
     ms = total_ms / launches
     flops = per_step[dom] * steps / launches
     ach = flops / (ms * 1e-3) / 1e12
-    return {"bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": ach / FP64_PEAK_TFLOPS, "traffic": MEASURED_TRAFFIC.get(dom) if mode == "dual" else None,
-            "kernel": dom, "avg_launch_ms": ms,
-            "launches": launches}
+    exe = EXECUTED.get(dom, per_step[dom]) * steps / launches / (ms * 1e-3) / 1e12
+    return {"bound": BOUND.get(dom, "mfma"), "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": ach / FP64_PEAK_TFLOPS, "executed_tflops": exe, "executed_frac": exe / FP64_PEAK_TFLOPS,
+            "traffic": (traffic or {}).get(dom) if mode == "dual" else None,
+            "kernel": dom, "avg_launch_ms": ms, "launches": launches}
 
 
 def batched_sweep(num_instances=64, steps=2, slots=16, seed=11):
@@ -332,13 +355,26 @@ def serve_mix(num_jobs=16, slots=8, seed=7, rate=8.0):
     return rep
 
 
-def cpu_baseline(n_sub=1024, chains=512):
-    """FP64 oracle timed on a bounded sample of the C3 step: sampling (full
-    network re-evaluation per proposal), local energies with materialised s',
-    gradients and the T-form MinSR solve on n_sub samples from `chains` chains;
-    each phase is scaled to 8192 samples by its complexity (sweeps by samples per
-    chain, E_loc / gradients linearly, the T Gram quadratically, the
-    eigendecomposition cubically)."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(sizes=(64, 128), chains=32, reps=3):
+    """FP64 oracle on the host cores, timed on bounded samples of the 10x10 CNN
+    step: per phase one warm-up and the median of `reps` runs at two sample
+    counts, the scaling exponent fitted from the two sizes, and the phase
+    extrapolated to 8192 samples with the algorithm's exponent (sampling:
+    linear in chains x sweeps, here 512 chains of 10 + 16 sweeps of 100
+    proposals from `chains` chains; E_loc and gradients: linear; T Gram:
+    quadratic; eigendecomposition: cubic).  numpy's BLAS uses every core of the
+    process's affinity set."""
     from oracle import nets as onets, operator as oop, sampler as osamp
     from oracle.stats import SampledObs
     shape, samples, nchains = (10, 10), 8192, 512
@@ -351,54 +387,118 @@ def cpu_baseline(n_sub=1024, chains=512):
             fan_in = leaf.shape[0] * leaf.shape[1] * leaf.shape[2]
             P[off:off + leaf.size] = rng.normal(size=leaf.size) * math.sqrt(2.0 / fan_in)
         off += leaf.size
-    psi = onets.NQS(onet, P, batchSize=4096)
+    psi = onets.NQS(onet, P, batchSize=64)
     H = tfim_strings(oop, shape)
-    spc_full = samples // nchains
-    spc_sub = max(1, n_sub // chains)
-    ms = osamp.MCSampler(psi, shape, 0, numChains=chains, thermalizationSweeps=10, sweepSteps=10)
-    t = {}
-    t0 = time.perf_counter(); s, lp, p = ms.sample(numSamples=spc_sub * chains); t["sampling"] = time.perf_counter() - t0
-    t0 = time.perf_counter(); H.get_O_loc(s, psi, lp, 0.0); t["eloc"] = time.perf_counter() - t0
-    t0 = time.perf_counter(); G = psi.gradients(s); t["grad"] = time.perf_counter() - t0
-    D = np.real(SampledObs(G, p)._data)
-    t0 = time.perf_counter(); T = D @ D.T; t["gram"] = time.perf_counter() - t0
-    t0 = time.perf_counter(); np.linalg.eigh(T); t["eigh"] = time.perf_counter() - t0
-    n = len(p)
-    scale_n = samples / n
-    sw = (10 + spc_full) * 10 / ((10 + spc_sub) * 10) * nchains / chains
-    est = t["sampling"] * sw + (t["eloc"] + t["grad"]) * scale_n + t["gram"] * scale_n ** 2 + t["eigh"] * scale_n ** 3
+
+    def med(fn):
+        fn()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts))
+
+    t0 = time.perf_counter()
+    meas = {}
+    for n in sizes:
+        ms = osamp.MCSampler(psi, shape, 0, numChains=chains, thermalizationSweeps=10, sweepSteps=10)
+        spc = n // chains
+        box = {}
+
+        def samp():
+            box["s"] = ms.sample(numSamples=spc * chains)
+        t = {"sampling": med(samp)}
+        s, lp, p = box["s"]
+        t["eloc"] = med(lambda: H.get_O_loc(s, psi, lp, 0.0))
+        t["grad"] = med(lambda: psi.gradients(s))
+        D = np.real(SampledObs(psi.gradients(s), p)._data)
+        t["gram"] = med(lambda: D @ D.T)
+        T = D @ D.T
+        t["eigh"] = med(lambda: np.linalg.eigh(T))
+        meas[n] = t
+    n1, n2 = sizes
+    expo = {"sampling": 1.0, "eloc": 1.0, "grad": 1.0, "gram": 2.0, "eigh": 3.0}
+    fitted = {k: math.log(max(meas[n2][k], 1e-9) / max(meas[n1][k], 1e-9)) / math.log(n2 / n1) for k in expo}
+    est = {}
+    for k, a in expo.items():
+        if k == "sampling":  # chains x (therm + samples per chain) proposal sweeps
+            work_full = nchains * (10 + samples // nchains)
+            work_sub = chains * (10 + n2 // chains)
+            est[k] = meas[n2][k] * work_full / work_sub
+        else:
+            est[k] = meas[n2][k] * (samples / n2) ** a
+    total = sum(est.values())
     cores = len(os.sched_getaffinity(0))
-    return {"value": 1.0 / est, "unit": "steps/s", "cores": cores, "kind": "port",
-            "sample": ("FP64 numpy oracle, one MinSR step on %d samples from %d chains (10x10 TFIM, same CNN); "
-                       "phases scaled to 8192 samples (sampling x%.2f, E_loc+gradients x%.0f, T Gram x%.0f, "
-                       "eigh x%.0f); measured s: %s" % (n, chains, sw, scale_n, scale_n ** 2, scale_n ** 3,
-                                                         {k: round(v, 3) for k, v in t.items()}))}
+    return {"value": 1.0 / total, "unit": "steps/s", "cores": cores, "kind": "port",
+            "cpu_model": _cpu_model(), "threads": cores, "wall_s": time.perf_counter() - t0,
+            "sample": ("jVMC-algorithm CPU restatement (FP64 numpy oracle): one MinSR step of the 10x10 CNN "
+                       "workload on %s samples from %d chains, per phase warm-up + median of %d; extrapolated "
+                       "to 8192 samples / 512 chains with the algorithm's exponents" % (list(sizes), chains, reps)),
+            "phase_s_measured": {str(n): {k: round(v, 4) for k, v in meas[n].items()} for n in sizes},
+            "phase_s_extrapolated": {k: round(v, 3) for k, v in est.items()},
+            "fitted_exponents": {k: round(v, 2) for k, v in fitted.items()}}
 
 
-def accuracy_check(w, n=512):
-    """|dE/E| and the relative error of the MinSR update against the FP64 oracle
-    on the same configurations and weights (n fresh samples of the workload's
-    current state): the accuracy condition of the metric."""
+def accuracy_check(w, n=1536):
+    """|dE/E| and the errors of S, F and the MinSR update against the FP64
+    oracle on the same configurations and weights (n fresh samples of the
+    workload's current state; 1536 = 3 per chain, more than P + 1 so that S
+    can be regular and the Cholesky route can be checked), through the same
+    device path the timed steps use (overlapped dual solve; FP32 O and the
+    compensated-BF16 Gram in the mixed form): the accuracy condition of the
+    metric.  `route` names the solve route the checked step took (cholesky /
+    eigendecomposition)."""
     import torch
     from oracle import nets as onets, operator as oop, solvers as osolv
     from oracle.stats import SampledObs as OObs
     jv = w.jv
-    s, lp, p = w.sampler.sample(numSamples=n)
-    El = jv.stats.SampledObs(w.H.get_O_loc(s, w.psi, lp, 0.0), p)
+
+    class Rec:
+        def __init__(self, smp):
+            self.s, self.last = smp, None
+
+        def sample(self, *a, **k):
+            self.last = self.s.sample(*a, **k)
+            return self.last
+
+    rec = Rec(w.sampler)
+    eq = jv.util.MinSR(rec, pinvTol=1e-10, solver="dual")
+    upd = eq(w.psi.get_parameters(), 0.0, psi=w.psi, hamiltonian=w.H, intStep=0, numSamples=n).cpu().numpy()
+    route = "cholesky" if eq.last_info.iterations == 1 else "eigendecomposition"
+    e_gpu = complex(eq.ElocMean0)
+    s, lp, p = rec.last
+    # S and F of the device data (FP32 in the mixed form) beside the oracle's
     G = jv.stats.SampledObs(w.psi.gradients(s, layout=0), p)
-    upd = jv.util.MinSR(None, pinvTol=1e-10, solver="dual").solve(El, G, False).cpu().numpy()
-    e_gpu = complex(El.mean()[0].item())
+    El = jv.stats.SampledObs(w.H.get_O_loc(s, w.psi, lp, 0.0), p)
+    D = G._data
+    nn, P = D.shape
+    S = torch.empty((P, P), dtype=torch.float64, device=D.device)
+    if D.dtype == torch.float32:
+        jv._lib.call("jv_gram_pp_mixed", jv._lib.ptr(D), 1, nn, P, 0, jv._lib.ptr(S), jv._lib.stream_ptr())
+    else:
+        jv._lib.call("jv_gram_pp", jv._lib.ptr(D), nn, P, 0, jv._lib.ptr(S), jv._lib.stream_ptr())
+    F = (D.to(torch.float64).T @ El._data.real.reshape(-1).to(torch.float64)).cpu().numpy()
+    S = S.cpu().numpy()
     onet = onets.CNN(w.shape, F=(3, 3), channels=(8, 8, 8), strides=(1, 1))
-    opsi = onets.NQS(onet, w.psi.get_parameters().cpu().numpy(), batchSize=4096)
+    opsi = onets.NQS(onet, w.psi.get_parameters().cpu().numpy(), batchSize=64)
     cs = s[0].cpu().numpy()
     pw = p[0].cpu().numpy()
     Ho = tfim_strings(oop, w.shape)
     Eo = OObs(Ho.get_O_loc(cs, opsi, opsi(cs), 0.0), pw)
-    ref = osolv.MinSR(None, pinvTol=1e-10).solve(Eo, OObs(opsi.gradients(cs), pw), False)
+    Go = OObs(opsi.gradients(cs), pw)
+    ref = osolv.MinSR(None, pinvTol=1e-10).solve(Eo, Go, False)
+    Do, eo = np.real(Go._data), np.real(Eo._data).reshape(-1)
+    So, Fo = Do.T @ Do, Do.T @ eo
     e_o = complex(Eo.mean()[0])
-    return {"samples": int(n), "against": "FP64 numpy oracle (T-form MinSR) on the same configurations and weights",
+    res, res_o = np.linalg.norm(Do @ upd + eo), np.linalg.norm(Do @ ref + eo)
+    return {"samples": int(n), "route": route,
+            "against": "FP64 numpy oracle (T-form MinSR) on the same configurations and weights",
             "energy": e_gpu.real, "dE_rel": abs(e_gpu - e_o) / abs(e_o),
-            "dtheta_rel": float(np.linalg.norm(upd - ref) / np.linalg.norm(ref))}
+            "dS_rel": float(np.linalg.norm(S - So) / np.linalg.norm(So)),
+            "dF_rel": float(np.linalg.norm(F - Fo) / np.linalg.norm(Fo)),
+            "dtheta_rel": float(np.linalg.norm(upd - ref) / np.linalg.norm(ref)),
+            "residual_rel": float(res / res_o - 1.0)}
 
 
 def main():
@@ -413,6 +513,7 @@ def main():
     ap.add_argument("--no-overlap", action="store_true",
                     help="c3: compute E_loc before the MinSR solve instead of beside it (isolated kernel timings)")
     ap.add_argument("--serve-jobs", type=int, default=16, help="c3: jobs of the served mix (0 = skip)")
+    ap.add_argument("--no-mixed", action="store_true", help="c3: skip the mixed-precision form")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="end with an elementwise 512 MiB -> 512 MiB FP64 device copy (a kernel of known "
                          "HBM traffic that scales the FETCH_SIZE / WRITE_SIZE counters)")
@@ -458,7 +559,21 @@ def main():
     if args.workload == "c3":
         out["config"]["solver_form"] = args.mode
         out["config"]["eloc_beside_solve"] = not args.no_overlap and args.mode == "dual"
-        out["roofline"] = roofline_c3(kt, args.steps, args.mode)
+        out["roofline"] = roofline_c3(kt, args.steps, args.mode, measured_traffic())
+        if args.mode == "dual" and not args.no_mixed:
+            # the mixed-precision form from the same initial state, same steps
+            wm, em, ktm, enm = run_gpu("c3", args.steps, args.warmup, "dual", not args.no_overlap, "mixed")
+            out["mixed_form"] = {
+                "value": args.steps / em, "unit": "steps/s", "ms_per_step": 1000.0 * em / args.steps,
+                "samples_per_s": args.steps / em * wm.samples,
+                "dtype": "fp32 network kernels + bf16x3 (compensated) Gram, fp64 solve",
+                "kernel_ms_per_step": {k: v[0] / args.steps for k, v in ktm.items() if v[1] > 0},
+                "minsr_cholesky_steps": wm.cholesky_steps, "energy_last": enm[-1] if enm else None}
+            if not args.no_cpu_baseline:
+                try:
+                    out["mixed_form"]["accuracy"] = accuracy_check(wm)
+                except Exception as e:
+                    out["mixed_form"]["accuracy"] = {"error": repr(e)}
         if args.t_steps > 0:
             om = "T" if args.mode == "dual" else "dual"
             _, e2, kt2, _ = run_gpu("c3", args.t_steps, 1, om)

@@ -41,6 +41,14 @@ extern "C" {
 
 #define JV_MAX_LAYERS 8
 
+/* Arithmetic of the network kernels (jv_net.precision).  FP64: the reference's
+ * precision throughout.  MIXED: the CNN window kernels (Metropolis and local
+ * energy) evaluate activations in FP32 with FP64 accumulation of log psi
+ * differences and of O_loc, and jv_net_gradients (layout 0) writes O in FP32
+ * (jv_net_gradients_f32); kernels without an FP32 form run in FP64. */
+#define JV_PREC_FP64 0
+#define JV_PREC_MIXED 1
+
 /* Network descriptor.  The caller fills the "input" fields; jv_net_init()
  * validates them and fills the "derived" fields (parameter layout).  The flat
  * real parameter vector follows jVMC's layout (vqs.py:430-466): leaves in JAX
@@ -58,6 +66,7 @@ typedef struct jv_net {
   int32_t filter[2];                  /* CNN: filter extents F (cnn.py:122) */
   int32_t channels[JV_MAX_LAYERS];    /* CNN: output channels per layer */
   int32_t layer_bias[JV_MAX_LAYERS];  /* CNN: bias per layer (cnn.py:144-145) */
+  int32_t precision;                  /* JV_PREC_FP64 (0) or JV_PREC_MIXED (extension, not in jVMC) */
   /* derived (jv_net_init) */
   int32_t num_sites;                  /* shape[0] * shape[1] */
   int32_t is_complex;                 /* complex (holomorphic) parameters */
@@ -184,6 +193,12 @@ int jv_net_forward(const jv_net* net, const double* params, const int32_t* confi
 int jv_net_gradients(const jv_net* net, const double* params, const int32_t* configs,
                      int64_t n, int32_t layout, double* grad, double* logpsi, void* stream);
 
+/* jv_net_gradients in the mixed-precision form: O (layout 0, real [n][num_params])
+ * in FP32 from FP32 activations; logpsi (FP64 sum, may be NULL).  CNN only.
+ * Replaces NQS.gradients (vqs.py:256-287) in the mixed form. */
+int jv_net_gradients_f32(const jv_net* net, const double* params, const int32_t* configs, int64_t n, float* grad,
+                         double* logpsi, void* stream);
+
 /* ------------------------------------------------------------------ symmetrised networks */
 /* log psi(s) = avg_g log psi_net(g s) (complex [n]).  EXP: log(sum_g b_g
  * exp(c_g)) with the jax logsumexp conventions: entries with b_g = 0 are
@@ -272,6 +287,13 @@ int jv_mc_weights(const double* logpsi, int64_t n, double mu, double log_prob_fa
 int jv_center(const double* obs, const double* w, int64_t n, int64_t p, int32_t is_complex,
               double* mean, double* data, void* stream);
 
+/* jv_center for FP32 observations (the mixed form): the weighted mean is
+ * accumulated and returned in FP64 (mean: [p] real or complex doubles), the
+ * centred data are rounded to FP32.  Replaces SampledObs._compute_data_and_mean
+ * (stats.py:197-205) in the mixed form. */
+int jv_center_f32(const float* obs, const double* w, int64_t n, int64_t p, int32_t is_complex, double* mean,
+                  float* data, void* stream);
+
 /* Column sums of squares: out[j] = sum_n |data[n][j]|^2.
  * Replaces SampledObs.var (stats.py:248-252). */
 int jv_col_norm2(const double* data, int64_t n, int64_t p, int32_t is_complex, double* out, void* stream);
@@ -301,6 +323,17 @@ int jv_gram_nn(const double* data, int64_t n, int64_t p, int32_t is_complex, dou
  * Replaces SampledObs.covar (stats.py:235-245) for the S matrix. */
 int jv_gram_pp(const double* data, int64_t n, int64_t p, int32_t is_complex, double* S,
                void* stream);
+
+/* Mixed-precision covariance S = D^dagger D ([p][p], FP64 output) of centred
+ * data D [n][p] stored in FP32 (data_f32 = 1) or FP64 (0), complex data
+ * interleaved.  Products on the BF16 matrix cores with compensated splits
+ * (x = bf16(x) + bf16(x - bf16(x)), three products per multiply, FP32
+ * accumulation within slices of the sample axis, slices summed in FP64):
+ * relative error of each entry ~1e-5 of sum_n |D_ni| |D_nj|.
+ * Replaces SampledObs.covar (stats.py:235-245) / the Gram of minsr.py:67-78
+ * in the mixed-precision form. */
+int jv_gram_pp_mixed(const void* data, int32_t data_f32, int64_t n, int64_t p, int32_t is_complex, double* S,
+                     void* stream);
 
 /* Symmetric / Hermitian eigendecomposition (ascending eigenvalues) in FP64:
  * for n >= 128 own Householder tridiagonalisation kernels, the tridiagonal
@@ -369,6 +402,15 @@ int jv_minsr_solve_dual(const double* data, const double* e, int64_t n, int64_t 
 int jv_minsr_solve_dual_deferred(const double* data, const double* e_raw, const double* w, void* e_ready,
                                  int64_t n, int64_t p, int32_t is_complex, double rcond, double diag_shift,
                                  int32_t try_cholesky, double* update, jv_solve_info* info, void* stream);
+
+/* jv_minsr_solve_dual_deferred in the mixed-precision form: centred data D in
+ * FP32 (real only, is_complex must be 0), S = D^T D from the compensated BF16
+ * Gram (jv_gram_pp_mixed) and b = D^T e accumulated in FP64; the factorisation
+ * or eigendecomposition, the pseudo-inverse and the E_loc statistics stay in
+ * FP64.  No separate reference function (minsr.py:53-80 in FP64). */
+int jv_minsr_solve_dual_deferred_mixed(const float* data, const double* e_raw, const double* w, void* e_ready,
+                                       int64_t n, int64_t p, int32_t is_complex, double rcond, double diag_shift,
+                                       int32_t try_cholesky, double* update, jv_solve_info* info, void* stream);
 
 /* TDVP solve (tdvp.py:134-213).  data: centred O in jVMC's flat layout, complex
  * [n][pr]; e: centred E_loc, complex [n]; w: weights [n]; x = rhsPrefactor

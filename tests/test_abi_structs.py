@@ -15,7 +15,7 @@ SRC = r"""
 int main(void) {
   printf("jv_net %zu\njv_optable %zu\njv_mc_args %zu\njv_solve_info %zu\njv_orbit %zu\n", sizeof(jv_net),
          sizeof(jv_optable), sizeof(jv_mc_args), sizeof(jv_solve_info), sizeof(jv_orbit));
-  P(jv_net, num_sites); P(jv_net, num_params); P(jv_net, off_conv_kernel); P(jv_net, off_conv_bias);
+  P(jv_net, precision); P(jv_net, num_sites); P(jv_net, num_params); P(jv_net, off_conv_kernel); P(jv_net, off_conv_bias);
   P(jv_optable, max_sites_changed); P(jv_optable, idx); P(jv_optable, diag); P(jv_optable, fermi); P(jv_mc_args, mu); P(jv_mc_args, seed); P(jv_mc_args, stream);
   P(jv_solve_info, rank); P(jv_solve_info, e_var); P(jv_solve_info, tdvp_error); P(jv_solve_info, e_mean);
   P(jv_orbit, avg); P(jv_orbit, complex_factor); P(jv_orbit, map); P(jv_orbit, factor);

@@ -219,3 +219,75 @@ def test_cholesky_route_concurrent_streams():
     for (u, info), (ua, infoa) in zip(results, alone):
         assert info.iterations == infoa.iterations == 1 and info.rank == n - 1
         assert np.array_equal(u, ua)
+
+
+def _with_spectrum(rng, n, lam, centred):
+    """D [n][p] (p = len(lam) + extra columns of zeros) whose T = D D^T has the
+    eigenvalues lam on an orthonormal basis; centred: the basis is orthogonal
+    to u0 = 1/sqrt(n), so u0 is an exact null vector."""
+    k = len(lam)
+    M = rng.normal(size=(n, k))
+    if centred:
+        M -= M.mean(0)
+    U, _ = np.linalg.qr(M)
+    V, _ = np.linalg.qr(rng.normal(size=(k + 40, k)))
+    return (U * np.sqrt(lam)) @ V.T
+
+
+@pytest.mark.parametrize("factor", [0.5, 1.01, 2.0, 10.0, 1e3])
+def test_t_form_near_cutoff_matches_pinv(factor):
+    """The smallest kept eigenvalue of the centred T placed at `factor` times the
+    cutoff pinvTol * lambda_max: the solve either takes the route (and keeps
+    exactly the reference's eigenvalues) or diagonalises; both give the
+    pseudo-inverse update."""
+    rng = np.random.default_rng(int(factor * 100))
+    n, rcond = 200, 1e-8
+    lam = np.logspace(0, -4, n - 1)
+    lam[-1] = factor * rcond * lam[0]
+    D = _with_spectrum(rng, n, lam, True)
+    e = rng.normal(size=n) + 1j * rng.normal(size=n)
+    u, info = _solve_t(D, e, rcond)
+    ref, rank = _reference_t(D, e, rcond)
+    print("near cutoff", factor, info.iterations, info.rank, rank, flush=True)
+    assert info.rank == rank
+    assert _rel(u, ref) < 1e-5
+    if factor <= 1.01:
+        assert info.iterations == 0  # a cut (or marginal) eigenvalue: no certificate
+
+
+@pytest.mark.parametrize("eta", [1e-9, 1e-6])
+def test_t_form_perturbed_null_vector_not_deflated(eta):
+    """Data whose rows do not sum to zero exactly (||T u0|| far above rounding):
+    u0 is no eigenvector to working precision, the deflation is refused and
+    the eigendecomposition gives the pseudo-inverse update."""
+    rng = np.random.default_rng(int(1 / eta) % 1000)
+    n, rcond = 200, 1e-8
+    lam = np.logspace(0, -3, n - 1)
+    D = _with_spectrum(rng, n, lam, True)
+    D = D + eta * rng.normal(size=(1, D.shape[1]))  # the same small offset on every row
+    e = rng.normal(size=n) + 1j * rng.normal(size=n)
+    u, info = _solve_t(D, e, rcond)
+    ref, rank = _reference_t(D, e, rcond)
+    assert info.iterations == 0 and info.rank == rank
+    assert _rel(u, ref) < 1e-6
+
+
+@pytest.mark.parametrize("factor", [0.5, 1.01, 2.0, 1e3])
+def test_dual_near_cutoff_matches_pinv(factor):
+    """The dual form with the smallest eigenvalue of S = D^T D at `factor` times
+    the cutoff: the route is taken only with a certificate; the update is the
+    pseudo-inverse's either way."""
+    rng = np.random.default_rng(int(factor * 10) + 3)
+    n, p, rcond = 600, 250, 1e-8
+    U, _ = np.linalg.qr(rng.normal(size=(n, p)))
+    V, _ = np.linalg.qr(rng.normal(size=(p, p)))
+    sv = np.sqrt(np.logspace(0, -4, p))
+    sv[-1] = np.sqrt(factor * rcond)
+    D = (U * sv) @ V.T
+    e = rng.normal(size=n) + 0j
+    u, info = _solve(D, e, rcond)
+    ref, rank = _reference(D, e, rcond)
+    assert info.rank == rank
+    assert _rel(u, ref) < 1e-5
+    if factor <= 1.01:
+        assert info.iterations == 0

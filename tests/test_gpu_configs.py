@@ -99,13 +99,20 @@ def test_time_evolution_mc_zz_golden():
 def test_tdvp_cross_validation_matches_oracle(mode, shift):
     """TDVP.__call__ with crossValidation (tdvp.py:306-324) on Metropolis samples
     identical to the oracle's (bit-exact sampler): update, tdvp_error, residual
-    and both cross-validation ratios.  A real RBM keeps the spectrum
-    non-degenerate, so the SNR-regularised update is unique."""
+    and both cross-validation ratios.  The spectrum stays non-degenerate, so the
+    SNR-regularised update is unique: 'real' uses a real RBM; 'imag' a CpxRBM
+    (S = i Im S0 has the eigenvalues +-lambda(G^H G); a real network's
+    Im S0 vanishes, leaving S = 0 and a NaN update in the reference too)."""
     rng = np.random.default_rng(41)
     L = 10
-    onet = onets.RBM((L,), numHidden=6, bias=True)
+    if mode == "imag":
+        onet = onets.CpxRBM((L,), numHidden=6, bias=True)
+        dnet = jv.nets.CpxRBM(numHidden=6, bias=True)
+    else:
+        onet = onets.RBM((L,), numHidden=6, bias=True)
+        dnet = jv.nets.RBM(numHidden=6, bias=True)
     P = 0.3 * rng.normal(size=onet.num_real_params())
-    psi = make_psi(jv.nets.RBM(numHidden=6, bias=True), (L,), P)
+    psi = make_psi(dnet, (L,), P)
     opsi = onets.NQS(onet, P)
     H, Ho = tfim_pair((L,), -1.0, -0.7)
     x = 1.j if mode == "imag" else 1.0

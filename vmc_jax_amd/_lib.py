@@ -19,6 +19,8 @@ JV_NET_CPXRBM = 1
 JV_NET_RBM = 2
 JV_NET_CNN = 3
 JV_MAX_LAYERS = 8
+JV_PREC_FP64 = 0
+JV_PREC_MIXED = 1
 
 _i32 = ctypes.c_int32
 _i64 = ctypes.c_int64
@@ -30,7 +32,7 @@ class JvNet(ctypes.Structure):
     _fields_ = [
         ("kind", _i32), ("ndim", _i32), ("shape", _i32 * 2), ("num_hidden", _i32),
         ("hidden_bias", _i32), ("visible_bias", _i32), ("num_layers", _i32), ("filter", _i32 * 2),
-        ("channels", _i32 * JV_MAX_LAYERS), ("layer_bias", _i32 * JV_MAX_LAYERS),
+        ("channels", _i32 * JV_MAX_LAYERS), ("layer_bias", _i32 * JV_MAX_LAYERS), ("precision", _i32),
         ("num_sites", _i32), ("is_complex", _i32), ("num_params", _i64), ("num_real_params", _i64),
         ("off_hidden_bias", _i64), ("off_kernel", _i64), ("off_visible_bias", _i64),
         ("off_conv_kernel", _i64 * JV_MAX_LAYERS), ("off_conv_bias", _i64 * JV_MAX_LAYERS),
@@ -76,6 +78,7 @@ _SIGS = {
     "jv_net_init": [ctypes.POINTER(JvNet)],
     "jv_net_forward": [ctypes.POINTER(JvNet), _vp, _vp, _i64, _vp, _vp],
     "jv_net_gradients": [ctypes.POINTER(JvNet), _vp, _vp, _i64, _i32, _vp, _vp, _vp],
+    "jv_net_gradients_f32": [ctypes.POINTER(JvNet), _vp, _vp, _i64, _vp, _vp, _vp],
     "jv_sym_forward": [ctypes.POINTER(JvNet), _vp, ctypes.POINTER(JvOrbit), _vp, _i64, _vp, _vp],
     "jv_sym_gradients": [ctypes.POINTER(JvNet), _vp, ctypes.POINTER(JvOrbit), _vp, _i64, _i32, _vp, _vp, _vp],
     "jv_sym_eloc": [ctypes.POINTER(JvNet), _vp, ctypes.POINTER(JvOrbit), ctypes.POINTER(JvOpTable), _vp, _vp, _vp,
@@ -88,9 +91,11 @@ _SIGS = {
     "jv_exact_weights": [_vp, _i64, _dbl, _dbl, _vp, ctypes.POINTER(_dbl), _vp],
     "jv_mc_weights": [_vp, _i64, _dbl, _dbl, _vp, _vp],
     "jv_center": [_vp, _vp, _i64, _i64, _i32, _vp, _vp, _vp],
+    "jv_center_f32": [_vp, _vp, _i64, _i64, _i32, _vp, _vp, _vp],
     "jv_col_norm2": [_vp, _i64, _i64, _i32, _vp, _vp],
     "jv_gram_nn": [_vp, _i64, _i64, _i32, _vp, _vp],
     "jv_gram_pp": [_vp, _i64, _i64, _i32, _vp, _vp],
+    "jv_gram_pp_mixed": [_vp, _i32, _i64, _i64, _i32, _vp, _vp],
     "jv_gemm": [_i64, _i64, _i64, _vp, _i64, _i64, _i32, _vp, _i64, _i64, _i32, _vp, _i64, _i32, _dbl, _vp],
     "jv_gemv": [_i32, _vp, _vp, _i64, _i64, _i32, _i32, _dbl, _vp, _vp],
     "jv_eigh": [_i64, _i32, _vp, _vp, _vp],
@@ -100,6 +105,8 @@ _SIGS = {
     "jv_minsr_solve_dual_deferred": [_vp, _vp, _vp, _vp, _i64, _i64, _i32, _dbl, _dbl, _i32, _vp,
                                      ctypes.POINTER(JvSolveInfo),
                                      _vp],
+    "jv_minsr_solve_dual_deferred_mixed": [_vp, _vp, _vp, _vp, _i64, _i64, _i32, _dbl, _dbl, _i32, _vp,
+                                           ctypes.POINTER(JvSolveInfo), _vp],
     "jv_tdvp_solve": [_vp, _vp, _vp, _i64, _i64, ctypes.POINTER(_dbl), _i32, _dbl, _i32, _dbl, _dbl, _dbl,
                       _dbl, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(JvSolveInfo), _vp],
     "jv_tdvp_solve_holomorphic": [_vp, _vp, _vp, _i64, _i64, _vp, _vp, _i64, ctypes.POINTER(_dbl), _i32, _dbl, _i32,

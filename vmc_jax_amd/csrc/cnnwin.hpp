@@ -22,6 +22,15 @@ namespace jv {
 // loads below compile to ds_read instead of flat loads.
 typedef const __attribute__((address_space(3))) double ldbl;
 __device__ inline ldbl* as_lds(const double* p) { return (ldbl*)p; }
+typedef const __attribute__((address_space(3))) float ldflt;
+__device__ inline ldflt* as_lds(const float* p) { return (ldflt*)p; }
+template <typename T>
+using lds_t = const __attribute__((address_space(3))) T;
+template <typename T>
+using lds_w = __attribute__((address_space(3))) T;
+// Wave-uniform weights through the constant address space (scalar loads).
+template <typename T>
+using cst_t = const __attribute__((address_space(4))) T;
 
 // Layer l of the packed weights: kernel ((k*G + gi)*cinp + ci)*CG + cg for
 // output channel c = gi*CG + cg (CG = CM/G, cinp = 1 for the first layer and
@@ -50,12 +59,15 @@ inline CnnPack make_cnn_pack(const CnnGeom& g, int cm, int groups) {
   return p;
 }
 
-// Writes the packed weights of params (jVMC flat layout) to out[pk.total].
-__global__ void cnn_pack(CnnGeom g, CnnPack pk, const double* __restrict__ Pg, double* __restrict__ out);
+// Writes the packed weights of params (jVMC flat layout) to out[pk.total]
+// (FP64, or rounded to FP32 for the mixed-precision kernels).
+template <typename T>
+__global__ void cnn_pack(CnnGeom g, CnnPack pk, const double* __restrict__ Pg, T* __restrict__ out);
 
 // Launches cnn_pack into workspace slot `slot`; returns the packed buffer or
 // nullptr (error set).
-double* pack_cnn_weights(const CnnGeom& g, const CnnPack& pk, const double* params, hipStream_t st, int slot);
+template <typename T>
+T* pack_cnn_weights(const CnnGeom& g, const CnnPack& pk, const double* params, hipStream_t st, int slot);
 
 // Largest channel count rounded up to the kernels' instantiations (0: none).
 int cnn_cmax(const CnnGeom& g);
@@ -65,33 +77,33 @@ int cnn_cmax(const CnnGeom& g);
 // layer l occupies A + l*lstride as CM planes of cstride doubles (positions
 // [0, n)).  All CM planes are written (padded channels hold log_cosh(0));
 // callers sum only the real channels of the last layer.
-template <int CM, int G, typename Sync>
-__device__ inline void win_forward(const CnnGeom& g, const cdbl* __restrict__ Wp, const CnnPack& pk,
-                                   const double* x0, double* A, int lstride, int cstride, const short* nbr,
+template <int CM, int G, typename T, typename Sync>
+__device__ inline void win_forward(const CnnGeom& g, const cst_t<T>* __restrict__ Wp, const CnnPack& pk,
+                                   const T* x0, T* A, int lstride, int cstride, const short* nbr,
                                    int tid, int nt, Sync sync) {
   for (int l = 0; l < g.nl; ++l) {
-    const cdbl* K = Wp + pk.koff[l];
-    const cdbl* B = Wp + pk.boff[l];
-    ldbl* X = as_lds(l == 0 ? x0 : A + (l - 1) * lstride);
-    ldbl* X0 = as_lds(x0);
-    double* Al = A + l * lstride;
+    const cst_t<T>* K = Wp + pk.koff[l];
+    const cst_t<T>* B = Wp + pk.boff[l];
+    lds_t<T>* X = (lds_t<T>*)(l == 0 ? x0 : A + (l - 1) * lstride);
+    lds_t<T>* X0 = (lds_t<T>*)x0;
+    T* Al = A + l * lstride;
     for (int p = tid; p < g.n; p += nt) {
-      double z[CM];
+      T z[CM];
 #pragma unroll
       for (int c = 0; c < CM; ++c) z[c] = B[c];
       for (int k = 0; k < g.FF; ++k) {
         const int q = nbr[p * g.FF + k];
         constexpr int CG = CM / G;
         if (l == 0) {
-          const double xv = X0[q];
-          const cdbl* w = K + k * CM;
+          const T xv = X0[q];
+          const cst_t<T>* w = K + k * CM;
 #pragma unroll
           for (int c = 0; c < CM; ++c) z[c] = fma(xv, w[(c / CG) * CG + c % CG], z[c]);
         } else {
-          const cdbl* w = K + k * CM * CM;
+          const cst_t<T>* w = K + k * CM * CM;
 #pragma unroll
           for (int ci = 0; ci < CM; ++ci) {
-            const double xv = X[ci * cstride + q];
+            const T xv = X[ci * cstride + q];
 #pragma unroll
             for (int c = 0; c < CM; ++c) z[c] = fma(xv, w[((c / CG) * CM + ci) * CG + c % CG], z[c]);
           }
@@ -117,10 +129,10 @@ __device__ inline int win_site(const CnnGeom& g, int si, int sj, int a, int b) {
 // element (a, b) of layer l >= 1 after flipping (si, sj).  Inputs inside the
 // layer-(l-1) window come from Xw ([ci][pwsz] planar), the others from the
 // stored activations Xb ([ci][n] planar).
-template <int CM>
-__device__ inline void win_preact(const CnnGeom& g, const cdbl* __restrict__ K, const cdbl* __restrict__ B,
-                                  const double* Xb, const double* Xw, int pex, int pey, int si, int sj, int a,
-                                  int b, double* z) {
+template <int CM, typename T>
+__device__ inline void win_preact(const CnnGeom& g, const cst_t<T>* __restrict__ K, const cst_t<T>* __restrict__ B,
+                                  const T* Xb, const T* Xw, int pex, int pey, int si, int sj, int a,
+                                  int b, T* z) {
   const int pwsz = pex * pey;
 #pragma unroll
   for (int c = 0; c < CM; ++c) z[c] = B[c];
@@ -134,12 +146,12 @@ __device__ inline void win_preact(const CnnGeom& g, const cdbl* __restrict__ K, 
       int qj = sj - bq;
       qj = qj < 0 ? qj + g.Ly : (qj >= g.Ly ? qj - g.Ly : qj);
       const bool in = rin && bq >= 0 && bq < pey;
-      ldbl* x = as_lds(in ? Xw + (aq * pey + bq) : Xb + (qi * g.Ly + qj));
+      lds_t<T>* x = (lds_t<T>*)(in ? Xw + (aq * pey + bq) : Xb + (qi * g.Ly + qj));
       const int xs = in ? pwsz : g.n;
-      const cdbl* w = K + (ky * g.Fy + kx) * CM * CM;
+      const cst_t<T>* w = K + (ky * g.Fy + kx) * CM * CM;
 #pragma unroll
       for (int ci = 0; ci < CM; ++ci) {
-        const double xv = x[ci * xs];
+        const T xv = x[ci * xs];
 #pragma unroll
         for (int c = 0; c < CM; ++c) z[c] = fma(xv, w[ci * CM + c], z[c]);
       }
@@ -148,10 +160,10 @@ __device__ inline void win_preact(const CnnGeom& g, const cdbl* __restrict__ K, 
 }
 
 // Layer-0 version: one input channel, the flipped site reads xflip.
-template <int CM>
-__device__ inline void win_preact0(const CnnGeom& g, const cdbl* __restrict__ K, const cdbl* __restrict__ B,
-                                   const double* x0, double xflip, int site, int si, int sj, int a, int b,
-                                   double* z) {
+template <int CM, typename T>
+__device__ inline void win_preact0(const CnnGeom& g, const cst_t<T>* __restrict__ K, const cst_t<T>* __restrict__ B,
+                                   const T* x0, T xflip, int site, int si, int sj, int a, int b,
+                                   T* z) {
 #pragma unroll
   for (int c = 0; c < CM; ++c) z[c] = B[c];
   for (int ky = 0; ky < g.Fx; ++ky) {
@@ -161,8 +173,8 @@ __device__ inline void win_preact0(const CnnGeom& g, const cdbl* __restrict__ K,
       int qj = sj - b + kx;
       qj = qj < 0 ? qj + g.Ly : (qj >= g.Ly ? qj - g.Ly : qj);
       const int q = qi * g.Ly + qj;
-      const double xv = q == site ? xflip : as_lds(x0)[q];
-      const cdbl* w = K + (ky * g.Fy + kx) * CM;
+      const T xv = q == site ? xflip : ((lds_t<T>*)x0)[q];
+      const cst_t<T>* w = K + (ky * g.Fy + kx) * CM;
 #pragma unroll
       for (int c = 0; c < CM; ++c) z[c] = fma(xv, w[c], z[c]);
     }

@@ -145,6 +145,23 @@ __host__ __device__ inline double log_cosh(double x) {
   for (int k = 1; k < 10; ++k) w = fma(w, z, LC_ATANH[k]);
   return (a - LN2) + 2.0 * fma(s * z, w, s);
 }
+// FP32 log cosh of the mixed-precision network kernels:
+// |x| - ln 2 + ln(1 + e^{-2|x|}) with the hardware base-2 exponential and
+// logarithm (v_exp_f32 / v_log_f32); absolute error a few 1e-8 for |x| < 10,
+// e^{-2|x|} underflows to 0 beyond |x| ~ 60, where the tail is below FP32
+// resolution anyway.
+__device__ inline float log_cosh(float x) {
+  const float a = fabsf(x);
+  const float t = __builtin_amdgcn_exp2f(-2.8853900817779268f * a);  // e^{-2a} = 2^{-2a log2 e}
+  return (a - 0.6931471805599453f) + 0.6931471805599453f * __builtin_amdgcn_logf(1.0f + t);
+}
+// FP32 tanh from the same exponential: sign(x) (1 - t) / (1 + t), t = e^{-2|x|}.
+__device__ inline float tanh_f32(float x) {
+  const float t = __builtin_amdgcn_exp2f(-2.8853900817779268f * fabsf(x));
+  const float r = (1.0f - t) / (1.0f + t);
+  return x < 0.0f ? -r : r;
+}
+
 // tanh = d log_cosh / dx, via exp(-2|Re x|) so it never overflows.
 __device__ inline cd ctanh(cd x) {
   double sg = signbit(x.re) ? -1.0 : 1.0;

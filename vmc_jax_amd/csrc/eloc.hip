@@ -4,84 +4,14 @@
 #include <algorithm>
 
 #include "cnnwin.hpp"
+#include "ops.hpp"
+#include "rbm.hpp"
 
 namespace jv {
 
 bool make_rbm_geom(const jv_net* net, RbmGeom& g);
 bool make_cnn_geom(const jv_net* net, CnnGeom& g);
 int set_lds_limit(const void* fn, size_t bytes);
-
-constexpr int MAXLEN = 8;  // longest operator string handled in-kernel
-
-struct OpDev {
-  int num_ops, max_len, num_diag;
-  int L;  // sites per configuration
-  const int32_t* idx;
-  const int32_t* map;
-  const double* matel;
-  const int32_t* diag;
-  const double* pref;
-  const int32_t* fermi;  // [num_ops][max_len] or nullptr
-};
-
-// Applies string o to s (branch_free.py:456-478, factors in stored order) and
-// returns m with the sites whose final value differs from s.
-__device__ inline cd apply_string(const OpDev& op, int o, const int32_t* s, int* sites, int* vals,
-                                  int& nch) {
-  cd m = {op.pref[2 * o], op.pref[2 * o + 1]};
-  int ts[MAXLEN], tv[MAXLEN];
-  int nt = 0;
-  for (int k = 0; k < op.max_len; ++k) {
-    const int base = o * op.max_len + k;
-    // indices are clamped like JAX gathers clamp (site to [0, L), state to [0, 1])
-    const int i = min(max(op.idx[base], 0), op.L - 1);
-    int slot = -1;
-    for (int t = 0; t < nt; ++t)
-      if (ts[t] == i) slot = t;
-    const int cur = min(max(slot >= 0 ? tv[slot] : s[i], 0), 1);
-    cd f = {op.matel[2 * (2 * base + cur)], op.matel[2 * (2 * base + cur) + 1]};
-    m = m * f;
-    if (op.fermi && op.fermi[base]) {
-      // Jordan-Wigner sign: parity of the occupied sites after i in the
-      // configuration this factor acts on (branch_free.py:463-466)
-      int occ = 0;
-      for (int j = i + 1; j < op.L; ++j) occ += s[j];
-      for (int t = 0; t < nt; ++t)
-        if (ts[t] > i) occ += tv[t] - s[ts[t]];
-      if (occ & 1) m = {-m.re, -m.im};
-    }
-    const int nv = min(max(op.map[2 * base + cur], 0), 1);
-    if (slot >= 0) tv[slot] = nv;
-    else { ts[nt] = i; tv[nt] = nv; ++nt; }
-  }
-  nch = 0;
-  for (int t = 0; t < nt; ++t)
-    if (tv[t] != s[ts[t]]) { sites[nch] = ts[t]; vals[nch] = tv[t]; ++nch; }
-  return m;
-}
-
-__device__ inline bool is_diag(const OpDev& op, int o) {
-  for (int d = 0; d < op.num_diag; ++d)
-    if (op.diag[d] == o) return true;
-  return false;
-}
-
-// Matrix element of slot o after the diagonal merge (branch_free.py:483-485):
-// with more than one diagonal string, slot diag[0] carries their sum and the
-// other diagonal slots are zero.
-__device__ inline cd merged_m(const OpDev& op, int o, cd m_raw, cd diag_sum) {
-  if (op.num_diag > 1 && is_diag(op, o)) return o == op.diag[0] ? diag_sum : cd{0.0, 0.0};
-  return m_raw;
-}
-
-__device__ inline cd diag_sum(const OpDev& op, const int32_t* s) {
-  cd acc = {0.0, 0.0};
-  int sites[MAXLEN], vals[MAXLEN], nch;
-  for (int d = 0; d < op.num_diag; ++d) acc = acc + apply_string(op, op.diag[d], s, sites, vals, nch);
-  return acc;
-}
-
-__device__ inline bool nonzero(cd m) { return sqrt(cabs2(m)) > 1e-6; }
 
 // ------------------------------------------------------------------ s' enumeration
 // One thread per sample: merged matrix elements, compaction order (base.py:91-114).
@@ -128,8 +58,7 @@ __global__ void sprime_write(OpDev op, const int32_t* __restrict__ cfg, int64_t 
 }
 
 // ------------------------------------------------------------------ RBM O_loc
-// One wavefront per sample.  theta and log cosh(theta) stay in LDS; each string
-// with |m| > 1e-6 costs one complex log-cosh per hidden unit.
+// One wavefront per sample (eloc_rbm_sample in rbm.hpp).
 __global__ void __launch_bounds__(256) eloc_rbm(RbmGeom g, const double* __restrict__ P, OpDev op,
                                                 const int32_t* __restrict__ cfg,
                                                 const double* __restrict__ lps, int64_t n,
@@ -138,70 +67,8 @@ __global__ void __launch_bounds__(256) eloc_rbm(RbmGeom g, const double* __restr
   const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
   const int nw = blockDim.x / WAVE;
   cd* th = (cd*)lds + (size_t)w * 2 * g.H;
-  cd* lc = th + g.H;
-  const double* Wr = P + g.ow;
-  const double* Wi = P + g.ow + (int64_t)g.L * g.H;
-  for (int64_t s = (int64_t)blockIdx.x * nw + w; s < n; s += (int64_t)gridDim.x * nw) {
-    const int32_t* c = cfg + s * g.L;
-    double lr = 0.0, li = 0.0;
-    for (int h = lane; h < g.H; h += WAVE) {
-      cd t = rbm_theta(g, P, c, h);
-      cd l = g.cpx ? log_cosh(t) : cd{log_cosh(t.re), 0.0};
-      th[h] = t;
-      lc[h] = l;
-      lr += l.re;
-      li += l.im;
-    }
-    if (g.oa >= 0) {
-      for (int i = lane; i < g.L; i += WAVE) {
-        double sg = (double)(2 * c[i] - 1);
-        lr = fma(sg, P[g.oa + i], lr);
-        li = fma(sg, P[g.oa + g.L + i], li);
-      }
-    }
-    lr = wave_sum(lr);
-    li = wave_sum(li);
-    // log psi_internal(s) - log psi_s given by the caller (base.py:164)
-    cd off = {lr - lps[2 * s], li - lps[2 * s + 1]};
-    wave_sync();
-    cd dsum = diag_sum(op, c);
-    cd acc = {0.0, 0.0};
-    int sites[MAXLEN], vals[MAXLEN], nch;
-    for (int o = 0; o < op.num_ops; ++o) {
-      cd m = merged_m(op, o, apply_string(op, o, c, sites, vals, nch), dsum);
-      if (!nonzero(m)) continue;
-      cd dl = {0.0, 0.0};
-      if (nch > 0) {
-        for (int h = lane; h < g.H; h += WAVE) {
-          cd t = th[h];
-          for (int j = 0; j < nch; ++j) {
-            double d = 2.0 * (double)(vals[j] - c[sites[j]]);
-            int64_t wi = (int64_t)sites[j] * g.H + h;
-            t.re = fma(d, Wr[wi], t.re);
-            if (g.cpx) t.im = fma(d, Wi[wi], t.im);
-          }
-          cd l = g.cpx ? log_cosh(t) : cd{log_cosh(t.re), 0.0};
-          dl.re += l.re - lc[h].re;
-          dl.im += l.im - lc[h].im;
-        }
-        dl.re = wave_sum(dl.re);
-        dl.im = wave_sum(dl.im);
-        if (g.oa >= 0) {
-          for (int j = 0; j < nch; ++j) {
-            double d = 2.0 * (double)(vals[j] - c[sites[j]]);
-            dl.re = fma(d, P[g.oa + sites[j]], dl.re);
-            dl.im = fma(d, P[g.oa + g.L + sites[j]], dl.im);
-          }
-        }
-      }
-      acc = acc + m * cexp(dl + off);
-    }
-    if (lane == 0) {
-      out[2 * s] = acc.re;
-      out[2 * s + 1] = acc.im;
-    }
-    wave_sync();
-  }
+  for (int64_t s = (int64_t)blockIdx.x * nw + w; s < n; s += (int64_t)gridDim.x * nw)
+    eloc_rbm_sample(g, P, op, cfg + s * g.L, lps + 2 * s, out + 2 * s, th, lane);
 }
 
 // ------------------------------------------------------------------ CNN O_loc
@@ -261,7 +128,7 @@ __global__ void __launch_bounds__(512) eloc_cnn(CnnGeom g, const double* __restr
       x0[i] = (double)(2 * c[i] - 1);
     }
     __syncthreads();
-    cnn_forward_group(g, P, x0, A, nullptr, nbr, tid, nt, sync);
+    cnn_forward_group<double>(g, P, x0, A, nullptr, nbr, tid, nt, sync);
     const int L1 = g.nl - 1;
     double part = 0.0;
     for (int t = tid; t < g.n * g.C[L1]; t += nt) part += A[g.aoff[L1] + t];
@@ -354,22 +221,24 @@ __device__ inline cd classify_string(const OpDev& op, int o, const int32_t* s, i
 // thread computes all CM channels of one window element of one flip, so a
 // layer costs one barrier per chunk.  Each flip's last-layer change is reduced
 // by one wave; the sample's O_loc is a block reduction over strings.
-__host__ __device__ inline size_t elo_win_bytes(const CnnGeom& g, int cm, int nops, int fch) {
-  size_t d = (size_t)g.n + (size_t)g.nl * cm * g.n + 4 * (size_t)nops + 16;
-  for (int l = 0; l < g.nl - 1; ++l) d += (size_t)fch * cm * win_ex(g, l) * win_ey(g, l);
-  d += (size_t)fch * win_ex(g, g.nl - 1) * win_ey(g, g.nl - 1);
+// LDS: FP64 string data, reductions and per-flip partial sums; activations and
+// window buffers in the arithmetic type (tsize bytes: 8 FP64, 4 mixed FP32).
+__host__ __device__ inline size_t elo_win_bytes(const CnnGeom& g, int cm, int nops, int fch, int tsize = 8) {
+  const size_t d = 4 * (size_t)nops + 16 + (size_t)fch * win_ex(g, g.nl - 1) * win_ey(g, g.nl - 1);
+  size_t t = (size_t)g.n + (size_t)g.nl * cm * g.n;
+  for (int l = 0; l < g.nl - 1; ++l) t += (size_t)fch * cm * win_ex(g, l) * win_ey(g, l);
   const size_t i32 = (size_t)g.n + 2 * (size_t)nops + 16;
   const size_t sh = (size_t)g.n * g.FF;
-  return 8 * d + 4 * i32 + 2 * sh;
+  return 8 * d + (size_t)tsize * t + 4 * i32 + 2 * sh;
 }
 
 // Flips per chunk: the fewest estimated VALU cycles per flip among the sizes
 // whose LDS fits two workgroups per CU (at least one flip).
-inline int elo_win_chunk(const CnnGeom& g, int cm, int nops, int threads) {
+inline int elo_win_chunk(const CnnGeom& g, int cm, int nops, int threads, int tsize = 8) {
   int best = 1;
   double best_cost = 1e300;
   for (int f = 1; f <= 64; ++f) {
-    if (f > 1 && elo_win_bytes(g, cm, nops, f) > 80 * 1024) break;
+    if (f > 1 && elo_win_bytes(g, cm, nops, f, tsize) > 80 * 1024) break;
     double cost = 0.0;
     for (int l = 0; l < g.nl; ++l) {
       const int wsz = win_ex(g, l) * win_ey(g, l);
@@ -385,8 +254,11 @@ inline int elo_win_chunk(const CnnGeom& g, int cm, int nops, int threads) {
   return best;
 }
 
-template <int CM>
-__global__ void __launch_bounds__(256) eloc_cnn_win(CnnGeom g, const double* __restrict__ Wpk, CnnPack pk, OpDev op,
+// T = double: the FP64 form; T = float: the mixed-precision form (FP32
+// activations and window recomputation, log psi, the per-flip changes and
+// O_loc accumulated in FP64).
+template <int CM, typename T>
+__global__ void __launch_bounds__(256) eloc_cnn_win(CnnGeom g, const T* __restrict__ Wpk, CnnPack pk, OpDev op,
                                                     const int32_t* __restrict__ cfg,
                                                     const double* __restrict__ lps, int64_t n, int fch,
                                                     double* __restrict__ out) {
@@ -394,29 +266,28 @@ __global__ void __launch_bounds__(256) eloc_cnn_win(CnnGeom g, const double* __r
   const int tid = threadIdx.x, nt = blockDim.x;
   const int lane = tid % WAVE, wv = tid / WAVE, nw = nt / WAVE;
   const int nops = op.num_ops, L1 = g.nl - 1;
-  const int lsz = CM * g.n;  // doubles per stored layer
+  const int lsz = CM * g.n;  // values per stored layer
   const int wlast = win_ex(g, L1) * win_ey(g, L1);
-  double* x0 = lds;
-  double* A = x0 + g.n;  // layer l as [c][n] at A + l*lsz
-  double* mre = A + (size_t)g.nl * lsz;
+  double* mre = lds;
   double* mim = mre + nops;
   double* cre = mim + nops;
   double* cim = cre + nops;
   double* red = cim + nops;
-  double* W[JV_MAX_LAYERS];
-  double* wp = red + 16;
+  double* part = red + 16;  // per flip [wlast]
+  T* x0 = (T*)(part + (size_t)fch * wlast);
+  T* A = x0 + g.n;  // layer l as [c][n] at A + l*lsz
+  T* W[JV_MAX_LAYERS];
+  T* wp = A + (size_t)g.nl * lsz;
   for (int l = 0; l < L1; ++l) {
     W[l] = wp;  // per flip [c][wsz]
     wp += (size_t)fch * CM * win_ex(g, l) * win_ey(g, l);
   }
-  double* part = wp;  // per flip [wlast]
-  wp += (size_t)fch * wlast;
   int32_t* cs = (int32_t*)wp;
   int32_t* code = cs + g.n;
   int32_t* flips = code + nops;
   int32_t* wcnt = flips + nops;
   short* nbr = (short*)(wcnt + 16);
-  const cdbl* Wp = as_const(Wpk);
+  const cst_t<T>* Wp = (const cst_t<T>*)Wpk;
   for (int t = tid; t < g.n * g.FF; t += nt) {
     const int p = t / g.FF, k = t % g.FF;
     const int i = p / g.Ly, j = p % g.Ly, ky = k / g.Fy, kx = k % g.Fy;
@@ -427,12 +298,12 @@ __global__ void __launch_bounds__(256) eloc_cnn_win(CnnGeom g, const double* __r
     const int32_t* c = cfg + s * g.n;
     for (int i = tid; i < g.n; i += nt) {
       cs[i] = c[i];
-      x0[i] = (double)(2 * c[i] - 1);
+      x0[i] = (T)(2 * c[i] - 1);
     }
     __syncthreads();
-    win_forward<CM, 1>(g, Wp, pk, x0, A, lsz, g.n, nbr, tid, nt, sync);
+    win_forward<CM, 1, T>(g, Wp, pk, x0, A, lsz, g.n, nbr, tid, nt, sync);
     double pl = 0.0;
-    for (int t = tid; t < g.C[L1] * g.n; t += nt) pl += A[(size_t)L1 * lsz + t];
+    for (int t = tid; t < g.C[L1] * g.n; t += nt) pl += (double)A[(size_t)L1 * lsz + t];
     const double lp = block_sum(pl, red) * g.inv_nrm;
     const double off_re = lp - lps[2 * s], off_im = -lps[2 * s + 1];
     const double ph_c = cos(off_im), ph_s = sin(off_im);
@@ -493,30 +364,29 @@ __global__ void __launch_bounds__(256) eloc_cnn_win(CnnGeom g, const double* __r
       for (int l = 0; l < g.nl; ++l) {
         const int ey = win_ey(g, l), wsz = win_ex(g, l) * ey;
         const int pex = win_ex(g, l - 1), pey = win_ey(g, l - 1);
-        const cdbl* K = Wp + pk.koff[l];
-        const cdbl* B = Wp + pk.boff[l];
+        const cst_t<T>* K = Wp + pk.koff[l];
+        const cst_t<T>* B = Wp + pk.boff[l];
         for (int t = tid; t < nf * wsz; t += nt) {
           const int f = t / wsz, j = t - f * wsz;
           const int cv = code[flips[f0 + f]];
           const int site = cv >> 1;
           const int si = site / g.Ly, sj = site - si * g.Ly;
           const int a = j / ey, b = j - a * ey;
-          double z[CM];
+          T z[CM];
           if (l == 0)
-            win_preact0<CM>(g, K, B, x0, (double)(2 * (cv & 1) - 1), site, si, sj, a, b, z);
+            win_preact0<CM, T>(g, K, B, x0, (T)(2 * (cv & 1) - 1), site, si, sj, a, b, z);
           else
-            win_preact<CM>(g, K, B, A + (size_t)(l - 1) * lsz, W[l - 1] + (size_t)f * CM * pex * pey, pex, pey,
-                           si, sj, a, b, z);
+            win_preact<CM, T>(g, K, B, A + (size_t)(l - 1) * lsz, W[l - 1] + (size_t)f * CM * pex * pey, pex, pey,
+                              si, sj, a, b, z);
           if (l < L1) {
-            __attribute__((address_space(3))) double* Wo =
-                (__attribute__((address_space(3))) double*)(W[l] + (size_t)f * CM * wsz + j);
+            lds_w<T>* Wo = (lds_w<T>*)(W[l] + (size_t)f * CM * wsz + j);
 #pragma unroll
             for (int cc = 0; cc < CM; ++cc) Wo[cc * wsz] = log_cosh(z[cc]);
           } else {
-            ldbl* Ab = as_lds(A + (size_t)L1 * lsz + win_site(g, si, sj, a, b));
+            lds_t<T>* Ab = (lds_t<T>*)(A + (size_t)L1 * lsz + win_site(g, si, sj, a, b));
             double acc = 0.0;
 #pragma unroll
-            for (int cc = 0; cc < CM; ++cc) acc += log_cosh(z[cc]) - Ab[cc * g.n];
+            for (int cc = 0; cc < CM; ++cc) acc += (double)(log_cosh(z[cc]) - Ab[cc * g.n]);
             part[f * wlast + j] = acc;
           }
         }
@@ -628,25 +498,30 @@ int jv_eloc(const jv_net* net, const double* params, const jv_optable* op, const
   }
   if (make_cnn_geom(net, cg) && op_single_site && win_ok(cg) && cnn_cmax(cg) > 0) {
     const int cm = cnn_cmax(cg);
-    const int fch = elo_win_chunk(cg, cm, op->num_ops, 256);
-    const size_t lds = elo_win_bytes(cg, cm, op->num_ops, fch);
+    const bool mixed = net->precision == JV_PREC_MIXED;
+    const int ts = mixed ? 4 : 8;
+    const int fch = elo_win_chunk(cg, cm, op->num_ops, 256, ts);
+    const size_t lds = elo_win_bytes(cg, cm, op->num_ops, fch, ts);
     if (lds <= 160 * 1024) {
       const CnnPack pk = make_cnn_pack(cg, cm, 1);
       ScopedTimer tm("eloc", st);
-      const double* wpk = pack_cnn_weights(cg, pk, params, st, 40);
-      if (!wpk) return JV_E_NOMEM;
       const int64_t blocks = std::min<int64_t>(n, 1 << 20);
-      switch (cm) {
-#define JV_ELO(CMv)                                                                                     \
-  case CMv:                                                                                             \
-    rc = set_lds_limit((const void*)eloc_cnn_win<CMv>, lds);                                            \
-    if (rc) return rc;                                                                                  \
-    hipLaunchKernelGGL(eloc_cnn_win<CMv>, dim3((unsigned)blocks), dim3(256), lds, st, cg, wpk, pk, d, configs, \
-                       logpsi_s, n, fch, out);                                                          \
-    break;
-        JV_ELO(4) JV_ELO(8) JV_ELO(16) JV_ELO(32)
-#undef JV_ELO
+#define JV_ELO(CMv, Tv, slot)                                                                            \
+  case CMv: {                                                                                            \
+    const Tv* wpk = pack_cnn_weights<Tv>(cg, pk, params, st, slot);                                      \
+    if (!wpk) return JV_E_NOMEM;                                                                         \
+    rc = set_lds_limit((const void*)eloc_cnn_win<CMv, Tv>, lds);                                         \
+    if (rc) return rc;                                                                                   \
+    hipLaunchKernelGGL((eloc_cnn_win<CMv, Tv>), dim3((unsigned)blocks), dim3(256), lds, st, cg, wpk, pk, d, \
+                       configs, logpsi_s, n, fch, out);                                                  \
+    break;                                                                                               \
+  }
+      if (mixed) {
+        switch (cm) { JV_ELO(4, float, 42) JV_ELO(8, float, 42) JV_ELO(16, float, 42) JV_ELO(32, float, 42) }
+      } else {
+        switch (cm) { JV_ELO(4, double, 40) JV_ELO(8, double, 40) JV_ELO(16, double, 40) JV_ELO(32, double, 40) }
       }
+#undef JV_ELO
       JV_LAUNCH_CHECK("eloc_cnn_win");
       return JV_OK;
     }

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "common.hpp"
@@ -34,7 +35,8 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 // Column partial sums of w_n * O[n][j] over row chunks (deterministic order).
 constexpr int CHUNK_ROWS = 256;
 
-__global__ void col_wsum_partial(const double* __restrict__ O, const double* __restrict__ w, int64_t n,
+template <typename TIn>
+__global__ void col_wsum_partial(const TIn* __restrict__ O, const double* __restrict__ w, int64_t n,
                                  int64_t p, int cpx, double* __restrict__ part) {
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= p) return;
@@ -61,31 +63,49 @@ __global__ void col_reduce(const double* __restrict__ part, int nchunks, int64_t
   out[j] = a;
 }
 
-__global__ void center_apply(const double* __restrict__ O, const double* __restrict__ w,
+template <typename TIn>
+__global__ void center_apply(const TIn* __restrict__ O, const double* __restrict__ w,
                              const double* __restrict__ mean, int64_t n, int64_t p, int cpx,
-                             double* __restrict__ D) {
+                             TIn* __restrict__ D) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int64_t len = p * (cpx ? 2 : 1);
   if (t >= n * len) return;
   int64_t r = t / len, j = t % len;
-  D[t] = sqrt(w[r]) * (O[t] - mean[j]);
+  D[t] = (TIn)(sqrt(w[r]) * ((double)O[t] - mean[j]));
 }
 
-int center(const double* O, const double* w, int64_t n, int64_t p, int cpx, double* mean, double* D,
-           hipStream_t st, int ws_slot = 2) {
+// Doubles of partial column sums the centring needs: one row of len values
+// per chunk of CHUNK_ROWS samples.
+inline int64_t center_partials(int64_t n, int64_t p, int cpx) {
+  return (n + CHUNK_ROWS - 1) / CHUNK_ROWS * p * (cpx ? 2 : 1);
+}
+
+// Centring with the partial sums in `part` (center_partials(n, p, cpx)
+// doubles, every one written by col_wsum_partial before col_reduce reads it,
+// so nothing needs clearing).
+// TIn = float: the mixed form's FP32 observations, mean in FP64, centred data
+// rounded to FP32.
+template <typename TIn>
+int center_with(const TIn* O, const double* w, int64_t n, int64_t p, int cpx, double* mean, TIn* D,
+                double* part, hipStream_t st) {
   int nchunks = (int)((n + CHUNK_ROWS - 1) / CHUNK_ROWS);
   int64_t len = p * (cpx ? 2 : 1);
-  double* part = (double*)workspace(sizeof(double) * (size_t)nchunks * len, ws_slot);
-  if (!part) return fail(JV_E_NOMEM, "jv_center: workspace allocation failed");
-  hipLaunchKernelGGL(col_wsum_partial, dim3((unsigned)((p + 255) / 256), nchunks), dim3(256), 0, st, O, w, n,
+  hipLaunchKernelGGL(col_wsum_partial<TIn>, dim3((unsigned)((p + 255) / 256), nchunks), dim3(256), 0, st, O, w, n,
                      p, cpx, part);
   JV_LAUNCH_CHECK("col_wsum_partial");
   hipLaunchKernelGGL(col_reduce, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, st, part, nchunks, len, mean);
   JV_LAUNCH_CHECK("col_reduce");
-  hipLaunchKernelGGL(center_apply, dim3((unsigned)((n * len + 255) / 256)), dim3(256), 0, st, O, w, mean, n, p,
-                     cpx, D);
+  hipLaunchKernelGGL(center_apply<TIn>, dim3((unsigned)((n * len + 255) / 256)), dim3(256), 0, st, O, w, mean, n,
+                     p, cpx, D);
   JV_LAUNCH_CHECK("center_apply");
   return JV_OK;
+}
+
+int center(const double* O, const double* w, int64_t n, int64_t p, int cpx, double* mean, double* D,
+           hipStream_t st, int ws_slot = 2) {
+  double* part = (double*)workspace(sizeof(double) * (size_t)center_partials(n, p, cpx), ws_slot);
+  if (!part) return fail(JV_E_NOMEM, "jv_center: workspace allocation failed");
+  return center_with(O, w, n, p, cpx, mean, D, part, st);
 }
 
 // ------------------------------------------------------------------ GEMM (FP64 MFMA)
@@ -401,8 +421,8 @@ __global__ void gemv_n(const double* __restrict__ M, const double* __restrict__ 
 }
 
 // y[j] = alpha * sum_i op(M[i][j]) x[i] (op 'T' or 'C'), row-chunk partials.
-template <bool CPX>
-__global__ void gemv_t_partial(const double* __restrict__ M, const double* __restrict__ x, int64_t rows,
+template <bool CPX, typename TM = double>
+__global__ void gemv_t_partial(const TM* __restrict__ M, const double* __restrict__ x, int64_t rows,
                                int64_t cols, int conjM, double* __restrict__ part) {
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= cols) return;
@@ -416,7 +436,7 @@ __global__ void gemv_t_partial(const double* __restrict__ M, const double* __res
       ar += mr * xr - mi * xi;
       ai += mr * xi + mi * xr;
     } else {
-      ar = fma(M[r * cols + j], x[r], ar);
+      ar = fma((double)M[r * cols + j], x[r], ar);
     }
   }
   if (CPX) {
@@ -430,6 +450,20 @@ __global__ void gemv_t_partial(const double* __restrict__ M, const double* __res
 __global__ void scale_vec(double* __restrict__ v, int64_t len, double alpha) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < len) v[i] *= alpha;
+}
+
+// y = M^T x for a real FP32 matrix M [rows][cols] (the mixed form's centred
+// data), accumulated in FP64.
+int gemv_t_f32(const float* M, const double* x, int64_t rows, int64_t cols, double* y, hipStream_t st, int ws_slot) {
+  const int nchunks = (int)((rows + CHUNK_ROWS - 1) / CHUNK_ROWS);
+  double* part = (double*)workspace(sizeof(double) * (size_t)nchunks * cols, ws_slot);
+  if (!part) return fail(JV_E_NOMEM, "gemv: workspace allocation failed");
+  hipLaunchKernelGGL((gemv_t_partial<false, float>), dim3((unsigned)((cols + 255) / 256), nchunks), dim3(256), 0, st, M,
+                     x, rows, cols, 0, part);
+  JV_LAUNCH_CHECK("gemv_t_partial");
+  hipLaunchKernelGGL(col_reduce, dim3((unsigned)((cols + 255) / 256)), dim3(256), 0, st, part, nchunks, cols, y);
+  JV_LAUNCH_CHECK("col_reduce");
+  return JV_OK;
 }
 
 int gemv(bool trans, const double* M, const double* x, int64_t rows, int64_t cols, int cpx, int conjM,
@@ -559,21 +593,38 @@ __global__ void __launch_bounds__(256) sum_finish(const double* __restrict__ par
   if (threadIdx.x == 0) *out = a;
 }
 
-// Sc = S - c I with c = ||S||_F (rcond + 8 p eps) (||S||_F bounds the largest
-// eigenvalue, 8 p eps ||S||_F the rounding of the Cholesky factorisation) and
-// S2 = S.
-// Sc = S + a E - c I and S2 = S + a E, where E has ones in the blk x blk
-// diagonal blocks and a = *add (no add: a = 0), the deflation of the centring
-// null vector found by null_probe / null_decide.
+// Margin of the Cholesky test.  If the factorisation of a symmetric p x p
+// matrix A completes in floating point, L L^T = A + dA with
+// ||dA||_2 <= p gamma_{p+1} ||A||_2, gamma_k = k u / (1 - k u), u = 2^-53
+// (Higham, Accuracy and Stability of Numerical Algorithms, 2nd ed., Thm 10.3
+// with || |L| |L^T| ||_2 <= p ||A||_2), so lambda_min(A) >= -||dA||_2.  For
+// A = S' - c I (S' = S, or S + a E after a deflation) and
+// c = F (rcond + d + x) / (1 - d), d = p gamma_{p+1}, F >= ||S'||_F >= lambda_max,
+// a completed factorisation gives lambda_min(S') >= c - d (F + c) >=
+// (rcond + x) F: every eigenvalue of S' exceeds the pseudo-inverse cutoff
+// rcond lambda_max plus the deflation's perturbation bound x F.
+__device__ inline double chol_margin(double F, int64_t p, double rcond, double x) {
+  const double u = 1.1102230246251565e-16;
+  const double g = (double)(p + 1) * u / (1.0 - (double)(p + 1) * u);
+  const double d = (double)p * g;
+  return F * (rcond + d + x) / (1.0 - d);
+}
+
+// Sc = S' - c I and S2 = S' with S' = S + a E, where E has ones in the
+// blk x blk diagonal blocks and a = add[0] (no add: a = 0) deflates the
+// centring null vector found by null_probe / null_decide (add[2]: its relative
+// perturbation bound x); c = chol_margin(||S||_F + ||a E||_F, ...).
 __global__ void shifted_copy(const double* __restrict__ S, int64_t p, const double* __restrict__ fro2, double rcond,
                              const double* __restrict__ add, int64_t blk, double* __restrict__ Sc,
                              double* __restrict__ S2) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= p * p) return;
-  const double c = sqrt(*fro2) * (rcond + 8.0 * (double)p * 2.220446049250313e-16);
+  const double a = add ? add[0] : 0.0;
+  const double F = sqrt(*fro2) + fabs(a) * (double)blk * sqrt((double)(p / blk));
+  const double c = chol_margin(F, p, rcond, add ? add[2] : 0.0);
   const int64_t i = t / p, j = t % p;
   double v = S[t];
-  if (add && i / blk == j / blk) v += *add;
+  if (add && i / blk == j / blk) v += a;
   Sc[t] = v - (i == j ? c : 0.0);
   S2[t] = v;
 }
@@ -581,61 +632,79 @@ __global__ void shifted_copy(const double* __restrict__ S, int64_t p, const doub
 // Centred data (row n scaled by sqrt(w_n), sum_n sqrt(w_n) D_n = 0) make
 // u0 = sqrt(w) a null vector of T = D D^dagger, so the pseudo-inverse of T
 // always cuts one eigenvalue and T itself never passes the Cholesky test.  For
-// uniform weights u0 = 1/sqrt(n): part[i] = |(T u0)_i|^2 and part[n + i] =
-// Re T_ii (one wave per row).
+// uniform weights u0 = 1/sqrt(n): part[i] = |(T u0)_i|^2, part[n + i] =
+// Re T_ii and part[2n + i] = sum_j |T_ij|^2 (one wave per row).
 __global__ void __launch_bounds__(256) null_probe(const double* __restrict__ T, int64_t n, int cpx,
                                                   double* __restrict__ part) {
   const int lane = threadIdx.x % WAVE;
   const int64_t i = (int64_t)blockIdx.x * 4 + threadIdx.x / WAVE;
   if (i >= n) return;
-  double sr = 0.0, si = 0.0;
+  double sr = 0.0, si = 0.0, sq = 0.0;
   if (cpx) {
     const double* row = T + 2 * i * n;
     for (int64_t j = lane; j < n; j += WAVE) {
       sr += row[2 * j];
       si += row[2 * j + 1];
+      sq = fma(row[2 * j], row[2 * j], fma(row[2 * j + 1], row[2 * j + 1], sq));
     }
   } else {
     const double* row = T + i * n;
-    for (int64_t j = lane; j < n; j += WAVE) sr += row[j];
+    for (int64_t j = lane; j < n; j += WAVE) {
+      sr += row[j];
+      sq = fma(row[j], row[j], sq);
+    }
   }
   sr = wave_sum(sr);
   si = wave_sum(si);
+  sq = wave_sum(sq);
   if (lane == 0) {
     part[i] = (sr * sr + si * si) / (double)n;
     part[n + i] = cpx ? T[2 * (i * n + i)] : T[i * n + i];
+    part[2 * n + i] = sq;
   }
 }
 
-// addv[0] = mu / n (mu = max Re T_ii <= lambda_max(T)) and addv[1] = 1 when
-// ||T u0|| <= rcond mu / 2, else both 0.  Then T has an eigenvalue within
-// ||T u0|| of zero, below the cutoff rcond lambda_max; T + mu u0 u0^T moves it
-// up, and by interlacing a successful Cholesky test of the deflated matrix
-// (c from ||T||_F: mu adds at most ||T||_F to the norm, which the 8 p eps
-// margin covers) shows every other eigenvalue of T above the cutoff.
-__global__ void __launch_bounds__(1024) null_decide(const double* __restrict__ part, int64_t n, double rcond,
+// Deflation test of the centring null vector, r = T u0 (u0 = 1/sqrt(n)):
+// accepted when ||r|| <= NULL_K n eps ||T||_F (u0 is an eigenvector of T up
+// to rounding: T~ = T - r u0^T - u0 r^T + (u0^T r) u0 u0^T has u0 exactly,
+// ||T - T~||_2 <= 2 ||r||, and T~ shares all other eigenvalues with T to within
+// 2 ||r||) and ||r|| <= rcond mu / 2 (mu = max Re T_ii <= lambda_max, so the
+// eigenvalue near u0, at most 2 ||r||, lies below the cutoff rcond lambda_max
+// and the reference's pseudo-inverse cuts it).  Then addv = (mu / n, 1, x) with
+// x = 2 NULL_K n eps per deflated vector (one real, two for the real form of a
+// complex T), else (0, 0, 0).  T + mu u0 u0^T moves the cut eigenvalue up;
+// by interlacing, a successful Cholesky test of the deflated matrix with the
+// margin x shows every other eigenvalue of T above the cutoff.
+constexpr double NULL_K = 64.0;
+__global__ void __launch_bounds__(1024) null_decide(const double* __restrict__ part, int64_t n, double rcond, int cpx,
                                                     double* __restrict__ addv) {
-  __shared__ double rs[1024], rm[1024];
+  __shared__ double rs[1024], rm[1024], rf[1024];
   const int tid = threadIdx.x;
-  double s = 0.0, m = 0.0;
+  double s = 0.0, m = 0.0, f = 0.0;
   for (int64_t i = tid; i < n; i += 1024) {
     s += part[i];
     m = fmax(m, part[n + i]);
+    f += part[2 * n + i];
   }
   rs[tid] = s;
   rm[tid] = m;
+  rf[tid] = f;
   __syncthreads();
   for (int w = 512; w > 0; w >>= 1) {
     if (tid < w) {
       rs[tid] += rs[tid + w];
       rm[tid] = fmax(rm[tid], rm[tid + w]);
+      rf[tid] += rf[tid + w];
     }
     __syncthreads();
   }
   if (tid == 0) {
-    const bool defl = rm[0] > 0.0 && sqrt(rs[0]) <= 0.5 * rcond * rm[0];
+    const double eps = 2.220446049250313e-16;
+    const double r = sqrt(rs[0]), F = sqrt(rf[0]);
+    const bool defl = rm[0] > 0.0 && r <= 0.5 * rcond * rm[0] && r <= NULL_K * (double)n * eps * F;
     addv[0] = defl ? rm[0] / (double)n : 0.0;
     addv[1] = defl ? 1.0 : 0.0;
+    addv[2] = defl ? (cpx ? 4.0 : 2.0) * NULL_K * (double)n * eps : 0.0;
   }
 }
 
@@ -704,6 +773,14 @@ int jv_center(const double* obs, const double* w, int64_t n, int64_t p, int32_t 
               double* data, void* stream) {
   if (!obs || !w || !mean || !data || n < 1 || p < 1) return fail(JV_E_ARG, "jv_center: bad arguments");
   return center(obs, w, n, p, is_complex, mean, data, (hipStream_t)stream);
+}
+
+int jv_center_f32(const float* obs, const double* w, int64_t n, int64_t p, int32_t is_complex, double* mean,
+                  float* data, void* stream) {
+  if (!obs || !w || !mean || !data || n < 1 || p < 1) return fail(JV_E_ARG, "jv_center_f32: bad arguments");
+  double* part = (double*)workspace(sizeof(double) * (size_t)center_partials(n, p, is_complex), 2);
+  if (!part) return fail(JV_E_NOMEM, "jv_center_f32: workspace allocation failed");
+  return center_with<float>(obs, w, n, p, is_complex, mean, data, part, (hipStream_t)stream);
 }
 
 int jv_gram_nn(const double* data, int64_t n, int64_t p, int32_t is_complex, double* T, void* stream) {
@@ -803,12 +880,12 @@ int jv_minsr_solve(const double* data, const double* e, int64_t n, int64_t p, in
     const int64_t m = cpx ? 2 * n : n;
     // the centring null vector 1/sqrt(n) of uniformly weighted data is deflated
     // (its eigenvalue is cut by the pseudo-inverse, and b is projected off it)
-    double* probe = (double*)workspace(sizeof(double) * 2 * (size_t)n, 101);
-    double* addv = (double*)workspace(sizeof(double) * 2, 102);
+    double* probe = (double*)workspace(sizeof(double) * 3 * (size_t)n, 101);
+    double* addv = (double*)workspace(sizeof(double) * 4, 102);
     if (!probe || !addv) return fail(JV_E_NOMEM, "jv_minsr_solve: workspace allocation failed");
     hipLaunchKernelGGL(null_probe, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, T, n, cpx, probe);
     JV_LAUNCH_CHECK("null_probe");
-    hipLaunchKernelGGL(null_decide, dim3(1), dim3(1024), 0, st, probe, n, rcond, addv);
+    hipLaunchKernelGGL(null_decide, dim3(1), dim3(1024), 0, st, probe, n, rcond, cpx, addv);
     JV_LAUNCH_CHECK("null_decide");
     double* R = T;
     if (cpx) {
@@ -1525,9 +1602,17 @@ __global__ void neg_copy(const double* __restrict__ in, int64_t len, double* __r
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < len) out[i] = -in[i];
 }
-static int minsr_dual(const double* data, const double* e, const double* e_raw, const double* w,
+// Storage and Gram arithmetic of the dual solve's data: FP64 data with the FP64
+// MFMA Gram (the reference's precision), or the mixed form's FP32 data with
+// the compensated BF16 Gram.
+struct DualData {
+  const void* ptr;
+  bool f32;
+};
+static int minsr_dual(DualData data, const double* e, const double* e_raw, const double* w,
                       hipEvent_t e_ready, int64_t n, int64_t p, int32_t is_complex, double rcond, double diag_shift,
                       bool try_chol, double* update, hipStream_t st);
+int gram_pp_bf16x3(const void* D, int f32, int64_t n, int64_t p, int cpx, double* S, hipStream_t st, int slot);
 static int minsr_dual_finish_info(int rc, jv_solve_info* info, int64_t p, double rcond, bool deferred,
                                   hipStream_t st);
 }  // namespace jv
@@ -1539,8 +1624,8 @@ int jv_minsr_solve_dual(const double* data, const double* e, int64_t n, int64_t 
                         void* stream) {
   if (!data || !e || !update || n < 1 || p < 1) return fail(JV_E_ARG, "jv_minsr_solve_dual: bad arguments");
   hipStream_t st = (hipStream_t)stream;
-  return minsr_dual_finish_info(minsr_dual(data, e, nullptr, nullptr, nullptr, n, p, is_complex, rcond, diag_shift,
-                                           try_cholesky != 0, update, st),
+  return minsr_dual_finish_info(minsr_dual({data, false}, e, nullptr, nullptr, nullptr, n, p, is_complex, rcond,
+                                           diag_shift, try_cholesky != 0, update, st),
                                 info, p, rcond, false, st);
 }
 
@@ -1550,7 +1635,19 @@ int jv_minsr_solve_dual_deferred(const double* data, const double* e_raw, const 
   if (!data || !e_raw || !w || !update || n < 1 || p < 1)
     return fail(JV_E_ARG, "jv_minsr_solve_dual_deferred: bad arguments");
   hipStream_t st = (hipStream_t)stream;
-  return minsr_dual_finish_info(minsr_dual(data, nullptr, e_raw, w, (hipEvent_t)e_ready, n, p, is_complex, rcond,
+  return minsr_dual_finish_info(minsr_dual({data, false}, nullptr, e_raw, w, (hipEvent_t)e_ready, n, p, is_complex,
+                                           rcond, diag_shift, try_cholesky != 0, update, st),
+                                info, p, rcond, true, st);
+}
+
+int jv_minsr_solve_dual_deferred_mixed(const float* data, const double* e_raw, const double* w, void* e_ready,
+                                       int64_t n, int64_t p, int32_t is_complex, double rcond, double diag_shift,
+                                       int32_t try_cholesky, double* update, jv_solve_info* info, void* stream) {
+  if (!data || !e_raw || !w || !update || n < 1 || p < 1)
+    return fail(JV_E_ARG, "jv_minsr_solve_dual_deferred_mixed: bad arguments");
+  if (is_complex) return fail(JV_E_ARG, "jv_minsr_solve_dual_deferred_mixed: real data only");
+  hipStream_t st = (hipStream_t)stream;
+  return minsr_dual_finish_info(minsr_dual({data, true}, nullptr, e_raw, w, (hipEvent_t)e_ready, n, p, 0, rcond,
                                            diag_shift, try_cholesky != 0, update, st),
                                 info, p, rcond, true, st);
 }
@@ -1649,7 +1746,7 @@ static int chol_route(const double* S, int64_t p, double rcond, int* flags, bool
 // path the Householder reduction does not need b = D^T e: the producer of e_raw
 // overlaps the Gram and the reduction, and the centring, b and Q^T b run on the
 // auxiliary stream during dstedc.
-static int minsr_dual(const double* data, const double* e, const double* e_raw, const double* w,
+static int minsr_dual(DualData data, const double* e, const double* e_raw, const double* w,
                       hipEvent_t e_ready, int64_t n, int64_t p, int32_t is_complex, double rcond, double diag_shift,
                       bool try_chol, double* update, hipStream_t st) {
   void* stream = (void*)st;
@@ -1663,7 +1760,8 @@ static int minsr_dual(const double* data, const double* e, const double* e_raw, 
   double* b = (double*)workspace(sizeof(double) * (size_t)p * 2, 15);
   if (!S || !ev || !y || !x || !er || !rank || !b) return fail(JV_E_NOMEM, "jv_minsr_solve_dual: workspace allocation failed");
   // S = D^dagger D (+ shift): same nonzero spectrum as T = D D^dagger (+ shift)
-  int rc = jv_gram_pp(data, n, p, cpx, S, stream);
+  int rc = data.f32 ? gram_pp_bf16x3(data.ptr, 1, n, p, cpx, S, st, 19)
+                    : jv_gram_pp((const double*)data.ptr, n, p, cpx, S, stream);
   if (rc) return rc;
   if (!cpx && diag_shift != 0.0) {
     hipLaunchKernelGGL(add_diag, dim3((unsigned)((p + 255) / 256)), dim3(256), 0, st, S, p, diag_shift, 0);
@@ -1690,7 +1788,8 @@ static int minsr_dual(const double* data, const double* e, const double* e_raw, 
       JV_LAUNCH_CHECK("real_part");
       rhs = er;
     }
-    return gemv(true, data, rhs, n, p, cpx, 1, 1.0, b, s, deferred ? 49 : 12);  // b = D^dagger e
+    if (data.f32) return gemv_t_f32((const float*)data.ptr, rhs, n, p, b, s, deferred ? 49 : 12);  // b = D^T e
+    return gemv(true, (const double*)data.ptr, rhs, n, p, cpx, 1, 1.0, b, s, deferred ? 49 : 12);  // b = D^dagger e
   };
   hipLaunchKernelGGL(set_ints, dim3(1), dim3(64), 0, st, rank, 0, 0);  // rank[1]: route taken
   JV_LAUNCH_CHECK("set_ints");

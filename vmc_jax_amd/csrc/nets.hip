@@ -4,6 +4,7 @@
 #include <vector>
 
 #include "cnnwin.hpp"
+#include "rbm.hpp"
 
 namespace jv {
 
@@ -43,11 +44,12 @@ bool make_cnn_geom(const jv_net* net, CnnGeom& g) {
   return true;
 }
 
-// LDS bytes of the CNN forward / gradient kernel.
-size_t cnn_fwd_lds(const CnnGeom& g, bool grad) {
-  size_t d = (size_t)g.P + g.n + (size_t)g.atot * (grad ? 2 : 1) + 8;
+// LDS bytes of the CNN forward / gradient kernel (tsize: bytes of the
+// arithmetic type, 8 for FP64, 4 for the mixed form).
+size_t cnn_fwd_lds(const CnnGeom& g, bool grad, int tsize = 8) {
+  size_t d = (size_t)g.P + g.n + (size_t)g.atot * (grad ? 2 : 1);
   size_t s = 2 * (size_t)g.n * g.FF;
-  return d * 8 + s * 2 + 16;
+  return 8 * 8 + d * tsize + s * 2 + 16;
 }
 
 int set_lds_limit(const void* fn, size_t bytes) {
@@ -60,105 +62,50 @@ int set_lds_limit(const void* fn, size_t bytes) {
 }
 
 // ------------------------------------------------------------------ RBM kernel
-// One wavefront per sample; lanes own hidden units h = lane + 64 c.
+// One wavefront per sample (rbm_fwd_grad_sample in rbm.hpp).
 __global__ void __launch_bounds__(256) rbm_fwd_grad(RbmGeom g, const double* __restrict__ P,
                                                     const int32_t* __restrict__ cfg, int64_t n,
                                                     int layout, double* __restrict__ grad,
                                                     double* __restrict__ logpsi) {
   const int lane = threadIdx.x % WAVE;
   const int64_t wstride = (int64_t)gridDim.x * (blockDim.x / WAVE);
-  const int64_t Pc = (g.ob >= 0 ? g.H : 0) + (int64_t)g.L * g.H + (g.oa >= 0 ? g.L : 0);
-  // doubles per gradient row
-  const int64_t rowlen = layout == 0 ? (g.cpx ? 2 * Pc : Pc) : 2 * (g.cpx ? 2 * Pc : Pc);
-  for (int64_t s = (int64_t)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE; s < n; s += wstride) {
-    const int32_t* c = cfg + s * g.L;
-    double* row = grad ? grad + s * rowlen : nullptr;
-    // column writers: compact (complex or real) or jVMC layout (complex; [g, i g] per complex leaf)
-    auto put = [&](int64_t leaf_off_flat, int64_t leaf_off_compact, int64_t leaf_size, int64_t j, cd v) {
-      if (layout == 0) {
-        if (g.cpx) {
-          row[2 * (leaf_off_compact + j)] = v.re;
-          row[2 * (leaf_off_compact + j) + 1] = v.im;
-        } else {
-          row[leaf_off_compact + j] = v.re;
-        }
-      } else {
-        int64_t col = leaf_off_flat + j;
-        row[2 * col] = v.re;
-        row[2 * col + 1] = v.im;
-        if (g.cpx) {  // i * g
-          int64_t col2 = leaf_off_flat + leaf_size + j;
-          row[2 * col2] = -v.im;
-          row[2 * col2 + 1] = v.re;
-        }
-      }
-    };
-    const int64_t cb = 0, cw = (g.ob >= 0 ? g.H : 0), ca = cw + (int64_t)g.L * g.H;
-    double accr = 0.0, acci = 0.0;
-    for (int h = lane; h < g.H; h += WAVE) {
-      cd th = rbm_theta(g, P, c, h);
-      if (g.cpx) {
-        cd lc = log_cosh(th);
-        accr += lc.re;
-        acci += lc.im;
-      } else {
-        accr += log_cosh(th.re);
-      }
-      if (row) {
-        cd t = g.cpx ? ctanh(th) : cd{tanh(th.re), 0.0};
-        if (g.ob >= 0) put(g.ob, cb, g.H, h, t);
-        for (int i = 0; i < g.L; ++i) {
-          double sg = (double)(2 * c[i] - 1);
-          put(g.ow, cw, (int64_t)g.L * g.H, (int64_t)i * g.H + h, {sg * t.re, sg * t.im});
-        }
-      }
-    }
-    if (g.oa >= 0) {
-      for (int i = lane; i < g.L; i += WAVE) {
-        double sg = (double)(2 * c[i] - 1);
-        accr = fma(sg, P[g.oa + i], accr);
-        acci = fma(sg, P[g.oa + g.L + i], acci);
-        if (row) put(g.oa, ca, g.L, i, {sg, 0.0});
-      }
-    }
-    accr = wave_sum(accr);
-    acci = wave_sum(acci);
-    if (logpsi && lane == 0) {
-      logpsi[2 * s] = accr;
-      logpsi[2 * s + 1] = acci;
-    }
-  }
+  const int64_t rowlen = rbm_grad_row_doubles(g, layout);
+  for (int64_t s = (int64_t)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE; s < n; s += wstride)
+    rbm_fwd_grad_sample(g, P, cfg + s * g.L, layout, grad ? grad + s * rowlen : nullptr,
+                        logpsi ? logpsi + 2 * s : nullptr, lane);
 }
 
 // ------------------------------------------------------------------ CNN kernel
 // One workgroup per sample (grid-stride): forward into LDS, then reverse pass
-// for dlogpsi/dtheta (cnn.py:131-171).
+// for dlogpsi/dtheta (cnn.py:131-171).  T = double: the FP64 form; T = float:
+// the mixed form (FP32 activations and O, log psi summed in FP64).
+template <typename T>
 __global__ void __launch_bounds__(256) cnn_fwd_grad(CnnGeom g, const double* __restrict__ Pg,
                                                     const int32_t* __restrict__ cfg, int64_t n,
-                                                    int layout, double* __restrict__ grad,
+                                                    int layout, T* __restrict__ grad,
                                                     double* __restrict__ logpsi) {
   extern __shared__ double lds[];
   const bool want_grad = grad != nullptr;
-  double* P = lds;
-  double* x0 = P + g.P;
-  double* A = x0 + g.n;
-  double* D = want_grad ? A + g.atot : nullptr;
-  double* red = A + g.atot * (want_grad ? 2 : 1);
-  short* nbr = (short*)(red + 8);
+  double* red = lds;
+  T* P = (T*)(red + 8);
+  T* x0 = P + g.P;
+  T* A = x0 + g.n;
+  T* D = want_grad ? A + g.atot : nullptr;
+  short* nbr = (short*)(A + g.atot * (want_grad ? 2 : 1));
   short* inv = nbr + g.n * g.FF;
   const int tid = threadIdx.x, nt = blockDim.x;
-  for (int i = tid; i < g.P; i += nt) P[i] = Pg[i];
+  for (int i = tid; i < g.P; i += nt) P[i] = (T)Pg[i];
   cnn_build_tables(g, nbr, inv, tid, nt);
   __syncthreads();
   auto sync = [] { __syncthreads(); };
   for (int64_t s = blockIdx.x; s < n; s += gridDim.x) {
     const int32_t* c = cfg + s * g.n;
-    for (int i = tid; i < g.n; i += nt) x0[i] = (double)(2 * c[i] - 1);
+    for (int i = tid; i < g.n; i += nt) x0[i] = (T)(2 * c[i] - 1);
     __syncthreads();
-    cnn_forward_group(g, P, x0, A, D, nbr, tid, nt, sync);
+    cnn_forward_group<T>(g, P, x0, A, D, nbr, tid, nt, sync);
     const int L1 = g.nl - 1;
     double part = 0.0;
-    for (int t = tid; t < g.n * g.C[L1]; t += nt) part += A[g.aoff[L1] + t];
+    for (int t = tid; t < g.n * g.C[L1]; t += nt) part += (double)A[g.aoff[L1] + t];
     double lp = block_sum(part, red) * g.inv_nrm;
     if (logpsi && tid == 0) {
       logpsi[2 * s] = lp;
@@ -166,41 +113,41 @@ __global__ void __launch_bounds__(256) cnn_fwd_grad(CnnGeom g, const double* __r
     }
     if (!want_grad) continue;
     // reverse pass; D_l is overwritten by delta_l = d out / d z_l
-    double* row = grad + s * (int64_t)g.P * (layout == 0 ? 1 : 2);
-    auto put = [&](int64_t col, double v) {
+    T* row = grad + s * (int64_t)g.P * (layout == 0 ? 1 : 2);
+    auto put = [&](int64_t col, T v) {
       if (layout == 0) row[col] = v;
-      else { row[2 * col] = v; row[2 * col + 1] = 0.0; }
+      else { row[2 * col] = v; row[2 * col + 1] = (T)0; }
     };
-    for (int t = tid; t < g.n * g.C[L1]; t += nt) D[g.aoff[L1] + t] *= g.inv_nrm;
+    for (int t = tid; t < g.n * g.C[L1]; t += nt) D[g.aoff[L1] + t] *= (T)g.inv_nrm;
     __syncthreads();
     for (int l = L1; l >= 0; --l) {
       const int cin = g.Cin[l], cout = g.C[l];
-      const double* dl = D + g.aoff[l];
-      const double* X = l == 0 ? x0 : A + g.aoff[l - 1];
+      const T* dl = D + g.aoff[l];
+      const T* X = l == 0 ? x0 : A + g.aoff[l - 1];
       // kernel gradient: sum_p delta[p, co] X[nbr(p, k), ci]
       for (int t = tid; t < g.FF * cin * cout; t += nt) {
         int co = t % cout, kc = t / cout, ci = kc % cin, k = kc / cin;
-        double acc = 0.0;
+        T acc = 0;
         for (int p = 0; p < g.n; ++p) acc = fma(dl[p * cout + co], X[nbr[p * g.FF + k] * cin + ci], acc);
         put(g.koff[l] + t, acc);
       }
       if (g.boff[l] >= 0) {
         for (int co = tid; co < cout; co += nt) {
-          double acc = 0.0;
+          T acc = 0;
           for (int p = 0; p < g.n; ++p) acc += dl[p * cout + co];
           put(g.boff[l] + co, acc);
         }
       }
       if (l > 0) {
         // delta_{l-1}[q, ci] = tanh(z_{l-1}[q, ci]) * sum_{k, co} delta_l[inv(q, k), co] K[k, ci, co]
-        const double* K = P + g.koff[l];
-        double* dp = D + g.aoff[l - 1];
+        const T* K = P + g.koff[l];
+        T* dp = D + g.aoff[l - 1];
         for (int t = tid; t < g.n * cin; t += nt) {
           int q = t / cin, ci = t % cin;
-          double acc = 0.0;
+          T acc = 0;
           for (int k = 0; k < g.FF; ++k) {
-            const double* d = dl + (int)inv[q * g.FF + k] * cout;
-            const double* w = K + (k * cin + ci) * cout;
+            const T* d = dl + (int)inv[q * g.FF + k] * cout;
+            const T* w = K + (k * cin + ci) * cout;
             for (int co = 0; co < cout; ++co) acc = fma(d[co], w[co], acc);
           }
           dp[t] *= acc;
@@ -212,7 +159,8 @@ __global__ void __launch_bounds__(256) cnn_fwd_grad(CnnGeom g, const double* __r
 }
 
 // ------------------------------------------------------------------ packed CNN weights
-__global__ void cnn_pack(CnnGeom g, CnnPack pk, const double* __restrict__ Pg, double* __restrict__ out) {
+template <typename T>
+__global__ void cnn_pack(CnnGeom g, CnnPack pk, const double* __restrict__ Pg, T* __restrict__ out) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= pk.total) return;
   int l = 0;
@@ -234,16 +182,17 @@ __global__ void cnn_pack(CnnGeom g, CnnPack pk, const double* __restrict__ Pg, d
     const int c = gi * cg_n + cg;
     if (ci < g.Cin[l] && c < g.C[l]) v = Pg[g.koff[l] + ((int64_t)k * g.Cin[l] + ci) * g.C[l] + c];
   }
-  out[t] = v;
+  out[t] = (T)v;
 }
 
-double* pack_cnn_weights(const CnnGeom& g, const CnnPack& pk, const double* params, hipStream_t st, int slot) {
-  double* out = (double*)workspace(sizeof(double) * (size_t)pk.total, slot);
+template <typename T>
+T* pack_cnn_weights(const CnnGeom& g, const CnnPack& pk, const double* params, hipStream_t st, int slot) {
+  T* out = (T*)workspace(sizeof(T) * (size_t)pk.total, slot);
   if (!out) {
     fail(JV_E_NOMEM, "packed CNN weights: workspace allocation failed");
     return nullptr;
   }
-  hipLaunchKernelGGL(cnn_pack, dim3((unsigned)((pk.total + 255) / 256)), dim3(256), 0, st, g, pk, params, out);
+  hipLaunchKernelGGL(cnn_pack<T>, dim3((unsigned)((pk.total + 255) / 256)), dim3(256), 0, st, g, pk, params, out);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     hip_fail(e, "cnn_pack launch");
@@ -251,6 +200,9 @@ double* pack_cnn_weights(const CnnGeom& g, const CnnPack& pk, const double* para
   }
   return out;
 }
+
+template double* pack_cnn_weights<double>(const CnnGeom&, const CnnPack&, const double*, hipStream_t, int);
+template float* pack_cnn_weights<float>(const CnnGeom&, const CnnPack&, const double*, hipStream_t, int);
 
 int cnn_cmax(const CnnGeom& g) {
   int m = 0;
@@ -271,6 +223,8 @@ extern "C" {
 int jv_net_init(jv_net* net) {
   if (!net) return fail(JV_E_ARG, "jv_net_init: null descriptor");
   if (net->ndim < 1 || net->ndim > 2) return fail(JV_E_ARG, "ndim must be 1 or 2");
+  if (net->precision != JV_PREC_FP64 && net->precision != JV_PREC_MIXED)
+    return fail(JV_E_ARG, "precision must be JV_PREC_FP64 or JV_PREC_MIXED");
   if (net->ndim == 1) net->shape[1] = 1;
   if (net->shape[0] < 1 || net->shape[1] < 1) return fail(JV_E_ARG, "lattice extents must be positive");
   net->num_sites = net->shape[0] * net->shape[1];
@@ -347,16 +301,34 @@ int jv_net_gradients(const jv_net* net, const double* params, const int32_t* con
   }
   if (make_cnn_geom(net, cg)) {
     size_t lds = cnn_fwd_lds(cg, grad != nullptr);
-    int rc = set_lds_limit((const void*)cnn_fwd_grad, lds);
+    int rc = set_lds_limit((const void*)cnn_fwd_grad<double>, lds);
     if (rc) return rc;
     int64_t blocks = std::min<int64_t>(n, 8192);
     ScopedTimer tm(grad ? "net_grad" : "net_forward", st);
-    hipLaunchKernelGGL(cnn_fwd_grad, dim3((unsigned)blocks), dim3(256), lds, st, cg, params, configs, n,
+    hipLaunchKernelGGL(cnn_fwd_grad<double>, dim3((unsigned)blocks), dim3(256), lds, st, cg, params, configs, n,
                        layout, grad, logpsi);
     JV_LAUNCH_CHECK("cnn_fwd_grad");
     return JV_OK;
   }
   return fail(JV_E_ARG, "jv_net_gradients: unknown network kind (call jv_net_init first)");
+}
+
+int jv_net_gradients_f32(const jv_net* net, const double* params, const int32_t* configs, int64_t n, float* grad,
+                         double* logpsi, void* stream) {
+  if (!net || !params || !configs || !grad || n < 0) return fail(JV_E_ARG, "jv_net_gradients_f32: bad arguments");
+  if (n == 0) return JV_OK;
+  hipStream_t st = (hipStream_t)stream;
+  CnnGeom cg;
+  if (!make_cnn_geom(net, cg)) return fail(JV_E_ARG, "jv_net_gradients_f32: only the CNN has an FP32 form");
+  const size_t lds = cnn_fwd_lds(cg, true, 4);
+  int rc = set_lds_limit((const void*)cnn_fwd_grad<float>, lds);
+  if (rc) return rc;
+  const int64_t blocks = std::min<int64_t>(n, 8192);
+  ScopedTimer tm("net_grad", st);
+  hipLaunchKernelGGL(cnn_fwd_grad<float>, dim3((unsigned)blocks), dim3(256), lds, st, cg, params, configs, n, 0, grad,
+                     logpsi);
+  JV_LAUNCH_CHECK("cnn_fwd_grad");
+  return JV_OK;
 }
 
 int jv_net_forward(const jv_net* net, const double* params, const int32_t* configs, int64_t n,

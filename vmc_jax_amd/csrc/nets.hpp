@@ -72,18 +72,21 @@ __device__ inline void cnn_build_tables(const CnnGeom& g, short* nbr, short* inv
 
 // One output of layer l: z = b + sum_{k, ci} X[nbr(p,k), ci] K[k, ci, co].
 // X is the layer input: x0 (one channel) for l == 0, else layer l-1 activations.
-__device__ inline double cnn_preact(const CnnGeom& g, const double* P, int l, const double* X,
-                                   const short* nbr, int p, int co) {
+template <typename T>
+__device__ inline T cnn_preact(const CnnGeom& g, const T* P, int l, const T* X, const short* nbr, int p, int co) {
   const int cin = g.Cin[l], cout = g.C[l];
-  const double* K = P + g.koff[l];
-  double z = g.boff[l] >= 0 ? P[g.boff[l] + co] : 0.0;
+  const T* K = P + g.koff[l];
+  T z = g.boff[l] >= 0 ? P[g.boff[l] + co] : (T)0;
   for (int k = 0; k < g.FF; ++k) {
-    const double* x = X + (int)nbr[p * g.FF + k] * cin;
-    const double* w = K + (k * cin) * cout + co;
+    const T* x = X + (int)nbr[p * g.FF + k] * cin;
+    const T* w = K + (k * cin) * cout + co;
     for (int ci = 0; ci < cin; ++ci) z = fma(x[ci], w[ci * cout], z);
   }
   return z;
 }
+
+__device__ inline double act_tanh(double z) { return tanh(z); }
+__device__ inline float act_tanh(float z) { return tanh_f32(z); }
 
 // Same, reading the changed values of a local update: input position q is taken
 // from Xnew when mask[q] is set, else from Xold.
@@ -105,20 +108,19 @@ __device__ inline double cnn_preact_masked(const CnnGeom& g, const double* P, in
 // Full forward of one configuration by a group of `nt` threads (thread index
 // `tid`), activations into A (an activation set), optional tanh(z) into D.
 // Caller synchronises the group before reading A; `sync` is a group barrier.
-template <typename Sync>
-__device__ inline void cnn_forward_group(const CnnGeom& g, const double* P, const double* x0,
-                                         double* A, double* D, const short* nbr, int tid, int nt,
-                                         Sync sync) {
+template <typename T, typename Sync>
+__device__ inline void cnn_forward_group(const CnnGeom& g, const T* P, const T* x0, T* A, T* D, const short* nbr,
+                                         int tid, int nt, Sync sync) {
   for (int l = 0; l < g.nl; ++l) {
-    const double* X = l == 0 ? x0 : A + g.aoff[l - 1];
-    double* Al = A + g.aoff[l];
-    double* Dl = D ? D + g.aoff[l] : nullptr;
+    const T* X = l == 0 ? x0 : A + g.aoff[l - 1];
+    T* Al = A + g.aoff[l];
+    T* Dl = D ? D + g.aoff[l] : nullptr;
     int cout = g.C[l];
     for (int t = tid; t < g.n * cout; t += nt) {
       int p = t / cout, co = t % cout;
-      double z = cnn_preact(g, P, l, X, nbr, p, co);
+      T z = cnn_preact(g, P, l, X, nbr, p, co);
       Al[t] = log_cosh(z);
-      if (Dl) Dl[t] = tanh(z);
+      if (Dl) Dl[t] = act_tanh(z);
     }
     sync();
   }

@@ -3,6 +3,7 @@
 #include <algorithm>
 
 #include "cnnwin.hpp"
+#include "rbm.hpp"
 
 namespace jv {
 
@@ -16,10 +17,7 @@ __global__ void zero_i64(int64_t* p, int n) {
 }
 
 // ------------------------------------------------------------------ RBM chains
-// One wavefront per chain.  theta_h and Re log cosh(theta_h) are cached in LDS
-// and updated on acceptance; a proposal costs one log-cosh per hidden unit
-// instead of the full network re-evaluation of sampler.py:335.  theta is
-// recomputed from scratch whenever a sample is emitted.
+// One wavefront per chain (mc_rbm_chain in rbm.hpp).
 __global__ void __launch_bounds__(256) mc_rbm(RbmGeom g, const double* __restrict__ P, jv_mc_args a,
                                               int32_t* __restrict__ walkers, int32_t* __restrict__ configs,
                                               int64_t* __restrict__ counters) {
@@ -27,63 +25,9 @@ __global__ void __launch_bounds__(256) mc_rbm(RbmGeom g, const double* __restric
   const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
   const int chain = blockIdx.x * nw + w;
   if (chain >= a.num_chains) return;
-  const size_t per = (size_t)3 * g.H + (g.L + 1) / 2;
-  double* base = lds + (size_t)w * per;
-  cd* th = (cd*)base;
-  double* lc = base + 2 * g.H;
-  int32_t* s = (int32_t*)(base + 3 * g.H);
-  const double* Wr = P + g.ow;
-  const double* Wi = P + g.ow + (int64_t)g.L * g.H;
-  for (int i = lane; i < g.L; i += WAVE) s[i] = walkers[(int64_t)chain * g.L + i];
-  wave_sync();
-  auto refresh = [&]() {
-    for (int h = lane; h < g.H; h += WAVE) {
-      cd t = rbm_theta(g, P, s, h);
-      th[h] = t;
-      lc[h] = g.cpx ? re_log_cosh(t) : log_cosh(t.re);
-    }
-  };
-  refresh();
-  int64_t nacc = 0;
-  const int total = a.therm_steps + a.samples_per_chain * a.sweep_steps;
-  for (int step = 0; step < total; ++step) {
-    int site;
-    double u;
-    mc_draw(a.seed, (uint32_t)step, (uint32_t)chain, a.call_index, a.stream, g.L, site, u);
-    const double d = -2.0 * (double)(2 * s[site] - 1);
-    double dl = 0.0;
-    for (int h = lane; h < g.H; h += WAVE) {
-      const int64_t wi = (int64_t)site * g.H + h;
-      cd t = {fma(d, Wr[wi], th[h].re), g.cpx ? fma(d, Wi[wi], th[h].im) : 0.0};
-      double l = g.cpx ? re_log_cosh(t) : log_cosh(t.re);
-      dl += l - lc[h];
-    }
-    dl = wave_sum(dl);
-    if (g.oa >= 0) dl = fma(d, P[g.oa + site], dl);
-    const bool acc = u < exp(a.mu * dl);  // random.bernoulli(key, P), sampler.py:336-340
-    if (acc) {
-      for (int h = lane; h < g.H; h += WAVE) {
-        const int64_t wi = (int64_t)site * g.H + h;
-        cd t = {fma(d, Wr[wi], th[h].re), g.cpx ? fma(d, Wi[wi], th[h].im) : 0.0};
-        th[h] = t;
-        lc[h] = g.cpx ? re_log_cosh(t) : log_cosh(t.re);
-      }
-      wave_sync();
-      if (lane == 0) s[site] ^= 1;
-      ++nacc;
-    }
-    wave_sync();
-    const int k = step + 1 - a.therm_steps;
-    if (k > 0 && k % a.sweep_steps == 0) {
-      const int64_t t = k / a.sweep_steps - 1;
-      int32_t* out = configs + (t * a.num_chains + chain) * g.L;
-      for (int i = lane; i < g.L; i += WAVE) out[i] = s[i];
-      refresh();
-      wave_sync();
-    }
-  }
-  for (int i = lane; i < g.L; i += WAVE) walkers[(int64_t)chain * g.L + i] = s[i];
+  const int64_t nacc = mc_rbm_chain(g, P, a, walkers, configs, chain, lane, lds + (size_t)w * mc_rbm_lds_doubles(g));
   if (lane == 0) {
+    const int total = a.therm_steps + a.samples_per_chain * a.sweep_steps;
     atomicAdd((unsigned long long*)&counters[0], (unsigned long long)total);
     atomicAdd((unsigned long long*)&counters[1], (unsigned long long)nacc);
   }
@@ -138,7 +82,7 @@ __global__ void __launch_bounds__(256) mc_cnn(CnnGeom g, const double* __restric
     x0[i] = (double)(2 * s[i] - 1);
   }
   wave_sync();
-  cnn_forward_group(g, P, x0, A, nullptr, nbr, lane, WAVE, [] { wave_sync(); });
+  cnn_forward_group<double>(g, P, x0, A, nullptr, nbr, lane, WAVE, [] { wave_sync(); });
   int64_t nacc = 0;
   const int total = a.therm_steps + a.samples_per_chain * a.sweep_steps;
   for (int step = 0; step < total; ++step) {
@@ -201,10 +145,12 @@ __host__ __device__ inline int mc_win_pos_len(const CnnGeom& g) {
   for (int l = 0; l < g.nl; ++l) t += win_ex(g, l) * win_ey(g, l);
   return t;
 }
-__host__ __device__ inline size_t mc_win_bytes(const CnnGeom& g, int cm, int cs) {
-  const size_t d = (size_t)g.nl * cm * cs + g.n + 16;                                // U, x0, red
+// LDS: the FP64 reduction slots, then the planes U and x0 in the arithmetic
+// type (tsize bytes: 8 FP64, 4 mixed FP32), the tables and the neighbours.
+__host__ __device__ inline size_t mc_win_bytes(const CnnGeom& g, int cm, int cs, int tsize = 8) {
+  const size_t t = (size_t)g.nl * cm * cs + g.n;  // U, x0
   const size_t i32 = (size_t)g.n + 2 * (size_t)mc_win_tab_len(g) + 2 * (size_t)mc_win_pos_len(g);
-  return 8 * d + 4 * i32 + 2 * (size_t)g.n * g.FF;
+  return 8 * 16 + (size_t)tsize * t + 4 * i32 + 2 * (size_t)g.n * g.FF;
 }
 
 // Input-location table of the proposal at `site` (all threads of the block).
@@ -242,8 +188,11 @@ inline int mc_win_cs(const CnnGeom& g) {
   return 0;
 }
 
-template <int CM, int G, int CS>
-__global__ void __launch_bounds__(64 * G) mc_cnn_win(CnnGeom g, const double* __restrict__ Wpk, CnnPack pk,
+// T = double: the FP64 form (acceptance decisions bit-exact against the
+// oracle); T = float: the mixed-precision form (FP32 window recomputation,
+// the change of log psi summed in FP64 before the acceptance test).
+template <int CM, int G, int CS, typename T>
+__global__ void __launch_bounds__(64 * G) mc_cnn_win(CnnGeom g, const T* __restrict__ Wpk, CnnPack pk,
                                                      jv_mc_args a, int32_t* __restrict__ walkers,
                                                      int32_t* __restrict__ configs,
                                                      int64_t* __restrict__ counters) {
@@ -254,14 +203,14 @@ __global__ void __launch_bounds__(64 * G) mc_cnn_win(CnnGeom g, const double* __
   const int chain = blockIdx.x;
   const int L1 = g.nl - 1;
   const int tlen = mc_win_tab_len(g), plen = mc_win_pos_len(g);
-  double* U = lds;
-  double* x0 = U + (size_t)g.nl * CM * CS;
-  double* red = x0 + g.n;
-  int32_t* s = (int32_t*)(red + 16);
+  double* red = lds;
+  T* U = (T*)(red + 16);
+  T* x0 = U + (size_t)g.nl * CM * CS;
+  int32_t* s = (int32_t*)(x0 + g.n);
   int32_t* tabs = s + g.n;       // [2][tlen]
   int32_t* poss = tabs + 2 * tlen;  // [2][plen]
   short* nbr = (short*)(poss + 2 * plen);
-  const cdbl* Wp = as_const(Wpk);
+  const cst_t<T>* Wp = (const cst_t<T>*)Wpk;
   for (int t = tid; t < g.n * g.FF; t += nt) {
     const int p = t / g.FF, k = t % g.FF;
     const int i = p / g.Ly, j = p % g.Ly, ky = k / g.Fy, kx = k % g.Fy;
@@ -269,10 +218,10 @@ __global__ void __launch_bounds__(64 * G) mc_cnn_win(CnnGeom g, const double* __
   }
   for (int i = tid; i < g.n; i += nt) {
     s[i] = walkers[(int64_t)chain * g.n + i];
-    x0[i] = (double)(2 * s[i] - 1);
+    x0[i] = (T)(2 * s[i] - 1);
   }
   __syncthreads();
-  win_forward<CM, G>(g, Wp, pk, x0, U, CM * CS, CS, nbr, tid, nt, [] { __syncthreads(); });
+  win_forward<CM, G, T>(g, Wp, pk, x0, U, CM * CS, CS, nbr, tid, nt, [] { __syncthreads(); });
   const int total = a.therm_steps + a.samples_per_chain * a.sweep_steps;
   // per-lane batch of 64 draws
   int dsite = 0;
@@ -298,30 +247,30 @@ __global__ void __launch_bounds__(64 * G) mc_cnn_win(CnnGeom g, const double* __
     int toff = 0, poff = 0;
     for (int l = 0; l < g.nl; ++l) {
       const int wsz = win_ex(g, l) * win_ey(g, l);
-      const cdbl* K = Wp + pk.koff[l];
-      const cdbl* B = Wp + pk.boff[l] + w * CG;
-      const double* X = l == 0 ? x0 : U + (size_t)(l - 1) * CM * CS;
-      double* Ul = U + (size_t)l * CM * CS + (size_t)w * CG * CS;
+      const cst_t<T>* K = Wp + pk.koff[l];
+      const cst_t<T>* B = Wp + pk.boff[l] + w * CG;
+      const T* X = l == 0 ? x0 : U + (size_t)(l - 1) * CM * CS;
+      T* Ul = U + (size_t)l * CM * CS + (size_t)w * CG * CS;
       for (int j = lane; j < wsz; j += WAVE) {
-        double z[CG];
+        T z[CG];
 #pragma unroll
         for (int c = 0; c < CG; ++c) z[c] = B[c];
         const int* tj = tab + toff + j * g.FF;
         if (l == 0) {
           for (int k = 0; k < g.FF; ++k) {
             const int q = tj[k];
-            const double xv = q == site ? -x0[q] : x0[q];
-            const cdbl* wk = K + (k * G + w) * CG;
+            const T xv = q == site ? -x0[q] : x0[q];
+            const cst_t<T>* wk = K + (k * G + w) * CG;
 #pragma unroll
             for (int c = 0; c < CG; ++c) z[c] = fma(xv, wk[c], z[c]);
           }
         } else {
           for (int k = 0; k < g.FF; ++k) {
-            const double* xq = X + tj[k];
-            const cdbl* wk = K + (size_t)(k * G + w) * CM * CG;
+            const T* xq = X + tj[k];
+            const cst_t<T>* wk = K + (size_t)(k * G + w) * CM * CG;
 #pragma unroll
             for (int ci = 0; ci < CM; ++ci) {
-              const double xv = xq[ci * CS];
+              const T xv = xq[ci * CS];
 #pragma unroll
               for (int c = 0; c < CG; ++c) z[c] = fma(xv, wk[ci * CG + c], z[c]);
             }
@@ -334,9 +283,9 @@ __global__ void __launch_bounds__(64 * G) mc_cnn_win(CnnGeom g, const double* __
           const int p = pos[poff + j];
 #pragma unroll
           for (int c = 0; c < CG; ++c) {
-            const double v = log_cosh(z[c]);
+            const T v = log_cosh(z[c]);
             Ul[c * CS + g.n + j] = v;
-            diff += v - Ul[c * CS + p];
+            diff += (double)(v - Ul[c * CS + p]);
           }
         }
       }
@@ -359,7 +308,7 @@ __global__ void __launch_bounds__(64 * G) mc_cnn_win(CnnGeom g, const double* __
         const int wsz = win_ex(g, l) * win_ey(g, l);
         for (; t < CM * wsz; t += nt) {
           const int c = t / wsz, j = t - c * wsz;
-          double* Uc = U + ((size_t)l * CM + c) * CS;
+          T* Uc = U + ((size_t)l * CM + c) * CS;
           Uc[pos[po + j]] = Uc[g.n + j];
         }
         t -= CM * wsz;
@@ -450,7 +399,7 @@ int jv_mc_sample(const jv_net* net, const double* params, const jv_mc_args* args
   CnnGeom cg;
   if (make_rbm_geom(net, rg)) {
     const int nw = 4;
-    size_t lds = nw * 8 * ((size_t)3 * rg.H + (rg.L + 1) / 2);
+    size_t lds = nw * 8 * mc_rbm_lds_doubles(rg);
     int rc = set_lds_limit((const void*)mc_rbm, lds);
     if (rc) return rc;
     ScopedTimer tm("mc_sweep", st);
@@ -458,25 +407,30 @@ int jv_mc_sample(const jv_net* net, const double* params, const jv_mc_args* args
                        params, a, walkers, configs, counters);
     JV_LAUNCH_CHECK("mc_rbm");
   } else if (make_cnn_geom(net, cg) && win_ok(cg) && cnn_cmax(cg) > 0 && mc_win_cs(cg) > 0 &&
-             mc_win_bytes(cg, cnn_cmax(cg), mc_win_cs(cg)) <= 160 * 1024) {
+             mc_win_bytes(cg, cnn_cmax(cg), mc_win_cs(cg), net->precision == JV_PREC_MIXED ? 4 : 8) <= 160 * 1024) {
+    const bool mixed = net->precision == JV_PREC_MIXED;
     const int cm = cnn_cmax(cg), cs = mc_win_cs(cg);
     const int groups = cm >= 8 ? 4 : 2;
-    const size_t lds = mc_win_bytes(cg, cm, cs);
+    const size_t lds = mc_win_bytes(cg, cm, cs, mixed ? 4 : 8);
     const CnnPack pk = make_cnn_pack(cg, cm, groups);
     ScopedTimer tm("mc_sweep", st);
-    const double* wpk = pack_cnn_weights(cg, pk, params, st, 41);
+    const void* wpk = mixed ? (const void*)pack_cnn_weights<float>(cg, pk, params, st, 43)
+                            : (const void*)pack_cnn_weights<double>(cg, pk, params, st, 41);
     if (!wpk) return JV_E_NOMEM;
     int rc = JV_OK;
     const void* fn = nullptr;
-#define JV_MCW(CMv, Gv, CSv)                                                                            \
-  if (cm == CMv && cs == CSv) {                                                                         \
-    fn = (const void*)mc_cnn_win<CMv, Gv, CSv>;                                                         \
+#define JV_MCW(CMv, Gv, CSv, Tv)                                                                        \
+  if (cm == CMv && cs == CSv && (sizeof(Tv) == 4) == mixed) {                                          \
+    fn = (const void*)mc_cnn_win<CMv, Gv, CSv, Tv>;                                                     \
     rc = set_lds_limit(fn, lds);                                                                        \
     if (rc) return rc;                                                                                  \
-    hipLaunchKernelGGL((mc_cnn_win<CMv, Gv, CSv>), dim3((unsigned)a.num_chains), dim3(64 * Gv), lds, st, cg, \
-                       wpk, pk, a, walkers, configs, counters);                                         \
+    hipLaunchKernelGGL((mc_cnn_win<CMv, Gv, CSv, Tv>), dim3((unsigned)a.num_chains), dim3(64 * Gv), lds, st, \
+                       cg, (const Tv*)wpk, pk, a, walkers, configs, counters);                          \
   }
-#define JV_MCW_CS(CMv, Gv) JV_MCW(CMv, Gv, 128) JV_MCW(CMv, Gv, 256) JV_MCW(CMv, Gv, 512) JV_MCW(CMv, Gv, 1024)
+#define JV_MCW_CS(CMv, Gv)                                                                              \
+  JV_MCW(CMv, Gv, 128, double) JV_MCW(CMv, Gv, 256, double) JV_MCW(CMv, Gv, 512, double)               \
+  JV_MCW(CMv, Gv, 1024, double) JV_MCW(CMv, Gv, 128, float) JV_MCW(CMv, Gv, 256, float)                \
+  JV_MCW(CMv, Gv, 512, float) JV_MCW(CMv, Gv, 1024, float)
     JV_MCW_CS(4, 2) JV_MCW_CS(8, 4) JV_MCW_CS(16, 4) JV_MCW_CS(32, 4)
 #undef JV_MCW_CS
 #undef JV_MCW

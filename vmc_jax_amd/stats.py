@@ -13,6 +13,8 @@ def _as2d(obs):
     o = obs.reshape(obs.shape[1], -1) if obs.dim() >= 2 else obs.reshape(-1, 1)
     if torch.is_complex(o):
         return o.to(torch.complex128).contiguous(), True
+    if o.dtype == torch.float32:  # the mixed form's FP32 gradients stay FP32
+        return o.contiguous(), False
     return o.to(torch.float64).contiguous(), False
 
 
@@ -29,8 +31,14 @@ class SampledObs:
         o, cpx = _as2d(observations)
         n, m = o.shape
         self._cpx = cpx
-        self._mean = torch.empty(m, dtype=o.dtype, device=o.device)
         self._data = torch.empty_like(o)
+        if o.dtype == torch.float32:
+            # mean accumulated in FP64, centred data rounded to FP32
+            self._mean = torch.empty(m, dtype=torch.float64, device=o.device)
+            _lib.call("jv_center_f32", _lib.ptr(o), _lib.ptr(self._weights), n, m, 0, _lib.ptr(self._mean),
+                      _lib.ptr(self._data), _lib.stream_ptr())
+            return
+        self._mean = torch.empty(m, dtype=o.dtype, device=o.device)
         _lib.call("jv_center", _lib.ptr(o), _lib.ptr(self._weights), n, m, int(cpx), _lib.ptr(self._mean),
                   _lib.ptr(self._data), _lib.stream_ptr())
 

@@ -32,13 +32,13 @@ class MinSR:
         self.last_info = None
         self._chol_skip = 0
 
-    # Real dual solves try the Cholesky route first (exact when no eigenvalue
-    # falls below the cutoff).  After a refused attempt, the attempt is skipped
-    # for the next CHOLESKY_RETRY - 1 solves: the spectrum changes slowly from
-    # step to step, and a refused attempt costs a factorisation.  A skipped
-    # solve whose eigendecomposition cut nothing makes the next solve try
-    # again.  The T form tries on every solve that can pass.  The update is the
-    # same either way.
+    # Solves try the Cholesky route first (exact when no eigenvalue falls below
+    # the cutoff).  After a refused attempt, the attempt is skipped for the
+    # next CHOLESKY_RETRY - 1 solves: the spectrum changes slowly from step to
+    # step, and a refused attempt costs a factorisation (and, in the T form, a
+    # host synchronisation before the eigendecomposition).  A skipped solve
+    # whose eigendecomposition cut nothing the route would not make the next
+    # solve try again.  The update is the same either way.
     CHOLESKY_RETRY = 8
     # The holomorphic T form (complex T = G G^dagger, N < P) tries the route
     # too: centred, uniformly weighted samples leave T one null vector, the
@@ -76,8 +76,11 @@ class MinSR:
         return float(np.real(self.ElocMean0))
 
     def solve(self, eloc, gradients, holomorphic, psi=None):
-        """minsr.py:53-80 on centred data; gradients in the compact layout."""
-        D = gradients._data.contiguous()
+        """minsr.py:53-80 on centred data; gradients in the compact layout (FP32
+        data of the mixed form are widened to FP64 here; the mixed Gram runs on
+        the overlapped path of __call__)."""
+        D = gradients._data.to(torch.float64).contiguous() if gradients._data.dtype == torch.float32 \
+            else gradients._data.contiguous()
         e = eloc._data.reshape(-1).to(torch.complex128).contiguous()
         if not holomorphic and torch.is_complex(D):
             # real parameters with a complex output (a SymNet with complex factors):
@@ -97,12 +100,13 @@ class MinSR:
             # T form: a refused attempt costs one factorisation beside an
             # eigendecomposition many times its price, so every solve that can
             # pass tries (T has rank <= P: with N - 1 > P it is always cut)
-            hint = int((not cpx or self.try_cholesky_holomorphic) and n - 1 <= p)
+            hint = int((not cpx or self.try_cholesky_holomorphic) and n - 1 <= p) and self._cholesky_hint()
             _lib.call("jv_minsr_solve", _lib.ptr(D), _lib.ptr(e), n, p, int(cpx), float(self.pinvTol),
                       float(self.diagonalShift), hint, _lib.ptr(upd), ctypes.byref(info), _lib.stream_ptr())
         self.last_info = info
-        if dual:
-            self._note_route(hint, p)
+        # back-off after a refused attempt; a spectrum the route accepts keeps
+        # P (dual) or N - 1 (T form, the deflated centring vector) eigenvalues
+        self._note_route(hint, p if dual else n - 1)
         if cpx:
             # T_ref = 2 G G^dagger: pinv scales by 1/2; [v, -i v] layout per leaf
             return psi.expand_compact(0.5 * upd)
@@ -125,7 +129,9 @@ class MinSR:
         upd = torch.empty(P, dtype=torch.float64, device=D.device)
         info = _lib.JvSolveInfo()
         hint = self._cholesky_hint()
-        _lib.call("jv_minsr_solve_dual_deferred", _lib.ptr(D), _lib.ptr(Eraw), _lib.ptr(p), ready._as_parameter_,
+        # FP32 data (the mixed form): compensated-BF16 Gram, FP64 solve
+        fn = "jv_minsr_solve_dual_deferred_mixed" if D.dtype == torch.float32 else "jv_minsr_solve_dual_deferred"
+        _lib.call(fn, _lib.ptr(D), _lib.ptr(Eraw), _lib.ptr(p), ready._as_parameter_,
                   n, P, 0, float(self.pinvTol), float(self.diagonalShift), hint, _lib.ptr(upd), ctypes.byref(info),
                   _lib.stream_ptr())
         self.last_info = info

@@ -23,7 +23,16 @@ def _as_configs(s):
 
 
 class NQS:
-    def __init__(self, net, logarithmic=True, batchSize=1000, seed=1234, orbit=None, avgFun=None):
+    def __init__(self, net, logarithmic=True, batchSize=1000, seed=1234, orbit=None, avgFun=None, precision="fp64"):
+        """``precision`` (extension, not in jVMC): "fp64" evaluates the network in
+        FP64 as the reference does; "mixed" runs the CNN's Metropolis and
+        local-energy window kernels in FP32 (log psi changes and O_loc summed in
+        FP64) and returns the compact gradients (``layout=0``) in FP32, which
+        the MinSR solve turns into the compensated-BF16 Gram; networks without
+        an FP32 form keep FP64."""
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        self.precision = precision
         if isinstance(net, (tuple, list)):
             raise NotImplementedError("two-network ansatz (TwoNets) is not implemented")
         if not logarithmic:
@@ -50,6 +59,7 @@ class NQS:
         shape = tuple(int(x) for x in s.shape[2:])
         self.sampleShape = shape
         self.desc = self.net.descriptor(shape)
+        self.desc.precision = _lib.JV_PREC_MIXED if self.precision == "mixed" else _lib.JV_PREC_FP64
         _lib.check(_lib.lib.jv_net_init(self.desc), "jv_net_init")
         self.holomorphic = bool(self.desc.is_complex)
         self.realParams = not self.holomorphic
@@ -151,6 +161,11 @@ class NQS:
             raise NotImplementedError("gradients of a non-holomorphic complex-parameter network "
                                       "(avgFun_Coefficients_Sep over a complex network) are not implemented")
         n = flat.shape[0]
+        if layout == 0 and self.fp32_gradients:
+            g = torch.empty((n, self.numParameters), dtype=torch.float32, device=flat.device)
+            _lib.call("jv_net_gradients_f32", self.desc, _lib.ptr(self._params), _lib.ptr(flat), n, _lib.ptr(g), None,
+                      _lib.stream_ptr())
+            return g.reshape(s.shape[:2] + g.shape[1:])
         if layout == 1:
             g = torch.empty((n, self.numRealParameters), dtype=torch.complex128, device=flat.device)
         elif self._cpx_weights:
@@ -164,6 +179,12 @@ class NQS:
             _lib.call("jv_net_gradients", self.desc, _lib.ptr(self._params), _lib.ptr(flat), n, layout,
                       _lib.ptr(g), None, _lib.stream_ptr())
         return g.reshape(s.shape[:2] + g.shape[1:])
+
+    @property
+    def fp32_gradients(self):
+        """The mixed form of a CNN returns compact gradients in FP32."""
+        return (self.precision == "mixed" and self._sym is None and self.desc is not None
+                and self.desc.kind == _lib.JV_NET_CNN)
 
     def get_sampler_net(self):
         """vqs.py:337-352: Re log psi and the current parameters."""
