@@ -294,6 +294,17 @@ int jv_center(const double* obs, const double* w, int64_t n, int64_t p, int32_t 
 int jv_center_f32(const float* obs, const double* w, int64_t n, int64_t p, int32_t is_complex, double* mean,
                   float* data, void* stream);
 
+/* jv_center without any allocation, for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  The partial column sums live
+ * in a fixed per-device array of 2^21 doubles (16 MiB, held until the process
+ * ends); ceil(n / 256) * p * (is_complex ? 2 : 1) must not exceed it, else
+ * JV_E_ARG.  Every call on a device shares that array: calls that overlap on
+ * one device (different streams or threads) race and must be serialised by
+ * the caller.  Same results as jv_center. */
+int jv_center_noalloc(const double* obs, const double* w, int64_t n, int64_t p, int32_t is_complex, double* mean,
+                      double* data, void* stream);
+
 /* Column sums of squares: out[j] = sum_n |data[n][j]|^2.
  * Replaces SampledObs.var (stats.py:248-252). */
 int jv_col_norm2(const double* data, int64_t n, int64_t p, int32_t is_complex, double* out, void* stream);

@@ -92,6 +92,7 @@ _SIGS = {
     "jv_mc_weights": [_vp, _i64, _dbl, _dbl, _vp, _vp],
     "jv_center": [_vp, _vp, _i64, _i64, _i32, _vp, _vp, _vp],
     "jv_center_f32": [_vp, _vp, _i64, _i64, _i32, _vp, _vp, _vp],
+    "jv_center_noalloc": [_vp, _vp, _i64, _i64, _i32, _vp, _vp, _vp],
     "jv_col_norm2": [_vp, _i64, _i64, _i32, _vp, _vp],
     "jv_gram_nn": [_vp, _i64, _i64, _i32, _vp, _vp],
     "jv_gram_pp": [_vp, _i64, _i64, _i32, _vp, _vp],

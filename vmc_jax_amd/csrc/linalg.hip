@@ -108,6 +108,32 @@ int center(const double* O, const double* w, int64_t n, int64_t p, int cpx, doub
   return center_with(O, w, n, p, cpx, mean, D, part, st);
 }
 
+// Partial sums of jv_center_noalloc: a fixed array per device, reached through
+// its device address (hipGetSymbolAddress), resolved once per device.  Its
+// name in host code is only a host-side stand-in and is never passed to a
+// kernel or a copy.
+constexpr int64_t CENTER_NOALLOC_MAX_PARTIALS = 2 * 1024 * 1024;  // doubles (16 MiB per device)
+static __device__ __align__(256) double scratch[CENTER_NOALLOC_MAX_PARTIALS];
+
+constexpr int MAX_SCRATCH_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[MAX_SCRATCH_DEVICES] = {};
+
+int center_scratch(double** out) {
+  int d = 0;
+  JV_HIP(hipGetDevice(&d));
+  if (d < 0 || d >= MAX_SCRATCH_DEVICES) return fail(JV_E_ARG, "jv_center_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[d]) {
+    void* p = nullptr;
+    JV_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(JV_E_HIP, "jv_center_noalloc: hipGetSymbolAddress returned NULL");
+    g_scratch_ptr[d] = (double*)p;
+  }
+  *out = g_scratch_ptr[d];
+  return JV_OK;
+}
+
 // ------------------------------------------------------------------ GEMM (FP64 MFMA)
 // C[M][N] = alpha * opA * opB (+ beta C), opA(m, k) = A[m*sam + k*sak],
 // opB(k, n) = B[k*sbk + n*sbn] (strides in elements; complex elements are two
@@ -781,6 +807,20 @@ int jv_center_f32(const float* obs, const double* w, int64_t n, int64_t p, int32
   double* part = (double*)workspace(sizeof(double) * (size_t)center_partials(n, p, is_complex), 2);
   if (!part) return fail(JV_E_NOMEM, "jv_center_f32: workspace allocation failed");
   return center_with<float>(obs, w, n, p, is_complex, mean, data, part, (hipStream_t)stream);
+}
+
+int jv_center_noalloc(const double* obs, const double* w, int64_t n, int64_t p, int32_t is_complex, double* mean,
+                      double* data, void* stream) {
+  if (!obs || !w || !mean || !data || n < 1 || p < 1) return fail(JV_E_ARG, "jv_center_noalloc: bad arguments");
+  const int64_t need = center_partials(n, p, is_complex);
+  if (need > CENTER_NOALLOC_MAX_PARTIALS)
+    return fail(JV_E_ARG, "jv_center_noalloc: ceil(n / 256) * p * (is_complex ? 2 : 1) = " + std::to_string(need) +
+                              " partial sums exceed the fixed scratch of " +
+                              std::to_string(CENTER_NOALLOC_MAX_PARTIALS) + " doubles; use jv_center");
+  double* part = nullptr;
+  int rc = center_scratch(&part);
+  if (rc) return rc;
+  return center_with(obs, w, n, p, is_complex, mean, data, part, (hipStream_t)stream);
 }
 
 int jv_gram_nn(const double* data, int64_t n, int64_t p, int32_t is_complex, double* T, void* stream) {
