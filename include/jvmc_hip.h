@@ -471,6 +471,56 @@ int jv_subset(const double* data, const double* w, const double* mean, int64_t n
 int jv_tdvp_cross_validate(const double* D, const double* e, int64_t n, int64_t pr, const double* x,
                            int32_t make_real, double diag_shift, const double* update, double* out, void* stream);
 
+/* ------------------------------------------------------------------ batched field sweep */
+/* Many independent RBM states (uneven lattice sizes, hidden-unit counts,
+ * operators and sample counts) advanced by one launch per phase: a
+ * device-resident instance table maps each workgroup to one instance, which
+ * runs the same per-chain / per-sample routines as jv_mc_sample, jv_eloc and
+ * jv_net_gradients.  No counterpart in the reference, where each state is a
+ * separate jVMC driver (sampler.py:301-358, operator/base.py:166-192,
+ * vqs.py:256-287 per instance). */
+#define JV_BATCH_SAMPLE 0  /* Metropolis pass (then the Philox call index advances) */
+#define JV_BATCH_GRAD 1    /* log psi and O_k of the emitted samples */
+#define JV_BATCH_ELOC 2    /* O_loc of the emitted samples */
+
+typedef struct jv_batch_item {
+  jv_net net;                /* an RBM or CpxRBM after jv_net_init */
+  const double* params;      /* device flat parameters, read at every launch: update in place */
+  jv_optable op;             /* device operator tables */
+  const double* prefactor;   /* device complex [num_ops] */
+  jv_mc_args mc;             /* call_index: the first pass's; the batch advances it on the device */
+  int32_t* walkers;          /* [num_chains][L], persistent */
+  int32_t* configs;          /* [samples_per_chain * num_chains][L], sample index t*C + c */
+  double* logpsi;            /* complex [n] */
+  int64_t* counters;         /* [2] proposed, accepted of the last pass */
+  double* eloc;              /* complex [n] */
+  double* grad;              /* gradient rows in grad_layout (jv_net_gradients), may be NULL */
+  int32_t grad_layout;       /* 0 compact, 1 jVMC */
+  int32_t reserved;
+} jv_batch_item;
+
+typedef struct jv_batch jv_batch;
+typedef struct jv_graph jv_graph;
+
+/* Validates the items and uploads the instance table (synchronous). */
+int jv_batch_create(const jv_batch_item* items, int32_t count, jv_batch** out);
+int jv_batch_destroy(jv_batch* batch);
+/* Launch shape of a phase: waves per workgroup (1, 2, 4, 8) and, for the
+ * gradient and O_loc phases, samples per wave (1..64); re-uploads the table.
+ * Chosen by the package's autotuner (vmc_jax_amd/batch.py). */
+int jv_batch_set_shape(jv_batch* batch, int32_t phase, int32_t waves, int32_t samples_per_wave);
+/* One phase for every instance. */
+int jv_batch_phase(jv_batch* batch, int32_t phase, void* stream);
+/* Counters cleared, then sampling, gradients (+ log psi) and O_loc for every
+ * instance: 5 launches, no allocation, host synchronisation or memset. */
+int jv_batch_step(jv_batch* batch, void* stream);
+/* jv_batch_step captured into a HIP graph on `stream` (not the default
+ * stream) and instantiated; jv_graph_launch replays it (parameters, walkers
+ * and the Philox call index are read from device memory at replay). */
+int jv_graph_capture(jv_batch* batch, void* stream, jv_graph** out);
+int jv_graph_launch(jv_graph* graph, void* stream);
+int jv_graph_destroy(jv_graph* graph);
+
 #ifdef __cplusplus
 }
 #endif

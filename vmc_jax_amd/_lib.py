@@ -66,6 +66,17 @@ class JvSolveInfo(ctypes.Structure):
                 ("e_var", _dbl), ("tdvp_error", _dbl), ("e_mean", _dbl * 2)]
 
 
+JV_BATCH_SAMPLE = 0
+JV_BATCH_GRAD = 1
+JV_BATCH_ELOC = 2
+
+
+class JvBatchItem(ctypes.Structure):
+    _fields_ = [("net", JvNet), ("params", _vp), ("op", JvOpTable), ("prefactor", _vp), ("mc", JvMcArgs),
+                ("walkers", _vp), ("configs", _vp), ("logpsi", _vp), ("counters", _vp), ("eloc", _vp),
+                ("grad", _vp), ("grad_layout", _i32), ("reserved", _i32)]
+
+
 # name -> argtypes; every function returns int status
 _SIGS = {
     "jv_version": [],
@@ -113,6 +124,14 @@ _SIGS = {
     "jv_tdvp_solve_holomorphic": [_vp, _vp, _vp, _i64, _i64, _vp, _vp, _i64, ctypes.POINTER(_dbl), _i32, _dbl, _i32,
                                   _dbl, _dbl, _dbl, _dbl, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(JvSolveInfo), _vp],
     "jv_subset": [_vp, _vp, _vp, _i64, _i64, _i32, _i64, _i64, _i64, _vp, _vp, _vp, _vp],
+    "jv_batch_create": [ctypes.POINTER(JvBatchItem), _i32, ctypes.POINTER(_vp)],
+    "jv_batch_destroy": [_vp],
+    "jv_batch_set_shape": [_vp, _i32, _i32, _i32],
+    "jv_batch_phase": [_vp, _i32, _vp],
+    "jv_batch_step": [_vp, _vp],
+    "jv_graph_capture": [_vp, _vp, ctypes.POINTER(_vp)],
+    "jv_graph_launch": [_vp, _vp],
+    "jv_graph_destroy": [_vp],
     "jv_tdvp_cross_validate": [_vp, _vp, _i64, _i64, ctypes.POINTER(_dbl), _i32, _dbl, _vp, ctypes.POINTER(_dbl),
                                _vp],
 }

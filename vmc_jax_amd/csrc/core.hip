@@ -250,7 +250,7 @@ std::map<std::string, TimerRec> g_timers;
 bool timing_enabled() { return g_timing; }
 
 void* timer_begin(const char* name, hipStream_t st) {
-  if (!g_timing) return nullptr;
+  if (!g_timing || !name) return nullptr;
   hipEvent_t a, b;
   if (hipEventCreate(&a) != hipSuccess) return nullptr;
   if (hipEventCreate(&b) != hipSuccess) { (void)hipEventDestroy(a); return nullptr; }

@@ -189,32 +189,66 @@ __global__ void __launch_bounds__(256) gram3_tn(const T* __restrict__ X, int64_t
 }
 
 // S (FP64, row-major, ld = p) from the tile partials: real S[i][j] = R[i][j];
-// complex (cpx) S = A with the 2 x 2 combination above.  The lower triangle
-// is the mirror (conjugate) of the upper; slices are summed in slice order.
-__global__ void gram3_reduce(const float* __restrict__ part, int splits, int ntile_side, int64_t p, int cpx,
-                             double* __restrict__ S) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= p * p) return;
-  const int64_t i = t / p, j = t % p;
-  const int64_t a = std::min(i, j), b = std::max(i, j);
+// complex (cpx) S = A with the 2 x 2 combination above.  One workgroup per
+// 32 x 32 block of an upper tile: partial sums read along rows (coalesced),
+// added in slice order in FP64 and staged in LDS, so both the block and its
+// mirror (transpose, conjugate for complex data) are written along rows.  Only
+// entries on or above the diagonal of R are used, and the lower triangle of S
+// is their exact mirror.
+__global__ void __launch_bounds__(256) gram3_reduce(const float* __restrict__ part, int splits, int ntile_side,
+                                                    int64_t p, int cpx, double* __restrict__ S) {
+  __shared__ double M[32][33];
+  int tm, tn;
+  upper_tile(blockIdx.x / 16, ntile_side, tm, tn);
+  const int sa = (blockIdx.x % 16) / 4, sb = blockIdx.x % 4;
+  if (tm == tn && sa > sb) return;  // below the diagonal of a diagonal tile: the mirror of (sb, sa)
   const int ntiles = ntile_side * (ntile_side + 1) / 2;
-  auto rv = [&](int64_t r, int64_t c) -> double {  // R[r][c], r <= c tile-wise
-    const int tm = (int)(r / GT), tn = (int)(c / GT);
-    const int tile = tm * ntile_side - tm * (tm - 1) / 2 + (tn - tm);
-    const size_t off = (size_t)tile * GT * GT + (size_t)(r % GT) * GT + (size_t)(c % GT);
-    double s = 0.0;
-    for (int k = 0; k < splits; ++k) s += (double)part[(size_t)k * ntiles * GT * GT + off];
-    return s;
-  };
+  const int tile = blockIdx.x / 16;
+  const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;
+  const int64_t q = p * (cpx ? 2 : 1);
+  const int64_t r0 = (int64_t)tm * GT + 32 * sa, c0 = (int64_t)tn * GT + 32 * sb;  // R indices of the block
+  const size_t stride = (size_t)ntiles * GT * GT;
+  for (int k = 0; k < 4; ++k) {
+    const int i = ty + 8 * k;
+    const size_t off = (size_t)tile * GT * GT + (size_t)(32 * sa + i) * GT + (32 * sb + tx);
+    double v = 0.0;
+    for (int sp = 0; sp < splits; ++sp) v += (double)part[sp * stride + off];
+    M[i][tx] = v;
+  }
+  __syncthreads();
   if (!cpx) {
-    S[t] = rv(a, b);
+    for (int k = 0; k < 4; ++k) {
+      const int i = ty + 8 * k;
+      const int64_t r = r0 + i, c = c0 + tx;
+      if (r < p && c < p && r <= c) S[r * p + c] = M[i][tx];
+      // mirror: S[c0 + i][r0 + tx] = R[r0 + tx][c0 + i]
+      const int64_t mr = c0 + i, mc = r0 + tx;
+      if (mr < p && mc < p && mc < mr) S[mr * p + mc] = M[tx][i];
+    }
     return;
   }
-  const double re = rv(2 * a, 2 * b) + rv(2 * a + 1, 2 * b + 1);
-  double im = rv(2 * a, 2 * b + 1) - rv(2 * a + 1, 2 * b);
-  if (i > j) im = -im;
-  S[2 * t] = re;
-  S[2 * t + 1] = i == j ? 0.0 : im;
+  // complex: 16 x 16 outputs of the block, A = r0 / 2 + a, B = c0 / 2 + b
+  {
+    const int a = threadIdx.x / 16, b = threadIdx.x % 16;
+    const int64_t A = r0 / 2 + a, B = c0 / 2 + b;
+    if (2 * A < q && 2 * B < q && A <= B) {
+      const double re = M[2 * a][2 * b] + M[2 * a + 1][2 * b + 1];
+      const double im = A == B ? 0.0 : M[2 * a][2 * b + 1] - M[2 * a + 1][2 * b];
+      S[2 * (A * p + B)] = re;
+      S[2 * (A * p + B) + 1] = im;
+    }
+  }
+  {
+    const int a = threadIdx.x % 16, b = threadIdx.x / 16;  // mirror S[B][A], consecutive threads along A
+    const int64_t A = r0 / 2 + a, B = c0 / 2 + b;
+    if (2 * A < q && 2 * B < q && A < B) {
+      const double re = M[2 * a][2 * b] + M[2 * a + 1][2 * b + 1];
+      const double im = M[2 * a][2 * b + 1] - M[2 * a + 1][2 * b];
+      S[2 * (B * p + A)] = re;
+      S[2 * (B * p + A) + 1] = -im;
+    }
+  }
+  (void)q;
 }
 
 int set_lds_limit(const void* fn, size_t bytes);
@@ -254,8 +288,7 @@ int gram_pp_bf16x3(const void* D, int f32, int64_t n, int64_t p, int cpx, double
   }
 #undef JV_G3
   JV_LAUNCH_CHECK("gram3_tn");
-  hipLaunchKernelGGL(gram3_reduce, dim3((unsigned)((p * p + 255) / 256)), dim3(256), 0, st, part, (int)splits, nts, p,
-                     cpx, S);
+  hipLaunchKernelGGL(gram3_reduce, dim3((unsigned)(ntiles * 16)), dim3(256), 0, st, part, (int)splits, nts, p, cpx, S);
   JV_LAUNCH_CHECK("gram3_reduce");
   return JV_OK;
 }
