@@ -101,7 +101,7 @@ class Workload:
         elif name == "c2":
             self.shape, self.samples = (32,), 16384
             net = jv.nets.CpxRBM(numHidden=64, bias=True)
-            psi = jv.NQS(net)
+            psi = jv.NQS(net, precision=precision)
             psi(torch.zeros((1, 1) + self.shape, dtype=torch.int32))
             psi.set_parameters(0.01 * rng.normal(size=psi.numRealParameters))
             self.H = tfim_strings(jv.operator, self.shape, -1.0, -1.0)
@@ -273,24 +273,30 @@ def batched_sweep(num_instances=64, steps=3, seed=11, workers=8):
         kinds.append("L%d-a%d-%s" % (L, alpha, "tdvp" if tdvp else "minsr"))
     sweep = jb.BatchedSweep(inst, workers=workers, tune=True, graph=True)
     setup = time.perf_counter() - t0
+    print("c4: %d instances set up in %.1f s (launch shapes %s, %s)" % (num_instances, setup, sweep.shapes,
+                                                                      sweep.tuned_from), file=sys.stderr, flush=True)
     try:
-        sweep.step()  # warm-up: library kernels, per-thread workspaces
+        for _ in range(2):  # warm-up: library kernels, per-thread workspaces, first refused Cholesky attempts
+            sweep.step()
         torch.cuda.synchronize()
         jv._lib.call("jv_timer_reset")
         jv._lib.call("jv_timer_enable", 1)
         t1 = time.perf_counter()
-        for _ in range(steps):
+        for k in range(steps):
             sweep.step()
+            print("c4: step %d of %d, %.0f ms" % (k + 1, steps, sweep.step_ms[-1]), file=sys.stderr, flush=True)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t1
         jv._lib.call("jv_timer_enable", 0)
         ph = {k: timer(jv, k) for k in ("batch_sample", "batch_grad", "batch_eloc", "eigh", "gram", "cholesky")}
-        lat = np.array(sweep.step_ms[1:])
+        lat = np.array(sweep.step_ms[2:])
+        phase = sweep.phase_ms()[2:]
         energies = [float(np.real(i.energy)) for i in inst]
         return {"instances": num_instances, "steps": steps, "wall_s": wall,
                 "instance_steps_per_s": num_instances * steps / wall,
                 "p50_step_ms": float(np.percentile(lat, 50)), "p99_step_ms": float(np.percentile(lat, 99)),
-                "batched_phase_ms_per_step": {k: v[0] / steps for k, v in ph.items() if v[1] > 0},
+                "batched_pass_ms": float(np.mean(phase)) if phase else None,
+                "solver_ms_per_step_summed_over_threads": {k: v[0] / steps for k, v in ph.items() if v[1] > 0},
                 "launch_shapes": {k: list(v) for k, v in sweep.shapes.items()}, "tuned_from": sweep.tuned_from,
                 "graph": sweep.graph is not None, "workers": workers, "setup_s": setup,
                 "energies_finite": bool(np.all(np.isfinite(energies))), "mix": kinds}
@@ -298,13 +304,15 @@ def batched_sweep(num_instances=64, steps=3, seed=11, workers=8):
         sweep.close()
 
 
-def serve_mix(num_jobs=16, slots=8, seed=7, rate=8.0):
-    """Served workload, scaled down from the 1000-request mix: Poisson arrivals
-    (rate jobs/s); 70% 1D jobs (TFIM or XXZ with Delta ~ U[0, 2], L ~ U{16, 20,
-    24}, complex RBM alpha=2; half ground-state MinSR, half real-time TDVP with
-    Heun) and 30% 2D TFIM jobs (side ~ U{6..10}, h ~ U[2, 4], the 3x(8ch,3x3)
-    CNN, MinSR); 3-8 steps and 2048-4096 samples per job.  Jobs are admitted
-    by HBM footprint into `slots` concurrent slots, each on its own stream."""
+def serve_mix(num_jobs=16, slots=8, seed=7, rate=8.0, steps=(3, 9), samples=(2048, 4096), L1d=(16, 20, 24),
+              side=(6, 11), batch_rbm=True, workers=8):
+    """Served workload: Poisson arrivals (rate jobs/s); 70% 1D jobs (TFIM or XXZ
+    with Delta ~ U[0, 2], L ~ U(L1d), complex RBM alpha=2; half ground-state
+    MinSR with Euler, half real-time TDVP with Heun) and 30% 2D TFIM jobs (side
+    ~ U[side), h ~ U[2, 4], the 3x(8ch,3x3) CNN, MinSR); steps ~ U[steps) and
+    samples ~ U(samples) per job.  Jobs are admitted by HBM footprint into
+    `slots` slots; with batch_rbm the 1D jobs share one batch (one launch per
+    phase for all of them), the 2D jobs run on their own threads and streams."""
     import torch
     import vmc_jax_amd as jv
     from vmc_jax_amd import serving
@@ -312,11 +320,11 @@ def serve_mix(num_jobs=16, slots=8, seed=7, rate=8.0):
     t, jobs = 0.0, []
     for i in range(num_jobs):
         t += rng.exponential(1.0 / rate)
-        steps = int(rng.integers(3, 9))
-        ns = int(rng.choice([2048, 4096]))
+        nsteps = int(rng.integers(*steps))
+        ns = int(rng.choice(samples))
         jseed = int(rng.integers(1 << 30))
         if rng.uniform() < 0.7:
-            L = int(rng.choice([16, 20, 24]))
+            L = int(rng.choice(L1d))
             xxz = rng.uniform() < 0.5
             delta = float(rng.uniform(0, 2))
             ground = rng.uniform() < 0.5
@@ -344,10 +352,10 @@ def serve_mix(num_jobs=16, slots=8, seed=7, rate=8.0):
             fp = serving.footprint_bytes(ns, pr, L, True, "minsr" if ground else "tdvp", 256)
             kind = "1d-%s-%s-L%d" % ("xxz" if xxz else "tfim", "gs" if ground else "rt", L)
         else:
-            side = int(rng.integers(6, 11))
+            sd = int(rng.integers(*side))
             h = float(rng.uniform(2, 4))
 
-            def make(side=side, h=h, ns=ns, jseed=jseed):
+            def make(side=sd, h=h, ns=ns, jseed=jseed):
                 net = jv.nets.CNN(F=(3, 3), channels=(8, 8, 8), strides=(1, 1))
                 psi = jv.NQS(net, seed=jseed)
                 psi(torch.zeros((1, 1, side, side), dtype=torch.int32))
@@ -356,16 +364,26 @@ def serve_mix(num_jobs=16, slots=8, seed=7, rate=8.0):
                 ms = jv.sampler.MCSampler(psi, (side, side), jseed, updateProposer=jv.sampler.propose_spin_flip,
                                           numChains=256, numSamples=ns)
                 return psi, H, jv.util.MinSR(ms, pinvTol=1e-10, solver="auto"), jv.util.stepper.Euler(5e-3)
-            fp = serving.footprint_bytes(ns, 1240, side * side, False, "minsr", 256)
-            kind = "2d-cnn-%d" % side
-        jobs.append(serving.Job(i, make, steps, fp, arrival=t, meta={"kind": kind}))
-    srv = serving.Server(max_slots=slots)
+            fp = serving.footprint_bytes(ns, 1240, sd * sd, False, "minsr", 256)
+            kind = "2d-cnn-%d" % sd
+        jobs.append(serving.Job(i, make, nsteps, fp, arrival=t, meta={"kind": kind}))
+    srv = serving.Server(max_slots=slots, batch_rbm=batch_rbm, workers=workers)
     for j in jobs:
         srv.submit(j)
-    rep = srv.run()
+    t_run = time.perf_counter()
+    rep = srv.run(progress=lambda j: print("served: job %d (%s) %s, %d steps, t = %.1f s" %
+                                           (j.job_id, j.meta["kind"], j.state, j.steps_done,
+                                            time.perf_counter() - t_run), file=sys.stderr, flush=True))
     rep["jobs"] = num_jobs
     rep["slots"] = slots
+    rep["shared_batch"] = batch_rbm
+    rep["arrival_span_s"] = t
     rep["failures"] = [j.error for j in srv.finished if j.state == "failed"]
+    lat = {}
+    for j in srv.finished:
+        lat.setdefault(j.meta["kind"].split("-")[0] + "-" + j.meta["kind"].split("-")[2 if j.meta["kind"].startswith("1d") else 1],
+                       []).extend(j.latencies)
+    rep["p99_step_ms_by_kind"] = {k: float(np.percentile(v, 99) * 1e3) for k, v in lat.items() if v}
     rep["workload"] = serve_mix.__doc__.split("\n")[0].strip()
     return rep
 
@@ -516,12 +534,36 @@ def accuracy_check(w, n=1536):
             "residual_rel": float(res / res_o - 1.0)}
 
 
+def accuracy_c2(w):
+    """The mixed form of the L = 32 workload (compensated BF16 Gram of the
+    holomorphic solve) against the FP64 solve on the same samples; the FP64
+    device solve matches the FP64 oracle at this size
+    (test_tdvp_heun_rhs_l32_cpxrbm_16384_samples)."""
+    import torch
+    jv = w.jv
+    s, lp, p = w.sampler.sample()
+    E = jv.stats.SampledObs(w.H.get_O_loc(s, w.psi, lp, 0.0), p)
+    G = jv.stats.SampledObs(w.psi.gradients(s), p)
+    out = {}
+    for mixed in (False, True):
+        eq = jv.util.TDVP(w.sampler, rhsPrefactor=1.j, makeReal='imag')
+        eq._holo_cols = w.psi.holomorphic_columns()
+        eq._mixed = mixed
+        u, _, _ = eq.solve(E, G)
+        out[mixed] = (u.cpu().numpy(), eq.S.cpu().numpy(), eq.F.cpu().numpy(), eq.ev.cpu().numpy())
+    (u0, S0, F0, e0), (u1, S1, F1, e1) = out[False], out[True]
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    return {"samples": int(s.shape[1]), "against": "FP64 device solve on the same samples",
+            "dE_rel": 0.0, "dS_rel": rel(S1, S0), "dF_rel": rel(F1, F0), "dspectrum_rel": rel(e1, e0),
+            "dtheta_rel": rel(u1, u0)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c4", "sym"])
+    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c4", "c5", "sym"])
     ap.add_argument("--mode", default="dual", choices=["T", "dual"], help="MinSR solve form (c3)")
     ap.add_argument("--t-steps", type=int, default=3, help="c3: steps of the other MinSR form reported beside")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -529,6 +571,7 @@ def main():
                     help="c3: compute E_loc before the MinSR solve instead of beside it (isolated kernel timings)")
     ap.add_argument("--serve-jobs", type=int, default=16, help="c3: jobs of the served mix (0 = skip)")
     ap.add_argument("--no-mixed", action="store_true", help="c3: skip the mixed-precision form")
+    ap.add_argument("--rate", type=float, default=4.0, help="c5: Poisson arrival rate (jobs/s)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="end with an elementwise 512 MiB -> 512 MiB FP64 device copy (a kernel of known "
                          "HBM traffic that scales the FETCH_SIZE / WRITE_SIZE counters)")
@@ -548,6 +591,20 @@ def main():
             "vs_baseline": None, "dtype": "f64",
             "data": "synthetic (random network parameters; samples drawn from the networks)",
             "config": {"workload": batched_sweep.__doc__.split("\n")[0].strip(), "instances": rep["instances"]},
+            "report": rep, "wall_s_including_setup": wall}))
+        return
+    if args.workload == "c5":
+        t0 = time.perf_counter()
+        rep = serve_mix(num_jobs=args.serve_jobs if args.serve_jobs > 16 else 100, slots=32, seed=5,
+                        rate=args.rate, steps=(10, 31), samples=(2048, 4096, 8192), L1d=(16, 20, 24, 28, 32),
+                        side=(6, 13))
+        wall = time.perf_counter() - t0
+        print(json.dumps({
+            "metric": "served VMC jobs: aggregate steps/s, p50 / p99 step latency (c5)", "value": rep["steps_per_s"],
+            "unit": "steps/s", "n_gpus": 1, "steps": rep["steps"], "warmup": 0, "ms_per_step": rep["p50_step_ms"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic (random network parameters; samples drawn from the networks)",
+            "config": {"workload": rep["workload"], "jobs": rep["jobs"], "slots": rep["slots"], "rate": args.rate},
             "report": rep, "wall_s_including_setup": wall}))
         return
     w, elapsed, kt, energies = run_gpu(args.workload, args.steps, args.warmup, args.mode, not args.no_overlap)
@@ -571,6 +628,16 @@ def main():
         "minsr_cholesky_steps": w.cholesky_steps,
         "energy_last": energies[-1] if energies else None,
     }
+    if args.workload == "c2" and not args.no_mixed:
+        wm, em, ktm, enm = run_gpu("c2", args.steps, args.warmup, args.mode, True, "mixed")
+        out["mixed_form"] = {
+            "value": args.steps / em, "unit": "steps/s", "ms_per_step": 1000.0 * em / args.steps,
+            "dtype": "fp64 network kernels + bf16x3 (compensated) Gram, fp64 solve",
+            "kernel_ms_per_step": {k: v[0] / args.steps for k, v in ktm.items() if v[1] > 0}}
+        try:
+            out["mixed_form"]["accuracy"] = accuracy_c2(wm)
+        except Exception as e:
+            out["mixed_form"]["accuracy"] = {"error": repr(e)}
     if args.workload == "c3":
         out["config"]["solver_form"] = args.mode
         out["config"]["eloc_beside_solve"] = not args.no_overlap and args.mode == "dual"

@@ -452,6 +452,16 @@ int jv_tdvp_solve_holomorphic(const double* data, const double* e, const double*
                               double snr_tol, double pinv_tol, double pinv_cutoff, double* update, double* S,
                               double* F, double* ev, double* snr, jv_solve_info* info, void* stream);
 
+/* jv_tdvp_solve_holomorphic in the mixed-precision form: A = G^H G from the
+ * compensated BF16 Gram (jv_gram_pp_mixed on the FP64 data); S, F, the
+ * eigendecomposition, the SNR and the regulariser stay FP64.  No separate
+ * reference function (tdvp.py:134-213 in FP64). */
+int jv_tdvp_solve_holomorphic_mixed(const double* data, const double* e, const double* w, int64_t n, int64_t pr,
+                                    const int64_t* col_g, const int64_t* col_ig, int64_t pc, const double* x,
+                                    int32_t make_real, double diag_shift, int32_t use_snr, double glob_num_samples,
+                                    double snr_tol, double pinv_tol, double pinv_cutoff, double* update, double* S,
+                                    double* F, double* ev, double* snr, jv_solve_info* info, void* stream);
+
 /* SampledObs.subset (stats.py:310-329): rows r_k = start + k step, k < count, of
  * centred data [n][p] (real or complex) with weights w [n] and mean [p]:
  * out_w = w[r] / nrm (nrm = sum_k w[r_k]), out_mean = sum_k sqrt(out_w_k)

@@ -159,3 +159,27 @@ def test_mixed_minsr_step_tfim10x10():
     print("mixed MinSR: residual %.6e vs %.6e, update rel err %.3e" % (res, res_ref, rel_upd))
     assert abs(res - res_ref) <= 1e-4 * res_ref
     assert rel_upd <= 5e-2
+
+
+@pytest.mark.parametrize("n,p,cpx", [(8192, 1240, False), (1000, 333, False), (333, 7, True), (16384, 211, True),
+                                     (5, 300, False), (1, 1, False), (4097, 130, True)])
+def test_gram_pp_fp64(n, p, cpx):
+    """jv_gram_pp (FP64 matrix cores, 128 x 128 upper tiles, sample slices
+    summed in FP64) against numpy: 1e-13 relative in norm, exactly Hermitian."""
+    rng = np.random.default_rng(3 * n + p)
+    if cpx:
+        Dc = (rng.normal(size=(n, p)) + 1j * rng.normal(size=(n, p))) * rng.uniform(0.1, 3.0, size=p)
+        D = np.empty((n, 2 * p))
+        D[:, 0::2], D[:, 1::2] = Dc.real, Dc.imag
+        ref = Dc.conj().T @ Dc
+    else:
+        D = rng.normal(size=(n, p)) * rng.uniform(0.1, 3.0, size=p)
+        ref = D.T @ D
+    t = torch.as_tensor(D, device="cuda").contiguous()
+    S = torch.empty((p, p * (2 if cpx else 1)), dtype=torch.float64, device="cuda")
+    _lib.call("jv_gram_pp", _lib.ptr(t), n, p, int(cpx), _lib.ptr(S), _lib.stream_ptr())
+    torch.cuda.synchronize()
+    S = S.cpu().numpy()
+    S = S[:, 0::2] + 1j * S[:, 1::2] if cpx else S
+    assert np.linalg.norm(S - ref) <= 1e-13 * np.linalg.norm(ref)
+    assert np.array_equal(S, S.conj().T)

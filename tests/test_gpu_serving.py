@@ -67,3 +67,39 @@ def test_admission_queues_when_memory_is_short():
     starts = sorted(j.admitted_at for j in jobs)
     ends = sorted(j.finished_at for j in jobs)
     assert starts[1] >= ends[0] and starts[2] >= ends[1]  # strictly one at a time
+
+
+def rbm_heun_job(job_id, L, seed, steps):
+    def make():
+        psi = jv.NQS(jv.nets.CpxRBM(numHidden=L, bias=True), seed=seed)
+        psi(torch.zeros((1, 1, L), dtype=torch.int32))
+        H, _ = tfim_pair((L,), -1.0, -0.5)
+        ms = jv.sampler.MCSampler(psi, (L,), seed, updateProposer=jv.sampler.propose_spin_flip, numChains=64,
+                                  numSamples=1024)
+        eq = jv.util.TDVP(ms, rhsPrefactor=1.j, makeReal='imag')
+        return psi, H, eq, jv.util.stepper.Heun(timeStep=1e-3)
+    return serving.Job(job_id, make, steps, serving.footprint_bytes(1024, 4 * L * L, L, True, "tdvp", 64))
+
+
+def test_shared_batch_server_equals_threads():
+    """RBM jobs joined into the shared batch (one launch per phase for all of
+    them, rebuilt as jobs arrive and finish) give bit-identical energies and
+    parameters to the same jobs on their own threads; a CNN job runs beside
+    the batch on its own thread."""
+    def jobs():
+        return [rbm_job(0, 8, 3, 4), rbm_heun_job(1, 10, 4, 3), cnn_job(2, 6, 5, 3), rbm_job(3, 12, 6, 2)]
+    srv = serving.Server(max_slots=4)
+    ref = jobs()
+    for j in ref:
+        srv.submit(j)
+    rep_t = srv.run()
+    srv = serving.Server(max_slots=4, batch_rbm=True, workers=2)
+    got = jobs()
+    for j in got:
+        srv.submit(j)
+    rep_b = srv.run()
+    assert rep_t["jobs_done"] == rep_b["jobs_done"] == 4, ([j.error for j in ref], [j.error for j in got])
+    for a, b in zip(ref, got):
+        assert a.energies == b.energies, a.job_id
+        assert torch.equal(a.params, b.params), a.job_id
+    assert rep_b["p99_step_ms"] >= rep_b["p50_step_ms"] > 0

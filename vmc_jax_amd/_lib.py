@@ -123,6 +123,9 @@ _SIGS = {
                       _dbl, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(JvSolveInfo), _vp],
     "jv_tdvp_solve_holomorphic": [_vp, _vp, _vp, _i64, _i64, _vp, _vp, _i64, ctypes.POINTER(_dbl), _i32, _dbl, _i32,
                                   _dbl, _dbl, _dbl, _dbl, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(JvSolveInfo), _vp],
+    "jv_tdvp_solve_holomorphic_mixed": [_vp, _vp, _vp, _i64, _i64, _vp, _vp, _i64, ctypes.POINTER(_dbl), _i32, _dbl,
+                                        _i32, _dbl, _dbl, _dbl, _dbl, _vp, _vp, _vp, _vp, _vp,
+                                        ctypes.POINTER(JvSolveInfo), _vp],
     "jv_subset": [_vp, _vp, _vp, _i64, _i64, _i32, _i64, _i64, _i64, _vp, _vp, _vp, _vp],
     "jv_batch_create": [ctypes.POINTER(JvBatchItem), _i32, ctypes.POINTER(_vp)],
     "jv_batch_destroy": [_vp],

@@ -78,6 +78,7 @@ class Instance:
         self.pref = H.prefactors(0.0)
         eq.sampler = _SamplerInfo(self.n)
         self.energy = None
+        self.calls = 0  # sampling passes done (the device advances its own copy in step)
 
     def item(self):
         it = _lib.JvBatchItem()
@@ -87,7 +88,7 @@ class Instance:
         it.prefactor = self.pref.data_ptr()
         a = it.mc
         a.num_chains, a.therm_steps, a.samples_per_chain, a.sweep_steps = self.C, self.therm, self.spc, self.sweep
-        a.mu, a.seed, a.call_index, a.stream = 2.0, self.key & 0xFFFFFFFFFFFFFFFF, 0, 0
+        a.mu, a.seed, a.call_index, a.stream = 2.0, self.key & 0xFFFFFFFFFFFFFFFF, self.calls & 0xFFFFFFFF, 0
         it.walkers, it.configs = self.walkers.data_ptr(), self.configs.data_ptr()
         it.logpsi, it.counters = self.logpsi.data_ptr(), self.counters.data_ptr()
         it.eloc, it.grad, it.grad_layout = self.eloc.data_ptr(), self.grad.data_ptr(), self.layout
@@ -97,9 +98,9 @@ class Instance:
         d = self.psi.desc
         return [self.L, int(d.num_hidden), int(d.is_complex), int(self.H.table.num_ops), self.n, self.C, self.layout]
 
-    def solve_and_update(self):
-        """The instance's right-hand side from the batch buffers (tdvp.py:134-213
-        or minsr.py:53-80) and the Euler update theta += dt * theta_dot, in place."""
+    def rhs(self):
+        """The instance's right-hand side theta_dot from the batch buffers
+        (tdvp.py:134-213 or minsr.py:53-80) as a flat real vector."""
         Eloc = SampledObs(self.eloc.reshape(1, -1), self.p)
         grads = SampledObs(self.grad.reshape((1,) + tuple(self.grad.shape)), self.p)
         if self.layout == 1:
@@ -109,7 +110,12 @@ class Instance:
         else:
             upd = self.eq.solve(Eloc, grads, self.psi.holomorphic, self.psi)
             self.energy = complex(Eloc.mean()[0].item())
-        self.psi.params.add_(self.dt * self.psi._param_unflatten(upd))
+        return self.psi._param_unflatten(upd)
+
+    def solve_and_update(self):
+        """rhs() and the Euler update theta += dt * theta_dot, in place (the
+        batch reads the parameters from this buffer)."""
+        self.psi.params.add_(self.dt * self.rhs())
 
 
 def tune_cache_path():
@@ -130,7 +136,11 @@ class BatchedSweep:
               "grad": [(w, s) for w in (1, 2, 4, 8) for s in (1, 2, 4, 8)],
               "eloc": [(w, s) for w in (1, 2, 4, 8) for s in (1, 2, 4, 8)]}
 
-    def __init__(self, instances, workers=8, tune=True, graph=True):
+    def __init__(self, instances, workers=8, tune=True, graph=True, pool=None):
+        """tune: True runs the trials unless the mix is in the cache; "cache" uses
+        cached shapes only (default shapes otherwise); False keeps the defaults.
+        pool: a ThreadPoolExecutor for the solves (shared by a server that
+        rebuilds the batch when its jobs change)."""
         _lib.check(_lib.lib.jv_init(torch.cuda.current_device()), "jv_init")
         self.inst = list(instances)
         self.workers = int(workers)
@@ -141,7 +151,7 @@ class BatchedSweep:
         self.shapes = {"sample": (4, 1), "grad": (4, 1), "eloc": (4, 1)}
         self.tuned_from = None
         if tune:
-            self.autotune()
+            self.autotune(trials=tune is True)
         self.graph = None
         if graph:
             g = ctypes.c_void_p()
@@ -149,9 +159,11 @@ class BatchedSweep:
             _lib.check(_lib.lib.jv_graph_capture(self.batch, ctypes.c_void_p(self.stream.cuda_stream), ctypes.byref(g)),
                        "jv_graph_capture")
             self.graph = g
-        self._pool = ThreadPoolExecutor(max_workers=self.workers)
+        self._own_pool = pool is None
+        self._pool = pool if pool is not None else ThreadPoolExecutor(max_workers=self.workers)
         self._tls = threading.local()
         self.step_ms = []
+        self._phase_events = []
 
     def key(self):
         sig = json.dumps(sorted(i.signature() for i in self.inst))
@@ -160,7 +172,7 @@ class BatchedSweep:
     def _set_shape(self, batch, phase, shape):
         _lib.check(_lib.lib.jv_batch_set_shape(batch, PHASES[phase], shape[0], shape[1]), "jv_batch_set_shape")
 
-    def autotune(self, reps=3):
+    def autotune(self, reps=3, trials=True):
         """Launch shape per phase: the fastest of the candidate shapes over `reps`
         timed launches on this instance mix, on a scratch batch that shares the
         output buffers but has its own walkers, counters and call indices (the
@@ -171,6 +183,8 @@ class BatchedSweep:
         if k in cache:
             self.shapes = {ph: tuple(v) for ph, v in cache[k].items()}
             self.tuned_from = "cache"
+        elif not trials:
+            return
         else:
             scratch = []  # (walkers, counters, item): kept alive while the scratch batch exists
             for it, ins in zip(self._items, self.inst):
@@ -222,44 +236,70 @@ class BatchedSweep:
             s = self._tls.stream = torch.cuda.Stream()
         return s
 
-    def _solve(self, idx, ready):
+    def _solve(self, idx, ready, fn):
         s = self._worker_stream()
+        errors = {}
         with torch.cuda.stream(s):
             s.wait_event(ready)
             for i in idx:
-                self.inst[i].solve_and_update()
+                try:
+                    fn(self.inst[i])
+                except Exception as exc:  # reported per instance, the others continue
+                    errors[i] = exc
             done = torch.cuda.Event()
             done.record(s)
-        return done
+        return done, errors
 
     def sample_phase(self):
         """Sampling, log psi + O_k and O_loc of every instance on the batch stream."""
         st = ctypes.c_void_p(self.stream.cuda_stream)
         self.stream.wait_stream(torch.cuda.current_stream())
+        t0 = torch.cuda.Event(enable_timing=True)
+        t0.record(self.stream)
         if self.graph is not None:
             _lib.check(_lib.lib.jv_graph_launch(self.graph, st), "jv_graph_launch")
         else:
             _lib.check(_lib.lib.jv_batch_step(self.batch, st), "jv_batch_step")
-        ready = torch.cuda.Event()
+        for i in self.inst:
+            i.calls += 1
+        ready = torch.cuda.Event(enable_timing=True)
         ready.record(self.stream)
+        self._phase_events.append((t0, ready))
         return ready
+
+    def phase_ms(self):
+        """Device time of each batched sampling / O_k / O_loc pass so far."""
+        return [a.elapsed_time(b) for a, b in self._phase_events]
+
+    def evaluate(self, fn):
+        """One right-hand-side evaluation of every instance: the batched
+        sampling / O_k / O_loc pass, then fn(instance) for each instance on the
+        worker threads (largest solves first, round-robin), each on its
+        worker's stream.  Returns {instance index: exception} of failed ones."""
+        ready = self.sample_phase()
+        order = sorted(range(len(self.inst)), key=lambda i: -self.inst[i].grad.shape[1])
+        parts = [order[k::self.workers] for k in range(self.workers)]
+        futs = [self._pool.submit(self._solve, p, ready, fn) for p in parts if p]
+        cur = torch.cuda.current_stream()
+        errors = {}
+        for f in futs:
+            done, err = f.result()
+            cur.wait_event(done)
+            errors.update(err)
+        cur.synchronize()
+        return errors
 
     def step(self):
         """One Euler step of every instance."""
         t0 = time.perf_counter()
-        ready = self.sample_phase()
-        # largest solves first, round-robin over the workers
-        order = sorted(range(len(self.inst)), key=lambda i: -self.inst[i].grad.shape[1])
-        parts = [order[k::self.workers] for k in range(self.workers)]
-        futs = [self._pool.submit(self._solve, p, ready) for p in parts if p]
-        cur = torch.cuda.current_stream()
-        for f in futs:
-            cur.wait_event(f.result())
-        cur.synchronize()
+        errors = self.evaluate(lambda i: i.solve_and_update())
+        if errors:
+            raise next(iter(errors.values()))
         self.step_ms.append(1000.0 * (time.perf_counter() - t0))
 
     def close(self):
-        self._pool.shutdown(wait=True)
+        if self._own_pool:
+            self._pool.shutdown(wait=True)
         if self.graph is not None:
             _lib.lib.jv_graph_destroy(self.graph)
             self.graph = None

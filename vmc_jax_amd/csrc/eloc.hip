@@ -235,20 +235,22 @@ __host__ __device__ inline size_t elo_win_bytes(const CnnGeom& g, int cm, int no
 // Flips per chunk: the fewest estimated VALU cycles per flip among the sizes
 // whose LDS fits two workgroups per CU (at least one flip).
 // Workgroups per CU the chunk size aims at (LDS per workgroup <= 160 KiB /
-// this); VMC_JAX_AMD_ELOC_WGS_PER_CU overrides the default of two.
-inline int elo_wgs_per_cu() {
-  static const int v = [] {
+// this): two for the FP64 kernel, four for the FP32 one (measured on the
+// 10x10 CNN: FP64 4.39 ms at two, 4.63 at three, 8.78 at four; FP32 2.97 ms
+// at two, 2.44 at four); VMC_JAX_AMD_ELOC_WGS_PER_CU overrides both.
+inline int elo_wgs_per_cu(int tsize) {
+  static const int env = [] {
     const char* s = getenv("VMC_JAX_AMD_ELOC_WGS_PER_CU");
-    const int k = s ? atoi(s) : 2;
-    return k < 1 ? 1 : (k > 8 ? 8 : k);
+    const int k = s ? atoi(s) : 0;
+    return k < 0 ? 0 : (k > 8 ? 8 : k);
   }();
-  return v;
+  return env > 0 ? env : (tsize == 4 ? 4 : 2);
 }
 
 inline int elo_win_chunk(const CnnGeom& g, int cm, int nops, int threads, int tsize = 8) {
   int best = 1;
   double best_cost = 1e300;
-  const size_t cap = (size_t)160 * 1024 / elo_wgs_per_cu();
+  const size_t cap = (size_t)160 * 1024 / elo_wgs_per_cu(tsize);
   for (int f = 1; f <= 64; ++f) {
     if (f > 1 && elo_win_bytes(g, cm, nops, f, tsize) > cap) break;
     double cost = 0.0;

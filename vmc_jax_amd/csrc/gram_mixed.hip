@@ -51,119 +51,187 @@ __device__ inline void split_bf16(double x, __bf16& hi, __bf16& lo) {
   lo = (__bf16)(float)(x - (double)(float)hi);
 }
 
-// Four consecutive columns X[k][col .. col + 3] (col % 4 == 0): one 16-byte
-// (FP32) or two 16-byte (FP64) loads when VEC (q % 4 == 0, so a group of four
-// lies entirely inside or outside [0, q)), else four element loads.
-template <typename T, bool VEC>
-__device__ inline void load4(const T* __restrict__ X, int64_t q, int64_t k, int64_t col, T v[4]) {
-  if (VEC) {
-    if constexpr (sizeof(T) == 4) {
-      const float4 x = *(const float4*)(X + k * q + col);
-      v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w;
-    } else {
-      const double2 x = *(const double2*)(X + k * q + col);
-      const double2 y = *(const double2*)(X + k * q + col + 2);
-      v[0] = x.x; v[1] = x.y; v[2] = y.x; v[3] = y.y;
-    }
-  } else {
-#pragma unroll
-    for (int c = 0; c < 4; ++c) v[c] = X[k * q + (col + c < q ? col + c : 0)];
-  }
-}
-
-// Loads the 8 x 4 block X[k0 + 8 kg + r][c0 + 4 mg + c] (r < 8, c < 4) into v,
-// zero outside [0, kend) x [0, q).  Addresses are clamped so every load is
-// issued unconditionally.
-template <typename T, bool VEC>
-__device__ inline void load_block(const T* __restrict__ X, int64_t q, int64_t k0, int64_t kend, int64_t c0, int mg,
-                                  int kg, T v[8][4]) {
-  const int64_t col = c0 + 4 * mg;
-  const int64_t colc = col < q ? col : 0;
-#pragma unroll
-  for (int r = 0; r < 8; ++r) {
-    const int64_t k = k0 + 8 * kg + r;
-    const bool kin = k < kend;
-    T x[4];
-    load4<T, VEC>(X, q, kin ? k : 0, colc, x);
-#pragma unroll
-    for (int c = 0; c < 4; ++c) v[r][c] = (kin && col + c < q) ? x[c] : (T)0;
-  }
-}
-
-// Stores the split block into the [col][k] planes (hi, lo) of one operand.
-template <typename T>
-__device__ inline void store_block(__bf16* hi, __bf16* lo, int mg, int kg, const T v[8][4]) {
-#pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    bf16x8 h, l;
-#pragma unroll
-    for (int r = 0; r < 8; ++r) {
-      __bf16 a, b;
-      split_bf16(v[r][c], a, b);
-      h[r] = a;
-      l[r] = b;
-    }
-    const int off = (4 * mg + c) * GROW + 8 * kg;
-    *(bf16x8*)(hi + off) = h;
-    *(bf16x8*)(lo + off) = l;
-  }
-}
 }  // namespace
 
-// One upper tile (tm <= tn) of R = X^T X over the sample slice blockIdx.y,
-// written as FP32 to part[(blockIdx.y * ntiles + tile) * GT * GT + r * GT + c].
-// 4 waves, each owning a 64 x 64 quarter as 4 x 4 MFMA blocks of 16 x 16.
-template <typename T, bool VEC>
-__global__ void __launch_bounds__(256) gram3_tn(const T* __restrict__ X, int64_t k_total, int64_t q, int64_t kchunk,
-                                                int ntile_side, float* __restrict__ part) {
-  extern __shared__ __align__(16) unsigned char gram_lds[];
-  __bf16* Ahi = (__bf16*)gram_lds;
-  __bf16* Alo = Ahi + GPLANE;
-  __bf16* Bhi = Alo + GPLANE;
-  __bf16* Blo = Bhi + GPLANE;
+// ------------------------------------------------------------------ FP64 Gram
+// R = X^T X in FP64 on the FP64 matrix cores (v_mfma_f64_16x16x4f64): the
+// same 128 x 128 upper tiles and sample slices as the compensated Gram, 32
+// samples per LDS stage kept in X's own [sample][column] order (row pitch 130
+// doubles), two 16-byte loads per thread and row group issued one stage ahead
+// from clamped addresses.  A lane's fragment is A[row l % 16][k l / 16] and
+// B[k l / 16][column l % 16]; result element v of lane l is
+// C[4 v + l / 16][l % 16] (the FP64 instruction's row order, unlike the BF16
+// one's C[4 (l / 16) + v][l % 16]).
+constexpr int G6K = 32;
+constexpr int G6ROW = GT + 2;
+
+template <bool VEC>
+__global__ void __launch_bounds__(256) gram64_tn(const double* __restrict__ X, int64_t k_total, int64_t q,
+                                                 int64_t kchunk, int ntile_side, double* __restrict__ part) {
+  __shared__ double As[G6K][G6ROW];
+  __shared__ double Bs[G6K][G6ROW];
   int tm, tn;
   upper_tile(blockIdx.x, ntile_side, tm, tn);
   const int ntiles = ntile_side * (ntile_side + 1) / 2;
   const int tid = threadIdx.x, lane = tid % 64, wv = tid / 64;
   const int wm = (wv / 2) * 64, wn = (wv % 2) * 64;
-  const int mg = tid % 32, kg = tid / 32;
+  const int cp = tid % 64, rg = tid / 64;  // column pair, row group: rows rg + 4 j
   const int64_t m0 = (int64_t)tm * GT, n0 = (int64_t)tn * GT;
   const int64_t kbeg = (int64_t)blockIdx.y * kchunk;
   const int64_t kend = std::min<int64_t>(k_total, kbeg + kchunk);
+  typedef double f64x4 __attribute__((ext_vector_type(4)));
+  f64x4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
+  double2 va[8], vb[8];
+  const int64_t ca = m0 + 2 * cp, cb = n0 + 2 * cp;
+  auto load = [&](int64_t k0) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int64_t k = k0 + rg + 4 * j;
+      const bool kin = k < kend;
+      const int64_t kc = kin ? k : 0;
+      double2 a, b;
+      if (VEC) {  // q even: a column pair lies inside or outside [0, q) as a whole, 16-byte aligned
+        a = *(const double2*)(X + kc * q + (ca < q ? ca : 0));
+        b = *(const double2*)(X + kc * q + (cb < q ? cb : 0));
+      } else {
+        a.x = X[kc * q + (ca < q ? ca : 0)];
+        a.y = X[kc * q + (ca + 1 < q ? ca + 1 : 0)];
+        b.x = X[kc * q + (cb < q ? cb : 0)];
+        b.y = X[kc * q + (cb + 1 < q ? cb + 1 : 0)];
+      }
+      va[j] = double2{(kin && ca < q) ? a.x : 0.0, (kin && ca + 1 < q) ? a.y : 0.0};
+      vb[j] = double2{(kin && cb < q) ? b.x : 0.0, (kin && cb + 1 < q) ? b.y : 0.0};
+    }
+  };
+  if (kbeg < kend) load(kbeg);
+  const int fr = lane % 16, fk = lane / 16;
+  for (int64_t k0 = kbeg; k0 < kend; k0 += G6K) {
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      As[rg + 4 * j][2 * cp] = va[j].x;
+      As[rg + 4 * j][2 * cp + 1] = va[j].y;
+      Bs[rg + 4 * j][2 * cp] = vb[j].x;
+      Bs[rg + 4 * j][2 * cp + 1] = vb[j].y;
+    }
+    __syncthreads();
+    if (k0 + G6K < kend) load(k0 + G6K);
+#pragma unroll
+    for (int ks = 0; ks < G6K; ks += 4) {
+      double a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i] = As[ks + fk][wm + 16 * i + fr];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b[j] = Bs[ks + fk][wn + 16 * j + fr];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+  }
+  double* P = part + ((size_t)blockIdx.y * ntiles + blockIdx.x) * GT * GT;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = wm + 16 * i + 4 * v + lane / 16, c = wn + 16 * j + lane % 16;
+        P[r * GT + c] = acc[i][j][v];
+      }
+}
+
+// Split pass: X [k][q] (FP32 or FP64) -> Y, bf16, blocked for the Gram's
+// stage loads: Y[((kb * qp + c) * 2 + h) * GK + j] = (h ? lo : hi) of
+// X[kb * GK + j][c], zero for samples >= k_total or columns >= q (qp = the
+// column count rounded up to whole tiles).  One thread per 8 samples of one
+// column; consecutive threads take consecutive columns (coalesced reads).
+template <typename T>
+__global__ void __launch_bounds__(256) gram3_split(const T* __restrict__ X, int64_t k_total, int64_t q, int64_t qp,
+                                                   int64_t nkb, __bf16* __restrict__ Y) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nkb * qp * (GK / 8)) return;
+  const int64_t c = t % qp, rest = t / qp;
+  const int g = (int)(rest % (GK / 8));
+  const int64_t kb = rest / (GK / 8);
+  bf16x8 h, l;
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const int64_t k = kb * GK + 8 * g + r;
+    const bool in = k < k_total && c < q;
+    const T x = X[(in ? k : 0) * q + (in ? c : 0)];
+    __bf16 a, b;
+    split_bf16(in ? x : (T)0, a, b);
+    h[r] = a;
+    l[r] = b;
+  }
+  __bf16* dst = Y + ((kb * qp + c) * 2) * GK + 8 * g;
+  *(bf16x8*)dst = h;
+  *(bf16x8*)(dst + GK) = l;
+}
+
+// gram3_tn on pre-split data: a stage's 128 columns x 2 planes x 64 samples
+// of one operand are 32 KiB contiguous in Y, read as 8 coalesced 16-byte
+// loads per thread and stored to the LDS planes without conversion.
+__global__ void __launch_bounds__(256) gram3_pre(const __bf16* __restrict__ Y, int64_t nkb, int64_t qp, int64_t kbchunk,
+                                                 int ntile_side, float* __restrict__ part) {
+  extern __shared__ __align__(16) unsigned char gram_lds[];
+  __bf16* Ap = (__bf16*)gram_lds;          // [2][GT][GROW]: hi then lo
+  __bf16* Bp = Ap + 2 * GPLANE;
+  int tm, tn;
+  upper_tile(blockIdx.x, ntile_side, tm, tn);
+  const int ntiles = ntile_side * (ntile_side + 1) / 2;
+  const int tid = threadIdx.x, lane = tid % 64, wv = tid / 64;
+  const int wm = (wv / 2) * 64, wn = (wv % 2) * 64;
+  const int64_t kb0 = (int64_t)blockIdx.y * kbchunk;
+  const int64_t kb1 = std::min<int64_t>(nkb, kb0 + kbchunk);
   f32x4 acc[4][4];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-  T va[8][4], vb[8][4];
-  if (kbeg < kend) {
-    load_block<T, VEC>(X, q, kbeg, kend, m0, mg, kg, va);
-    load_block<T, VEC>(X, q, kbeg, kend, n0, mg, kg, vb);
-  }
-  const int fr = lane % 16, fk = 8 * (lane / 16);
-  for (int64_t k0 = kbeg; k0 < kend; k0 += GK) {
-    __syncthreads();
-    store_block(Ahi, Alo, mg, kg, va);
-    store_block(Bhi, Blo, mg, kg, vb);
-    __syncthreads();
-    if (k0 + GK < kend) {
-      load_block<T, VEC>(X, q, k0 + GK, kend, m0, mg, kg, va);
-      load_block<T, VEC>(X, q, k0 + GK, kend, n0, mg, kg, vb);
+  // chunk u (0 .. 2047) of a stage: column u / 16, plane (u / 8) % 2, samples 8 (u % 8) .. + 7
+  bf16x8 va[8], vb[8];
+  auto load = [&](int64_t kb) {
+    const bf16x8* A = (const bf16x8*)(Y + (kb * qp + (int64_t)tm * GT) * 2 * GK);
+    const bf16x8* B = (const bf16x8*)(Y + (kb * qp + (int64_t)tn * GT) * 2 * GK);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      va[j] = A[tid + 256 * j];
+      vb[j] = B[tid + 256 * j];
     }
+  };
+  if (kb0 < kb1) load(kb0);
+  const int fr = lane % 16, fk = 8 * (lane / 16);
+  for (int64_t kb = kb0; kb < kb1; ++kb) {
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int u = tid + 256 * j;
+      const int off = ((u / 8) % 2) * GPLANE + (u / 16) * GROW + 8 * (u % 8);
+      *(bf16x8*)(Ap + off) = va[j];
+      *(bf16x8*)(Bp + off) = vb[j];
+    }
+    __syncthreads();
+    if (kb + 1 < kb1) load(kb + 1);
 #pragma unroll
     for (int s = 0; s < GK; s += 32) {
       bf16x8 ah[4], al[4], bh[4], bl[4];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int off = (wm + 16 * i + fr) * GROW + s + fk;
-        ah[i] = *(const bf16x8*)(Ahi + off);
-        al[i] = *(const bf16x8*)(Alo + off);
+        ah[i] = *(const bf16x8*)(Ap + off);
+        al[i] = *(const bf16x8*)(Ap + GPLANE + off);
       }
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int off = (wn + 16 * j + fr) * GROW + s + fk;
-        bh[j] = *(const bf16x8*)(Bhi + off);
-        bl[j] = *(const bf16x8*)(Blo + off);
+        bh[j] = *(const bf16x8*)(Bp + off);
+        bl[j] = *(const bf16x8*)(Bp + GPLANE + off);
       }
 #pragma unroll
       for (int i = 0; i < 4; ++i)
@@ -175,7 +243,6 @@ __global__ void __launch_bounds__(256) gram3_tn(const T* __restrict__ X, int64_t
         }
     }
   }
-  // result element v of lane l in block (i, j): row 4 (l / 16) + v, column l % 16
   float* P = part + ((size_t)blockIdx.y * ntiles + blockIdx.x) * GT * GT;
 #pragma unroll
   for (int i = 0; i < 4; ++i)
@@ -195,7 +262,8 @@ __global__ void __launch_bounds__(256) gram3_tn(const T* __restrict__ X, int64_t
 // mirror (transpose, conjugate for complex data) are written along rows.  Only
 // entries on or above the diagonal of R are used, and the lower triangle of S
 // is their exact mirror.
-__global__ void __launch_bounds__(256) gram3_reduce(const float* __restrict__ part, int splits, int ntile_side,
+template <typename TP>
+__global__ void __launch_bounds__(256) gram3_reduce(const TP* __restrict__ part, int splits, int ntile_side,
                                                     int64_t p, int cpx, double* __restrict__ S) {
   __shared__ double M[32][33];
   int tm, tn;
@@ -262,33 +330,63 @@ int gram_pp_bf16x3(const void* D, int f32, int64_t n, int64_t p, int cpx, double
   if (side > 4096) return fail(JV_E_ARG, "gram_pp_bf16x3: matrix too large");
   const int nts = (int)side;
   const int64_t ntiles = side * (side + 1) / 2;
-  // slices of the sample axis: enough workgroups for two per CU, slices of at
-  // least 512 samples
+  // slices of the sample axis (whole 64-sample stages): enough workgroups for
+  // two per CU, slices of at least 512 samples
+  const int64_t nkb = (n + GK - 1) / GK, qp = side * GT;
   int64_t splits = 1;
-  while (ntiles * splits < 512 && (n + splits * 2 - 1) / (splits * 2) >= 512) splits *= 2;
-  int64_t kchunk = ((n + splits - 1) / splits + GK - 1) / GK * GK;
-  splits = (n + kchunk - 1) / kchunk;
-  if (splits < 1) splits = 1;
+  while (ntiles * splits < 512 && nkb / (splits * 2) >= 8) splits *= 2;
+  const int64_t kbchunk = (nkb + splits - 1) / splits;
+  splits = (nkb + kbchunk - 1) / kbchunk;
   float* part = (float*)workspace(sizeof(float) * (size_t)splits * ntiles * GT * GT, slot);
-  if (!part) return fail(JV_E_NOMEM, "gram_pp_bf16x3: workspace allocation failed");
+  __bf16* Y = (__bf16*)workspace(sizeof(__bf16) * (size_t)nkb * qp * 2 * GK, 120);
+  if (!part || !Y) return fail(JV_E_NOMEM, "gram_pp_bf16x3: workspace allocation failed");
   const size_t lds = 4 * (size_t)GPLANE * sizeof(__bf16);
-  const dim3 grid((unsigned)ntiles, (unsigned)splits);
   ScopedTimer tm("gram", st);
-  const bool vec = q % 4 == 0 && ((uintptr_t)D % 16) == 0;
-#define JV_G3(Tv, Vv)                                                                              \
-  {                                                                                                \
-    int rc = set_lds_limit((const void*)gram3_tn<Tv, Vv>, lds);                                    \
-    if (rc) return rc;                                                                             \
-    hipLaunchKernelGGL((gram3_tn<Tv, Vv>), grid, dim3(256), lds, st, (const Tv*)D, n, q, kchunk, nts, part); \
-  }
-  if (f32) {
-    if (vec) JV_G3(float, true) else JV_G3(float, false)
-  } else {
-    if (vec) JV_G3(double, true) else JV_G3(double, false)
-  }
-#undef JV_G3
-  JV_LAUNCH_CHECK("gram3_tn");
-  hipLaunchKernelGGL(gram3_reduce, dim3((unsigned)(ntiles * 16)), dim3(256), 0, st, part, (int)splits, nts, p, cpx, S);
+  const int64_t nthr = nkb * qp * (GK / 8);
+  if (f32)
+    hipLaunchKernelGGL(gram3_split<float>, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, st, (const float*)D, n,
+                       q, qp, nkb, Y);
+  else
+    hipLaunchKernelGGL(gram3_split<double>, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, st, (const double*)D,
+                       n, q, qp, nkb, Y);
+  JV_LAUNCH_CHECK("gram3_split");
+  int rc = set_lds_limit((const void*)gram3_pre, lds);
+  if (rc) return rc;
+  hipLaunchKernelGGL(gram3_pre, dim3((unsigned)ntiles, (unsigned)splits), dim3(256), lds, st, Y, nkb, qp, kbchunk,
+                     nts, part);
+  JV_LAUNCH_CHECK("gram3_pre");
+  hipLaunchKernelGGL(gram3_reduce<float>, dim3((unsigned)(ntiles * 16)), dim3(256), 0, st, part, (int)splits, nts, p,
+                     cpx, S);
+  JV_LAUNCH_CHECK("gram3_reduce");
+  return JV_OK;
+}
+
+// S = D^dagger D in FP64 from FP64 data D [n][p] (complex: interleaved), the
+// symmetric / Hermitian Gram of jv_gram_pp.  Partial tiles in workspace slot
+// `slot`.
+int gram_pp_f64(const double* D, int64_t n, int64_t p, int cpx, double* S, hipStream_t st, int slot) {
+  const int64_t q = p * (cpx ? 2 : 1);
+  const int64_t side = (q + GT - 1) / GT;
+  if (side > 4096) return fail(JV_E_ARG, "gram_pp_f64: matrix too large");
+  const int nts = (int)side;
+  const int64_t ntiles = side * (side + 1) / 2;
+  const int64_t nkb = (n + G6K - 1) / G6K;
+  int64_t splits = 1;
+  while (ntiles * splits < 512 && nkb / (splits * 2) >= 16) splits *= 2;
+  const int64_t kchunk = (nkb + splits - 1) / splits * G6K;
+  splits = (n + kchunk - 1) / kchunk;
+  double* part = (double*)workspace(sizeof(double) * (size_t)splits * ntiles * GT * GT, slot);
+  if (!part) return fail(JV_E_NOMEM, "gram_pp_f64: workspace allocation failed");
+  ScopedTimer tm("gram", st);
+  if (q % 2 == 0 && ((uintptr_t)D % 16) == 0)
+    hipLaunchKernelGGL(gram64_tn<true>, dim3((unsigned)ntiles, (unsigned)splits), dim3(256), 0, st, D, n, q, kchunk,
+                       nts, part);
+  else
+    hipLaunchKernelGGL(gram64_tn<false>, dim3((unsigned)ntiles, (unsigned)splits), dim3(256), 0, st, D, n, q, kchunk,
+                       nts, part);
+  JV_LAUNCH_CHECK("gram64_tn");
+  hipLaunchKernelGGL(gram3_reduce<double>, dim3((unsigned)(ntiles * 16)), dim3(256), 0, st, part, (int)splits, nts,
+                     p, cpx, S);
   JV_LAUNCH_CHECK("gram3_reduce");
   return JV_OK;
 }

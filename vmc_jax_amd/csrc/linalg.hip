@@ -28,6 +28,7 @@ int trd_wy_apply(const double* V, int n, double* x, const double* Tbuf, bool tra
 bool aux_stream(hipStream_t* st, hipEvent_t* e0, hipEvent_t* e1);
 
 rocblas_handle blas_handle(hipStream_t stream);
+int gram_pp_f64(const double* D, int64_t n, int64_t p, int cpx, double* S, hipStream_t st, int slot);
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
@@ -832,9 +833,9 @@ int jv_gram_nn(const double* data, int64_t n, int64_t p, int32_t is_complex, dou
 
 int jv_gram_pp(const double* data, int64_t n, int64_t p, int32_t is_complex, double* S, void* stream) {
   if (!data || !S || n < 1 || p < 1) return fail(JV_E_ARG, "jv_gram_pp: bad arguments");
-  // S = D^dagger D: opA(m, k) = conj(D[k][m]), opB(k, n) = D[k][n]
-  GemmArgs g{p, p, n, data, 1, p, 1, data, p, 1, 0, S, p, is_complex, 1, 1.0};
-  return gemm(g, (hipStream_t)stream);
+  // S = D^dagger D: the real Gram of D's interleaved storage (complex data:
+  // combined 2 x 2 blocks), 128 x 128 upper tiles on the FP64 matrix cores
+  return gram_pp_f64(data, n, p, is_complex, S, (hipStream_t)stream, 18);
 }
 
 int jv_eigh(int64_t n, int32_t is_complex, double* A, double* w, void* stream) {
@@ -1303,6 +1304,7 @@ int jv_gemv(int32_t trans, const double* M, const double* x, int64_t rows, int64
 }  // extern "C"
 
 namespace jv {
+int gram_pp_bf16x3(const void* D, int f32, int64_t n, int64_t p, int cpx, double* S, hipStream_t st, int slot);
 // The TDVP solve; with col_g / col_ig (device, pc entries each) the data has
 // the holomorphic [g, i g] structure and S is built and diagonalised through
 // the Pc x Pc Gram A = G^H G.
@@ -1310,7 +1312,8 @@ static int tdvp_solve_impl(const double* D, const double* e, const double* w, in
                            const double* x, int32_t make_real, double diag_shift, int32_t use_snr,
                            double glob_num_samples, double snr_tol, double pinv_tol, double pinv_cutoff,
                            double* update, double* S, double* F, double* ev, double* snr, jv_solve_info* info,
-                           const int64_t* col_g, const int64_t* col_ig, int64_t pc, void* stream) {
+                           const int64_t* col_g, const int64_t* col_ig, int64_t pc, void* stream,
+                           bool mixed_gram = false) {
   if (!D || !e || !w || !x || !update || !S || !F || !ev || !snr || n < 1 || pr < 1)
     return fail(JV_E_ARG, "jv_tdvp_solve: bad arguments");
   const bool holo = col_g != nullptr;
@@ -1347,7 +1350,8 @@ static int tdvp_solve_impl(const double* D, const double* e, const double* w, in
     if (!G || !A) return fail(JV_E_NOMEM, "jv_tdvp_solve: workspace allocation failed");
     hipLaunchKernelGGL(holo_gather, dim3((unsigned)((n * pc + 255) / 256)), dim3(256), 0, st, D, n, pr, col_g, pc, G);
     JV_LAUNCH_CHECK("holo_gather");
-    if ((rc = jv_gram_pp(G, n, pc, 1, A, stream))) return rc;
+    // the mixed form takes A from the compensated BF16 Gram (FP64 data in)
+    if ((rc = mixed_gram ? gram_pp_bf16x3(G, 0, n, pc, 1, A, st, 19) : jv_gram_pp(G, n, pc, 1, A, stream))) return rc;
     hipLaunchKernelGGL(holo_expand_S0, dim3((unsigned)((pc * pc + 255) / 256)), dim3(256), 0, st, A, pc, col_g,
                        col_ig, pr, S0);
     JV_LAUNCH_CHECK("holo_expand_S0");
@@ -1486,6 +1490,16 @@ int jv_tdvp_solve_holomorphic(const double* D, const double* e, const double* w,
   if (!col_g || !col_ig) return fail(JV_E_ARG, "jv_tdvp_solve_holomorphic: null column maps");
   return tdvp_solve_impl(D, e, w, n, pr, x, make_real, diag_shift, use_snr, glob_num_samples, snr_tol, pinv_tol,
                          pinv_cutoff, update, S, F, ev, snr, info, col_g, col_ig, pc, stream);
+}
+
+int jv_tdvp_solve_holomorphic_mixed(const double* D, const double* e, const double* w, int64_t n, int64_t pr,
+                                    const int64_t* col_g, const int64_t* col_ig, int64_t pc, const double* x,
+                                    int32_t make_real, double diag_shift, int32_t use_snr, double glob_num_samples,
+                                    double snr_tol, double pinv_tol, double pinv_cutoff, double* update, double* S,
+                                    double* F, double* ev, double* snr, jv_solve_info* info, void* stream) {
+  if (!col_g || !col_ig) return fail(JV_E_ARG, "jv_tdvp_solve_holomorphic_mixed: null column maps");
+  return tdvp_solve_impl(D, e, w, n, pr, x, make_real, diag_shift, use_snr, glob_num_samples, snr_tol, pinv_tol,
+                         pinv_cutoff, update, S, F, ev, snr, info, col_g, col_ig, pc, stream, true);
 }
 
 

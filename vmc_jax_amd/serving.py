@@ -16,6 +16,13 @@ equation and stepper, plus a step budget.  The server
 * isolates failures: a job whose step raises (or produces a non-finite energy)
   is marked failed and its slot and memory are released; other jobs continue.
 
+With ``batch_rbm=True`` the RBM jobs (CpxRBM / RBM with a Metropolis sampler,
+a time-independent operator and an Euler or Heun stepper) do not get threads:
+they join one shared batch (``vmc_jax_amd.batch``), so the sampling, O_k and
+O_loc of all of them run as one launch per phase, and the batch is rebuilt
+whenever jobs join or leave; their solves run on a pool of worker threads.
+Every other job keeps its own thread and stream beside the batch.
+
 Results do not depend on the mix: every job draws from its own Philox stream,
 all reductions have a fixed order, and the library keeps solver workspaces per
 host thread, so a job's trajectory is bit-identical alone and in the mix.
@@ -55,12 +62,13 @@ class Job:
         self.finished_at = None
         self.params = None
 
-    def run(self, stream, stop_event):
-        """Body of the job's worker thread."""
+    def run(self, stream, stop_event, built=None):
+        """Body of the job's worker thread (built: the objects of make() when the
+        server has already built them on this stream)."""
         try:
             ctx = torch.cuda.stream(stream) if stream is not None else _NullCtx()
             with ctx:
-                psi, H, eq, stepper = self.make()
+                psi, H, eq, stepper = built if built is not None else self.make()
                 while self.steps_done < self.num_steps and not stop_event.is_set():
                     t0 = time.perf_counter()
                     dp, dt = stepper.step(self.t, eq, psi.get_parameters(), hamiltonian=H, psi=psi,
@@ -85,6 +93,73 @@ class Job:
             self.finished_at = time.perf_counter()
 
 
+class _BatchedJob:
+    """A job advanced inside the shared batch: its Instance plus the stepper
+    stages (Euler: one right-hand side per step; Heun: two, stepper.py:45-93)."""
+
+    def __init__(self, job, psi, H, eq, stepper):
+        from .batch import Instance
+        ms = eq.sampler
+        self.job = job
+        self.inst = Instance(psi, H, eq, stepper.dt, key=ms.key, numChains=ms.numChains, numSamples=ms.numSamples,
+                             thermalizationSweeps=ms.thermalizationSweeps, sweepSteps=ms.sweepSteps)
+        self.inst.calls = ms.callIndex
+        self.inst.walkers.copy_(ms.states)
+        self.heun = type(stepper).__name__ == "Heun"
+        self.y = psi.params.clone() if self.heun else None
+        self.stage = 0
+        self.k0 = None
+        self.energy0 = None
+        self.t_start = None
+        self.completed = False
+
+    def on_rhs(self):
+        """Runs on a worker thread after the batched pass: the solve, then the
+        stepper stage; sets `completed` when a step has finished."""
+        k = self.inst.rhs()
+        dt = self.inst.dt
+        P = self.inst.psi.params
+        if not self.heun:
+            self.energy0 = self.inst.energy
+            P.add_(dt * k)
+            self.completed = True
+            return
+        if self.stage == 0:
+            self.k0 = k
+            self.energy0 = self.inst.energy
+            P.copy_(self.y + dt * k)
+            self.stage = 1
+            self.completed = False
+            return
+        self.y = self.y + 0.5 * dt * (self.k0 + k)
+        P.copy_(self.y)
+        self.stage = 0
+        self.completed = True
+
+
+def _batchable(psi, H, eq, stepper, max_dim):
+    """RBM job with a Metropolis sampler (mu = 2), a time-independent operator,
+    an Euler or Heun stepper, and a solve of dimension <= max_dim (larger
+    solves would hold every other batched job back for their duration)."""
+    from . import _lib
+    from .sampler import MCSampler
+    if psi._sym is not None or psi.desc.kind not in (_lib.JV_NET_CPXRBM, _lib.JV_NET_RBM):
+        return False
+    if not isinstance(getattr(eq, "sampler", None), MCSampler) or eq.sampler.mu != 2:
+        return False
+    n = eq.sampler.numSamples
+    dim = min(n, psi.numParameters) if type(eq).__name__ == "MinSR" else psi.numParameters
+    if dim > max_dim:
+        return False
+    if type(stepper).__name__ not in ("Euler", "Heun"):
+        return False
+    if type(eq).__name__ == "TDVP" and eq.crossValidation:
+        return False
+    if not H.compiled:
+        H.compile()
+    return bool(torch.equal(H.prefactors(0.0), H.prefactors(1.0)))  # time-independent strings only
+
+
 class _NullCtx:
     def __enter__(self):
         return self
@@ -98,8 +173,16 @@ class Server:
     ``stream_factory`` makes the per-slot streams (``torch.cuda.Stream`` by
     default; host-only tests pass ``lambda: None``)."""
 
-    def __init__(self, max_slots=8, hbm_budget=None, stream_factory=None):
+    def __init__(self, max_slots=8, hbm_budget=None, stream_factory=None, batch_rbm=False, workers=8,
+                 batch_max_dim=2112):
         self.max_slots = int(max_slots)
+        self.batch_rbm = bool(batch_rbm)
+        self.batch_max_dim = int(batch_max_dim)
+        self.workers = int(workers)
+        self.batched = {}        # job_id -> _BatchedJob
+        self._batch = None
+        self._batch_ids = None
+        self._pool = None
         self.stream_factory = stream_factory or torch.cuda.Stream
         if hbm_budget is None:
             free, total = torch.cuda.mem_get_info()
@@ -129,7 +212,7 @@ class Server:
 
     def _admit(self, now):
         admitted = []
-        while self.queue and len(self.active) < self.max_slots:
+        while self.queue and len(self.active) + len(self.batched) < self.max_slots:
             arrival, _, job = self.queue[0]
             if arrival > now or job.footprint > self.hbm_budget - self.used:
                 break  # FIFO by arrival: the head waits for memory
@@ -137,12 +220,79 @@ class Server:
             st = self._stream()
             job.state = "running"
             job.admitted_at = time.perf_counter()
-            th = threading.Thread(target=job.run, args=(st, self._stop), daemon=True)
-            self.active[job.job_id] = (job, th, st)
             self.used += job.footprint
-            th.start()
             admitted.append(job)
+            built = None
+            if self.batch_rbm:
+                try:
+                    with torch.cuda.stream(st):
+                        built = job.make()
+                    st.synchronize()
+                    if _batchable(*built, self.batch_max_dim):
+                        self.batched[job.job_id] = _BatchedJob(job, *built)
+                        self._streams.append(st)
+                        continue
+                except Exception as exc:
+                    self._fail(job, exc)
+                    self._streams.append(st)
+                    continue
+            th = threading.Thread(target=job.run, args=(st, self._stop, built), daemon=True)
+            self.active[job.job_id] = (job, th, st)
+            th.start()
         return admitted
+
+    def _fail(self, job, exc):
+        job.state = "failed"
+        job.error = repr(exc)
+        job.finished_at = time.perf_counter()
+        self.used -= job.footprint
+        self.finished.append(job)
+
+    def _tick_batch(self):
+        """One right-hand side of every batched job: the shared sampling / O_k /
+        O_loc pass, the solves on the worker pool, the stepper stages."""
+        from concurrent.futures import ThreadPoolExecutor
+        from .batch import BatchedSweep
+        ids = tuple(sorted(self.batched))
+        if ids != self._batch_ids:
+            if self._batch is not None:
+                self._batch.close()
+            if self._pool is None:
+                self._pool = ThreadPoolExecutor(max_workers=self.workers)
+            self._batch = BatchedSweep([self.batched[i].inst for i in ids], workers=self.workers, tune="cache",
+                                       graph=False, pool=self._pool)
+            self._batch_ids = ids
+        now = time.perf_counter()
+        for bj in self.batched.values():
+            if bj.stage == 0:
+                bj.t_start = now
+        owner = {id(self.batched[i].inst): self.batched[i] for i in ids}
+        errors = self._batch.evaluate(lambda inst: owner[id(inst)].on_rhs())
+        done_t = time.perf_counter()
+        for k, jid in enumerate(ids):
+            bj = self.batched[jid]
+            job = bj.job
+            exc = errors.get(k)
+            if exc is None and bj.completed:
+                e = float(np.real(bj.energy0))
+                if not math.isfinite(e):
+                    exc = FloatingPointError("non-finite energy at step %d" % job.steps_done)
+                else:
+                    job.latencies.append(done_t - bj.t_start)
+                    job.energies.append(e)
+                    job.t += bj.inst.dt
+                    job.steps_done += 1
+                    bj.completed = False
+            if exc is not None:
+                del self.batched[jid]
+                self._fail(job, exc)
+            elif job.steps_done >= job.num_steps:
+                del self.batched[jid]
+                job.params = bj.inst.psi.params.cpu()
+                job.state = "done"
+                job.finished_at = time.perf_counter()
+                self.used -= job.footprint
+                self.finished.append(job)
 
     def _reap(self):
         for jid in list(self.active):
@@ -154,7 +304,7 @@ class Server:
                 del self.active[jid]
                 self.finished.append(job)
 
-    def run(self, poll=0.002, time_scale=1.0):
+    def run(self, poll=0.002, time_scale=1.0, progress=None):
         """Runs until every submitted job has finished.  Arrival times are in
         seconds after the start of ``run`` (scaled by ``time_scale``)."""
         if self.stream_factory is torch.cuda.Stream:
@@ -162,11 +312,27 @@ class Server:
             from . import _lib
             _lib.call("jv_init", torch.cuda.current_device())
         t_start = time.perf_counter()
-        while self.queue or self.active:
-            now = (time.perf_counter() - t_start) / time_scale
-            self._reap()
-            self._admit(now)
-            time.sleep(poll)
+        try:
+            reported = 0
+            while self.queue or self.active or self.batched:
+                now = (time.perf_counter() - t_start) / time_scale
+                self._reap()
+                if progress is not None and len(self.finished) > reported:
+                    for j in self.finished[reported:]:
+                        progress(j)
+                    reported = len(self.finished)
+                self._admit(now)
+                if self.batched:
+                    self._tick_batch()
+                else:
+                    time.sleep(poll)
+        finally:
+            if self._batch is not None:
+                self._batch.close()
+                self._batch = None
+            if self._pool is not None:
+                self._pool.shutdown(wait=True)
+                self._pool = None
         self.wall = time.perf_counter() - t_start
         return self.report()
 

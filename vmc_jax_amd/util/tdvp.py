@@ -62,6 +62,9 @@ class TDVP:
         self.makeReal = imagFun if makeReal == 'imag' else realFun
         self._imag = makeReal == 'imag'
         self.metaData = None
+        # mixed-precision form: set from the network (NQS(precision="mixed")) at
+        # each call; holomorphic solves then take the compensated BF16 Gram
+        self._mixed = False
 
     def set_diagonal_shift(self, delta):
         self.diagonalShift = delta
@@ -121,7 +124,8 @@ class TDVP:
         cols = getattr(self, "_holo_cols", None)
         if cols is not None and cols[0].numel() * 2 == pr and \
                 cols[0].numel() >= HOLOMORPHIC_SOLVE["imag" if self._imag else "real"]:
-            _lib.call("jv_tdvp_solve_holomorphic", _lib.ptr(D), _lib.ptr(e), _lib.ptr(w), n, pr, _lib.ptr(cols[0]),
+            fn = "jv_tdvp_solve_holomorphic_mixed" if self._mixed else "jv_tdvp_solve_holomorphic"
+            _lib.call(fn, _lib.ptr(D), _lib.ptr(e), _lib.ptr(w), n, pr, _lib.ptr(cols[0]),
                       _lib.ptr(cols[1]), cols[0].numel(), *common)
         else:
             _lib.call("jv_tdvp_solve", _lib.ptr(D), _lib.ptr(e), _lib.ptr(w), n, pr, *common)
@@ -137,6 +141,7 @@ class TDVP:
         tmpParameters = psi.get_parameters()
         psi.set_parameters(netParameters)
         self._holo_cols = psi.holomorphic_columns() if psi.holomorphic else None
+        self._mixed = getattr(psi, "precision", "fp64") == "mixed"
         timer = _Timer(rhsArgs.get("outp", None))
         numSamples = rhsArgs.get("numSamples", None)
         timer.start("sampling")
