@@ -26,8 +26,14 @@ exact sampler, imaginary-time TDVP (Euler dt=1e-2, snrTol=2, pinvTol=1e-8).
 ``--workload c2``: 1D TFIM L=32, h=1, complex RBM alpha=2, 1024 chains x 16384
 samples (25 thermalisation sweeps, sweep length 32), real-time TDVP with Heun
 dt=1e-3.
-``--workload c4``: the batched field sweep (64 independent TFIM chains of
-uneven size and parameter count, run 16 at a time on their own streams).
+``--workload c4``: the batched field sweep (64 independent TFIM instances of
+uneven size and parameter count, one launch per phase for all of them, replayed
+from a captured HIP graph; solves on worker threads).
+``--workload c5``: served jobs (Poisson arrivals of ground-state and real-time
+jobs, continuous admission, RBM jobs joined into one shared batch).
+``--workload gram``: the Gram kernels alone (FP64 matrix cores and the
+compensated BF16 form) at the 10x10 CNN's dual size (8192 x 1240, real) and the
+L=32 complex RBM's holomorphic size (16384 x 2112, complex).
 ``--workload sym``: 1D TFIM L=16, h=1, a SymNet of a complex RBM alpha=2 over
 the 64-element orbit (translation, reflection, spin flip), 1024 chains x 4096
 samples, imaginary-time TDVP (Euler dt=1e-2).
@@ -52,6 +58,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6
+BF16_PEAK_TFLOPS = 2500.0  # dense MFMA
 HBM_PEAK_GBS = 8000.0
 METRIC = "TDVP steps/s + samples/s (TFIM 10x10 CNN) at |dE/E|<=1e-6; served p99 latency"
 
@@ -238,6 +245,59 @@ def roofline_c3(kt, steps, mode, traffic=None):
             "frac": ach / FP64_PEAK_TFLOPS, "executed_tflops": exe, "executed_frac": exe / FP64_PEAK_TFLOPS,
             "traffic": (traffic or {}).get(dom) if mode == "dual" else None,
             "kernel": dom, "avg_launch_ms": ms, "launches": launches}
+
+
+def gram_bench(reps=20):
+    """S = D^dagger D alone: `jv_gram_pp` (FP64 matrix cores) and
+    `jv_gram_pp_mixed` (compensated BF16, FP32 and FP64 inputs), timed with HIP
+    events on the stream the kernels run on.  Rates: `counted` = the full
+    Gram's 2 q^2 n flops (q = p real, 2p for complex data viewed as reals:
+    8 p^2 n), the count the Gram's algorithmic cost is quoted in; `syrk` = the
+    q (q + 1) n flops of the distinct entries; `executed` = the tiles the
+    kernel computes (upper 128 x 128 tiles, padded)."""
+    import torch
+    import vmc_jax_amd._lib as L
+    st = L.stream_ptr()
+
+    def timed(fn):
+        for _ in range(3):
+            fn()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        a.record()
+        for _ in range(reps):
+            fn()
+        b.record()
+        b.synchronize()
+        return a.elapsed_time(b) / reps
+
+    out = {}
+    g = torch.Generator(device="cuda").manual_seed(0)
+    for name, n, p, cpx in (("tfim10x10_cnn_dual", 8192, 1240, False), ("tfim_l32_cpxrbm_holomorphic", 16384, 2112, True)):
+        q = p * (2 if cpx else 1)
+        D64 = torch.randn(n, q, dtype=torch.float64, device="cuda", generator=g)
+        D32 = D64.float()
+        S = torch.empty(p, q, dtype=torch.float64, device="cuda")
+        side = (q + 127) // 128
+        counted, syrk = 2.0 * q * q * n, float(q) * (q + 1) * n
+        executed = 2.0 * (side * (side + 1) // 2) * 128 * 128 * n
+        rec = {"n": n, "p": p, "complex": cpx, "counted_flop": counted}
+        legs = [("fp64", lambda: L.call("jv_gram_pp", L.ptr(D64), n, p, int(cpx), L.ptr(S), st), FP64_PEAK_TFLOPS, 1),
+                ("bf16x3_from_fp64", lambda: L.call("jv_gram_pp_mixed", L.ptr(D64), 0, n, p, int(cpx), L.ptr(S), st),
+                 BF16_PEAK_TFLOPS, 3)]
+        if not cpx:
+            legs.append(("bf16x3_from_fp32",
+                         lambda: L.call("jv_gram_pp_mixed", L.ptr(D32), 1, n, p, 0, L.ptr(S), st), BF16_PEAK_TFLOPS, 3))
+        for leg, fn, peak, products in legs:
+            ms = timed(fn)
+            rate = lambda f: f / (ms * 1e-3) / 1e12
+            rec[leg] = {"ms": ms, "counted_tflops": rate(counted), "syrk_tflops": rate(syrk),
+                        "executed_tflops": rate(executed * products), "peak_tflops": peak,
+                        "counted_frac": rate(counted) * products / peak, "syrk_frac": rate(syrk) * products / peak,
+                        "executed_frac": rate(executed * products) / peak}
+        out[name] = rec
+        del D64, D32, S
+    return out
 
 
 def batched_sweep(num_instances=64, steps=3, seed=11, workers=8):
@@ -534,6 +594,54 @@ def accuracy_check(w, n=1536):
             "residual_rel": float(res / res_o - 1.0)}
 
 
+def c1_oracle_run(steps=200):
+    """The L=10 CpxRBM exact-sampler TDVP workload on the FP64 oracle, every
+    step timed on the host cores: the same initial parameters (N(0, 0.01^2),
+    seed 0, the flat layout shared with the device network), Euler dt = 1e-2,
+    snrTol = 2, pinvTol = 1e-8, makeReal 'real'.  Returns the per-step times
+    and the energies."""
+    from oracle import nets as onets, operator as oop, sampler as osamp, solvers as osol
+    L = 10
+    net = onets.CpxRBM((L,), numHidden=20, bias=True, visibleBias=True)
+    psi = onets.NQS(net, 0.01 * np.random.default_rng(0).normal(size=net.num_real_params()))
+    H = tfim_strings(oop, (L,), -1.0, -0.7)
+    eq = osol.TDVP(osamp.ExactSampler(psi, (L,)), snrTol=2, pinvTol=1e-8, rhsPrefactor=1., makeReal='real')
+    st = osol.Euler(timeStep=1e-2)
+    times, energies = [], []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        dp, _ = st.step(0.0, eq, psi.get_parameters(), hamiltonian=H, psi=psi)
+        psi.set_parameters(dp)
+        times.append(time.perf_counter() - t0)
+        energies.append(float(eq.get_energy_mean()))
+    return times, energies
+
+
+def other_workloads():
+    """The other configurations measured in the same run: c1 (all 200 steps),
+    c2 (Heun steps of the L=32 real-time workload) and c4 (batched field
+    sweep), each on the device in FP64."""
+    out = {}
+    for name, steps, warmup in (("c1", 200, 2), ("c2", 4, 1)):
+        try:
+            w, el, kt, en = run_gpu(name, steps, warmup, "dual")
+            out[name] = {"value": steps / el, "unit": "steps/s", "steps": steps, "warmup": warmup,
+                         "ms_per_step": 1000.0 * el / steps, "config": w.desc, "energy_last": en[-1],
+                         "kernel_ms_per_step": {k: v[0] / steps for k, v in kt.items() if v[1] > 0}}
+            print("other workload %s: %.1f steps/s" % (name, steps / el), file=sys.stderr, flush=True)
+        except Exception as e:  # report, never hide the headline
+            out[name] = {"error": repr(e)}
+    try:
+        rep = batched_sweep(steps=2)
+        out["c4"] = {"value": rep["instance_steps_per_s"], "unit": "instance-steps/s", "steps": rep["steps"],
+                     "instances": rep["instances"], "p50_step_ms": rep["p50_step_ms"],
+                     "p99_step_ms": rep["p99_step_ms"], "batched_pass_ms": rep["batched_pass_ms"],
+                     "config": batched_sweep.__doc__.split("\n")[0].strip()}
+    except Exception as e:
+        out["c4"] = {"error": repr(e)}
+    return out
+
+
 def accuracy_c2(w):
     """The mixed form of the L = 32 workload (compensated BF16 Gram of the
     holomorphic solve) against the FP64 solve on the same samples; the FP64
@@ -563,7 +671,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c4", "c5", "sym"])
+    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c4", "c5", "sym", "gram"])
     ap.add_argument("--mode", default="dual", choices=["T", "dual"], help="MinSR solve form (c3)")
     ap.add_argument("--t-steps", type=int, default=3, help="c3: steps of the other MinSR form reported beside")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -571,6 +679,7 @@ def main():
                     help="c3: compute E_loc before the MinSR solve instead of beside it (isolated kernel timings)")
     ap.add_argument("--serve-jobs", type=int, default=16, help="c3: jobs of the served mix (0 = skip)")
     ap.add_argument("--no-mixed", action="store_true", help="c3: skip the mixed-precision form")
+    ap.add_argument("--no-other-workloads", action="store_true", help="c3: skip the c1 / c2 / c4 legs")
     ap.add_argument("--rate", type=float, default=4.0, help="c5: Poisson arrival rate (jobs/s)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="end with an elementwise 512 MiB -> 512 MiB FP64 device copy (a kernel of known "
@@ -592,6 +701,25 @@ def main():
             "data": "synthetic (random network parameters; samples drawn from the networks)",
             "config": {"workload": batched_sweep.__doc__.split("\n")[0].strip(), "instances": rep["instances"]},
             "report": rep, "wall_s_including_setup": wall}))
+        return
+    if args.workload == "gram":
+        rep = gram_bench(reps=max(1, args.steps))
+        lead = rep["tfim10x10_cnn_dual"]["fp64"]
+        print(json.dumps({
+            "metric": "Gram S = D^T D TFLOP/s (counted 2 q^2 n)", "value": lead["counted_tflops"],
+            "unit": "TFLOP/s", "n_gpus": 1, "steps": max(1, args.steps), "warmup": 3, "ms_per_step": lead["ms"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic (standard normal D)",
+            "config": {"workload": "Gram of the 10x10 CNN dual MinSR (8192 x 1240) and the L=32 complex RBM "
+                                   "holomorphic TDVP (16384 x 2112 complex)"},
+            # the hardware's rate: the multiply-adds of the tiles the kernel computes
+            # (the counted rate credits the symmetric half it skips)
+            "roofline": {"bound": "mfma", "achieved": lead["executed_tflops"], "peak": FP64_PEAK_TFLOPS,
+                         "unit": "TFLOP/s", "frac": lead["executed_frac"], "counted_frac": lead["counted_frac"],
+                         "traffic": None, "kernel": "gram64_tn"},
+            "report": rep}))
+        if args.calibrate_copy:
+            calibration_copy()
         return
     if args.workload == "c5":
         t0 = time.perf_counter()
@@ -628,6 +756,21 @@ def main():
         "minsr_cholesky_steps": w.cholesky_steps,
         "energy_last": energies[-1] if energies else None,
     }
+    if args.workload == "c1" and not args.no_cpu_baseline:
+        # the oracle over the same 200-step trajectory: CPU baseline in full and
+        # the energy after every step against the device's
+        times, en_cpu = c1_oracle_run(args.warmup + args.steps)
+        med = float(np.median(times))
+        out["cpu_baseline"] = {"value": 1.0 / med, "unit": "steps/s", "cores": os.cpu_count(), "kind": "port",
+                               "cpu_model": _cpu_model(),
+                               "sample": "jVMC-algorithm CPU restatement (FP64 numpy oracle): the whole %d-step "
+                                         "trajectory, median step time" % len(times),
+                               "wall_s": float(sum(times))}
+        dev = energies
+        ref = en_cpu[args.warmup:]
+        out["accuracy"] = {"against": "FP64 numpy oracle, same initial parameters, same trajectory",
+                           "max_dE_rel_over_steps": float(max(abs(a - b) / abs(b) for a, b in zip(dev, ref))),
+                           "energy_last_device": dev[-1], "energy_last_oracle": ref[-1]}
     if args.workload == "c2" and not args.no_mixed:
         wm, em, ktm, enm = run_gpu("c2", args.steps, args.warmup, args.mode, True, "mixed")
         out["mixed_form"] = {
@@ -668,6 +811,8 @@ def main():
                 out["served"] = serve_mix(args.serve_jobs)
             except Exception as e:
                 out["served"] = {"error": repr(e)}
+        if not args.no_other_workloads:
+            out["other_workloads"] = other_workloads()
         if not args.no_cpu_baseline:  # the oracle leg: accuracy checker and CPU baseline
             try:
                 out["accuracy"] = accuracy_check(w)
@@ -679,14 +824,19 @@ def main():
                 out["cpu_baseline"] = {"error": repr(e)}
     print(json.dumps(out))
     if args.calibrate_copy:
-        import torch
-        a = torch.ones(64 << 20, dtype=torch.float64, device="cuda")
-        b = torch.empty_like(a)
-        torch.cuda.synchronize()
-        torch.mul(a, 1.0, out=b)
-        torch.cuda.synchronize()
-        print("calibration copy: %d bytes read, %d bytes written" % (a.numel() * 8, b.numel() * 8),
-              file=sys.stderr)
+        calibration_copy()
+
+
+def calibration_copy():
+    """A 512 MiB -> 512 MiB FP64 elementwise copy: a kernel of known HBM
+    traffic that scales the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    a = torch.ones(64 << 20, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    torch.mul(a, 1.0, out=b)
+    torch.cuda.synchronize()
+    print("calibration copy: %d bytes read, %d bytes written" % (a.numel() * 8, b.numel() * 8), file=sys.stderr)
 
 
 if __name__ == "__main__":

@@ -164,6 +164,11 @@ int jv_init(int device);
  * release theirs when they end; the main thread's are kept until this call or
  * process exit (no HIP call is made from exit handlers). */
 int jv_shutdown(void);
+/* Bytes of device memory held by the calling thread's workspaces and solver
+ * library handle on the current device (no reference counterpart: jax sizes
+ * its buffers inside XLA).  A server compares it with its per-job HBM
+ * estimate. */
+int jv_workspace_bytes(int64_t* bytes);
 /* Kernel timing (no reference counterpart; measurement support): when enabled,
  * HIP events are recorded on the launch stream around the named kernels
  * ("gram", "gemm", "eigh", "eloc", "mc_sweep", "net_grad", "net_forward").

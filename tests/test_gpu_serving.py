@@ -103,3 +103,47 @@ def test_shared_batch_server_equals_threads():
         assert a.energies == b.energies, a.job_id
         assert torch.equal(a.params, b.params), a.job_id
     assert rep_b["p99_step_ms"] >= rep_b["p50_step_ms"] > 0
+
+
+def test_served_workload_sizes_alone_and_in_the_mix():
+    """Jobs at the served workload's upper sizes: an L = 64 complex RBM
+    ground-state job (MinSR, 16384 samples from 1024 chains; Pc = 4160, so the
+    complex dual Gram's eigendecomposition) and a 12 x 12 TFIM job with the
+    3-layer, 8-channel CNN (MinSR, 2048 samples): energies and parameters
+    bit-identical alone and in the mix."""
+    def big_rbm(job_id):
+        L = 64
+
+        def make():
+            psi = jv.NQS(jv.nets.CpxRBM(numHidden=L, bias=True), seed=21)
+            psi(torch.zeros((1, 1, L), dtype=torch.int32))
+            H, _ = tfim_pair((L,), -1.0, -1.0)
+            ms = jv.sampler.MCSampler(psi, (L,), 21, updateProposer=jv.sampler.propose_spin_flip, numChains=1024,
+                                      numSamples=16384)
+            return psi, H, jv.util.MinSR(ms, pinvTol=1e-8), jv.util.stepper.Euler(timeStep=1e-2)
+        pr = 2 * (L * L + L)
+        return serving.Job(job_id, make, 2, serving.footprint_bytes(16384, pr, L, True, "minsr", 1024))
+
+    def big_cnn(job_id):
+        side = 12
+
+        def make():
+            net = jv.nets.CNN(F=(3, 3), channels=(8, 8, 8), strides=(1, 1))
+            psi = jv.NQS(net, seed=22)
+            psi(torch.zeros((1, 1, side, side), dtype=torch.int32))
+            H, _ = tfim_pair((side, side), -1.0, -3.0)
+            ms = jv.sampler.MCSampler(psi, (side, side), 22, updateProposer=jv.sampler.propose_spin_flip,
+                                      numChains=256, numSamples=2048)
+            return psi, H, jv.util.MinSR(ms, pinvTol=1e-10), jv.util.stepper.Euler(timeStep=5e-3)
+        return serving.Job(job_id, make, 2, serving.footprint_bytes(2048, 1240, side * side, False, "minsr", 256))
+
+    print("served sizes: alone", flush=True)
+    _, (r_alone,) = run([big_rbm(0)], 1)
+    _, (c_alone,) = run([big_cnn(1)], 1)
+    print("served sizes: mix", flush=True)
+    rep, mix = run([big_rbm(0), big_cnn(1)], 2)
+    assert rep["jobs_done"] == 2, [j.error for j in mix]
+    assert mix[0].energies == r_alone.energies and len(r_alone.energies) == 2
+    assert torch.equal(mix[0].params, r_alone.params)
+    assert mix[1].energies == c_alone.energies
+    assert torch.equal(mix[1].params, c_alone.params)

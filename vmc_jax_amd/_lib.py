@@ -83,6 +83,7 @@ _SIGS = {
     "jv_set_device": [ctypes.c_int],
     "jv_init": [ctypes.c_int],
     "jv_shutdown": [],
+    "jv_workspace_bytes": [_vp],
     "jv_timer_enable": [ctypes.c_int],
     "jv_timer_reset": [],
     "jv_timer_read": [ctypes.c_char_p, ctypes.POINTER(_dbl), ctypes.POINTER(_i64)],

@@ -222,6 +222,21 @@ int jv_init(int device) {
   return JV_OK;
 }
 
+int jv_workspace_bytes(int64_t* bytes) {
+  if (!bytes) return jv::fail(JV_E_ARG, "jv_workspace_bytes: null output");
+  int d = 0;
+  JV_HIP(hipGetDevice(&d));
+  size_t total = 0;
+  if ((int)jv::t_state.dev.size() > d) {
+    const auto& s = jv::t_state.dev[d];
+    for (size_t b : s.size) total += b;
+    size_t lib = 0;  // the solver library handle's own device memory
+    if (s.blas && rocblas_get_device_memory_size(s.blas, &lib) == rocblas_status_success) total += lib;
+  }
+  *bytes = (int64_t)total;
+  return JV_OK;
+}
+
 int jv_shutdown(void) {
   // releases the calling thread's workspaces and handles on every device
   for (size_t d = 0; d < jv::t_state.dev.size(); ++d) {

@@ -42,6 +42,47 @@ __device__ inline void upper_tile(int t, int nt, int& tm, int& tn) {
   tn = r + t;
 }
 
+// Workgroups are dispatched to the 8 XCDs round-robin by linear id, and each
+// XCD has its own L2.  Workgroup w of nwg takes the (slice, tile) pair at
+// position start(w % 8) + w / 8 of the slice-major order, so every XCD works
+// through one contiguous run of pairs: whole sample slices when there are at
+// least 8 of them (each slice's column panels are then fetched by one XCD),
+// else a run of tiles of one slice.  Within a slice the tiles are ordered by
+// square super-blocks of XSB x XSB tiles, so a run touches few column panels.
+// Returns the tile (tm <= tn), its row-major upper-triangle index and the slice.
+constexpr int XCDS = 8;
+constexpr int XSB = 8;
+__device__ inline void xcd_pair(int w, int nwg, int nt, int& tm, int& tn, int& tile, int& slice) {
+  const int x = w % XCDS, j = w / XCDS;
+  const int base = nwg / XCDS, rem = nwg % XCDS;
+  const int q = x * base + (x < rem ? x : rem) + j;
+  const int ntiles = nt * (nt + 1) / 2;
+  slice = q / ntiles;
+  int k = q - slice * ntiles;
+  const int nsb = (nt + XSB - 1) / XSB;
+  tm = tn = tile = 0;
+  for (int I = 0; I < nsb; ++I) {
+    const int r0 = I * XSB, r1 = min(nt, r0 + XSB);
+    for (int J = I; J < nsb; ++J) {
+      const int c0 = J * XSB, c1 = min(nt, c0 + XSB);
+      const int cnt = I == J ? (r1 - r0) * (r1 - r0 + 1) / 2 : (r1 - r0) * (c1 - c0);
+      if (k < cnt) {
+        if (I == J) {
+          upper_tile(k, r1 - r0, tm, tn);
+          tm += r0;
+          tn += r0;
+        } else {
+          tm = r0 + k / (c1 - c0);
+          tn = c0 + k % (c1 - c0);
+        }
+        tile = tm * nt - tm * (tm - 1) / 2 + (tn - tm);
+        return;
+      }
+      k -= cnt;
+    }
+  }
+}
+
 __device__ inline void split_bf16(float x, __bf16& hi, __bf16& lo) {
   hi = (__bf16)x;
   lo = (__bf16)(x - (float)hi);
@@ -70,14 +111,14 @@ __global__ void __launch_bounds__(256) gram64_tn(const double* __restrict__ X, i
                                                  int64_t kchunk, int ntile_side, double* __restrict__ part) {
   __shared__ double As[G6K][G6ROW];
   __shared__ double Bs[G6K][G6ROW];
-  int tm, tn;
-  upper_tile(blockIdx.x, ntile_side, tm, tn);
+  int tm, tn, tile, slice;
+  xcd_pair(blockIdx.x, gridDim.x, ntile_side, tm, tn, tile, slice);
   const int ntiles = ntile_side * (ntile_side + 1) / 2;
   const int tid = threadIdx.x, lane = tid % 64, wv = tid / 64;
   const int wm = (wv / 2) * 64, wn = (wv % 2) * 64;
   const int cp = tid % 64, rg = tid / 64;  // column pair, row group: rows rg + 4 j
   const int64_t m0 = (int64_t)tm * GT, n0 = (int64_t)tn * GT;
-  const int64_t kbeg = (int64_t)blockIdx.y * kchunk;
+  const int64_t kbeg = (int64_t)slice * kchunk;
   const int64_t kend = std::min<int64_t>(k_total, kbeg + kchunk);
   typedef double f64x4 __attribute__((ext_vector_type(4)));
   f64x4 acc[4][4];
@@ -133,7 +174,7 @@ __global__ void __launch_bounds__(256) gram64_tn(const double* __restrict__ X, i
         for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
     }
   }
-  double* P = part + ((size_t)blockIdx.y * ntiles + blockIdx.x) * GT * GT;
+  double* P = part + ((size_t)slice * ntiles + tile) * GT * GT;
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -182,12 +223,12 @@ __global__ void __launch_bounds__(256) gram3_pre(const __bf16* __restrict__ Y, i
   extern __shared__ __align__(16) unsigned char gram_lds[];
   __bf16* Ap = (__bf16*)gram_lds;          // [2][GT][GROW]: hi then lo
   __bf16* Bp = Ap + 2 * GPLANE;
-  int tm, tn;
-  upper_tile(blockIdx.x, ntile_side, tm, tn);
+  int tm, tn, tile, slice;
+  xcd_pair(blockIdx.x, gridDim.x, ntile_side, tm, tn, tile, slice);
   const int ntiles = ntile_side * (ntile_side + 1) / 2;
   const int tid = threadIdx.x, lane = tid % 64, wv = tid / 64;
   const int wm = (wv / 2) * 64, wn = (wv % 2) * 64;
-  const int64_t kb0 = (int64_t)blockIdx.y * kbchunk;
+  const int64_t kb0 = (int64_t)slice * kbchunk;
   const int64_t kb1 = std::min<int64_t>(nkb, kb0 + kbchunk);
   f32x4 acc[4][4];
 #pragma unroll
@@ -243,7 +284,7 @@ __global__ void __launch_bounds__(256) gram3_pre(const __bf16* __restrict__ Y, i
         }
     }
   }
-  float* P = part + ((size_t)blockIdx.y * ntiles + blockIdx.x) * GT * GT;
+  float* P = part + ((size_t)slice * ntiles + tile) * GT * GT;
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -352,8 +393,7 @@ int gram_pp_bf16x3(const void* D, int f32, int64_t n, int64_t p, int cpx, double
   JV_LAUNCH_CHECK("gram3_split");
   int rc = set_lds_limit((const void*)gram3_pre, lds);
   if (rc) return rc;
-  hipLaunchKernelGGL(gram3_pre, dim3((unsigned)ntiles, (unsigned)splits), dim3(256), lds, st, Y, nkb, qp, kbchunk,
-                     nts, part);
+  hipLaunchKernelGGL(gram3_pre, dim3((unsigned)(ntiles * splits)), dim3(256), lds, st, Y, nkb, qp, kbchunk, nts, part);
   JV_LAUNCH_CHECK("gram3_pre");
   hipLaunchKernelGGL(gram3_reduce<float>, dim3((unsigned)(ntiles * 16)), dim3(256), 0, st, part, (int)splits, nts, p,
                      cpx, S);
@@ -379,11 +419,11 @@ int gram_pp_f64(const double* D, int64_t n, int64_t p, int cpx, double* S, hipSt
   if (!part) return fail(JV_E_NOMEM, "gram_pp_f64: workspace allocation failed");
   ScopedTimer tm("gram", st);
   if (q % 2 == 0 && ((uintptr_t)D % 16) == 0)
-    hipLaunchKernelGGL(gram64_tn<true>, dim3((unsigned)ntiles, (unsigned)splits), dim3(256), 0, st, D, n, q, kchunk,
-                       nts, part);
+    hipLaunchKernelGGL(gram64_tn<true>, dim3((unsigned)(ntiles * splits)), dim3(256), 0, st, D, n, q, kchunk, nts,
+                       part);
   else
-    hipLaunchKernelGGL(gram64_tn<false>, dim3((unsigned)ntiles, (unsigned)splits), dim3(256), 0, st, D, n, q, kchunk,
-                       nts, part);
+    hipLaunchKernelGGL(gram64_tn<false>, dim3((unsigned)(ntiles * splits)), dim3(256), 0, st, D, n, q, kchunk, nts,
+                       part);
   JV_LAUNCH_CHECK("gram64_tn");
   hipLaunchKernelGGL(gram3_reduce<double>, dim3((unsigned)(ntiles * 16)), dim3(256), 0, st, part, (int)splits, nts,
                      p, cpx, S);

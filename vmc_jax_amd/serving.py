@@ -354,20 +354,95 @@ class Server:
         self._stop.set()
 
 
-def footprint_bytes(num_samples, num_real_params, num_sites, complex_params, solver, chains):
-    """HBM estimate of one job: O (compact and jVMC layouts for TDVP), centred
-    copies, the solver matrix and eigenvectors, SNR buffers, walkers and
-    samples."""
+def _cdiv(a, b):
+    return -(-a // b)
+
+
+def _gram_slice_bytes(n, q):
+    """Partial tiles of the FP64 Gram of an n x q real matrix: 128 x 128 upper
+    tiles, the sample axis cut into slices of whole 32-sample stages while the
+    tiles number fewer than 512 and each slice keeps >= 16 stages (the rule of
+    gram_pp_f64 in csrc/gram_mixed.hip)."""
+    side = _cdiv(q, 128)
+    ntiles, nkb, splits = side * (side + 1) // 2, _cdiv(n, 32), 1
+    while ntiles * splits < 512 and nkb // (splits * 2) >= 16:
+        splits *= 2
+    splits = _cdiv(n, _cdiv(nkb, splits) * 32)
+    return splits * ntiles * 128 * 128 * 8
+
+
+def _bf16_gram_bytes(n, q):
+    """The compensated BF16 Gram's FP32 partial tiles plus its pre-split
+    operand (hi and lo planes, 64-sample stages; gram_pp_bf16x3)."""
+    side = _cdiv(q, 128)
+    ntiles, nkb, splits = side * (side + 1) // 2, _cdiv(n, 64), 1
+    while ntiles * splits < 512 and nkb // (splits * 2) >= 8:
+        splits *= 2
+    splits = _cdiv(nkb, _cdiv(nkb, splits))
+    return splits * ntiles * 128 * 128 * 4 + nkb * side * 128 * 2 * 64 * 2
+
+
+def _gemm_split_bytes(m, k, cpx):
+    """Split-K partials of the symmetric FP64 GEMM that forms the m x m
+    T = D D^dagger (64 x 64 tiles; K split while fewer than 1024 workgroups and
+    each part keeps >= 64 of k; csrc/linalg.hip gemm)."""
+    ntn = _cdiv(m, 64)
+    blocks, splits = ntn * (ntn + 1) // 2, 1
+    while blocks * splits < 1024 and k // (splits * 2) >= 64:
+        splits *= 2
+    return splits * m * m * (16 if cpx else 8) if splits > 1 else 0
+
+
+def footprint_bytes(num_samples, num_real_params, num_sites, complex_params, solver, chains, precision="fp64"):
+    """HBM estimate (bytes) of one job: samples, walkers and s' strings; O and
+    its centred copy; every workspace buffer of the solve route the equation
+    takes.  ``solver``: "tdvp", or "minsr", whose route follows MinSR's
+    default: the P x P dual Gram when the samples outnumber the (compact)
+    parameters, else the N x N T form (Cholesky attempt on T, or on its real
+    2N x 2N form for complex data, then the eigendecomposition).  Counted per
+    route: the Gram's split-sample partial tiles, the shifted copy, the second
+    factor and the inverse of the Cholesky route, the eigenvectors, the
+    explicit Q and its product, the divide-and-conquer buffers, the complex
+    tridiagonal path's buffers and, where the library's eigendecomposition
+    runs, its workspace, plus the solver library handle of the job's thread
+    (rocBLAS / rocSOLVER keep ~134 MB of device memory per handle; counted
+    for every route, since a real solve needs the library once its spectrum
+    is cut).  ``precision="mixed"`` adds the compensated Gram's buffers.  A
+    25 % margin and 64 MiB cover rounding of buffer sizes."""
     n, pr = int(num_samples), int(num_real_params)
     pc = pr // 2 if complex_params else pr
-    b = 0
-    b += n * num_sites * 4 * 2 + chains * num_sites * 4   # configs, s' scratch, walkers
-    b += n * 16 * 6                                       # log psi, E_loc, weights, centred copies
+    esz = 16 if complex_params else 8
+    b = n * num_sites * 4 * 6 + chains * num_sites * 4    # samples, s' slots of the operator, walkers
+    b += n * 16 * 8                                       # log psi, E_loc, weights, centred E_loc copies
     if solver == "tdvp":
-        b += n * pr * 16 * 4                              # O, centred O, SNR data and projection
-        b += pr * pr * 16 * 3                             # S0, S, eigenvectors
+        b += n * pr * 16 * 4                              # O (flat layout) and centred O, SNR data Q and R
+        b += n * pc * 16 * 2                              # holomorphic G and the SNR projection M
+        b += pr * pr * 8 * 5                              # S0 (complex), Z, S
+        b += pc * pc * 8 * 10                             # A, eigenvectors, tridiagonal Z and Zc, D&C buffers
+        b += pr * pr * 16 * 3                             # library eigendecomposition workspace
+        b += _gram_slice_bytes(n, 2 * pr)
+        if precision == "mixed":
+            b += _bf16_gram_bytes(n, 2 * pc)
+    elif solver == "minsr":
+        b += n * pc * esz * 2                             # O and centred O
+        if n > pc:                                        # dual form
+            m = pc
+            b += m * m * esz * 4                          # S, eigenvectors, library eigendecomposition workspace
+            b += m * m * 8 * 12                           # shifted copy, second factor, inverse, Z, Q, Q x,
+            #                                               D&C buffers (3), complex tridiagonal Zc (2)
+            b += _gram_slice_bytes(n, m * (2 if complex_params else 1))
+            if precision == "mixed":
+                b += _bf16_gram_bytes(n, m * (2 if complex_params else 1))
+        else:                                             # T form
+            m = 2 * n if complex_params else n            # the Cholesky route's real form
+            b += n * n * esz * 4                          # T, eigenvectors, library eigendecomposition workspace
+            b += m * m * 8 * 7                            # shifted copy, factor, inverse, Z, D&C buffers (3)
+            b += _gemm_split_bytes(n, pc, complex_params)
     else:
-        m = min(n, pc)
-        b += n * pc * (16 if complex_params else 8) * 2   # O and centred O
-        b += m * m * (16 if complex_params else 8) * 2    # Gram and eigenvectors
+        raise ValueError("solver must be 'tdvp' or 'minsr'")
+    b += LIBRARY_HANDLE_BYTES
     return int(1.25 * b) + (64 << 20)
+
+
+# device memory a thread's solver library handle holds (measured: 134 MB)
+LIBRARY_HANDLE_BYTES = 160 << 20
