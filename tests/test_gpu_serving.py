@@ -131,6 +131,7 @@ def test_served_workload_sizes_alone_and_in_the_mix():
             net = jv.nets.CNN(F=(3, 3), channels=(8, 8, 8), strides=(1, 1))
             psi = jv.NQS(net, seed=22)
             psi(torch.zeros((1, 1, side, side), dtype=torch.int32))
+            psi.set_parameters(net.he_normal(psi.desc, np.random.default_rng(22)))  # the workload's init
             H, _ = tfim_pair((side, side), -1.0, -3.0)
             ms = jv.sampler.MCSampler(psi, (side, side), 22, updateProposer=jv.sampler.propose_spin_flip,
                                       numChains=256, numSamples=2048)
@@ -140,6 +141,7 @@ def test_served_workload_sizes_alone_and_in_the_mix():
     print("served sizes: alone", flush=True)
     _, (r_alone,) = run([big_rbm(0)], 1)
     _, (c_alone,) = run([big_cnn(1)], 1)
+    assert r_alone.state == c_alone.state == "done", (r_alone.error, c_alone.error)
     print("served sizes: mix", flush=True)
     rep, mix = run([big_rbm(0), big_cnn(1)], 2)
     assert rep["jobs_done"] == 2, [j.error for j in mix]

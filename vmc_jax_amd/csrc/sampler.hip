@@ -342,6 +342,204 @@ __global__ void __launch_bounds__(64 * G) mc_cnn_win(CnnGeom g, const T* __restr
 
 
 
+// ------------------------------------------------------------------ CNN chains, 3x3 filters, windows <= 64
+// The same chain, proposal order, arithmetic and acceptance as mc_cnn_win for
+// 3x3 filters, at most three layers and windows of at most 64 elements (one
+// window element per lane in every layer).  What a proposal at site s reads
+// differs from mc_cnn_win only in how the input locations are found: a
+// window element's nine inputs are either inside the previous layer's window
+// (a fixed slot n + aq * pey + bq) or at the stored position of the site
+// shifted by (-aq, -bq); which of the two, and the slot or the shift, depend
+// on the lane and the layer only.  Every lane keeps these 9 codes per layer
+// in registers, so a proposal computes its input addresses with a few integer
+// operations and issues all input loads of a layer at once, with no table
+// pass and no table reads between the draws.
+constexpr int MCW3_MAXL = 3;
+
+__host__ __device__ inline bool mc_win3_ok(const CnnGeom& g) {
+  return g.Fx == 3 && g.Fy == 3 && g.nl >= 1 && g.nl <= MCW3_MAXL && win_ex(g, g.nl - 1) <= g.Lx &&
+         win_ey(g, g.nl - 1) <= g.Ly && win_ex(g, g.nl - 1) * win_ey(g, g.nl - 1) <= WAVE;
+}
+// LDS: the FP64 reduction slots, the planes U and x0 in the arithmetic type,
+// the configuration and the neighbours of the first forward pass.
+__host__ __device__ inline size_t mc_win3_bytes(const CnnGeom& g, int cm, int cs, int tsize = 8) {
+  const size_t t = (size_t)g.nl * cm * cs + g.n;
+  return 8 * 16 + (size_t)tsize * t + 4 * (size_t)g.n + 2 * (size_t)g.n * g.FF;
+}
+// Wrap a row (column) index shifted by at most one period.
+__device__ inline int wrap1(int q, int L) { return q < 0 ? q + L : (q >= L ? q - L : q); }
+
+template <int CM, int G, int CS, typename T>
+__global__ void __launch_bounds__(64 * G) mc_cnn_win3(CnnGeom g, const T* __restrict__ Wpk, CnnPack pk,
+                                                      jv_mc_args a, int32_t* __restrict__ walkers,
+                                                      int32_t* __restrict__ configs,
+                                                      int64_t* __restrict__ counters) {
+  constexpr int CG = CM / G;
+  extern __shared__ double lds[];
+  const int tid = threadIdx.x, nt = blockDim.x, lane = tid % WAVE;
+  const int w = __builtin_amdgcn_readfirstlane(tid / WAVE);
+  const int chain = blockIdx.x;
+  const int L1 = g.nl - 1;
+  double* red = lds;
+  T* U = (T*)(red + 16);
+  T* x0 = U + (size_t)g.nl * CM * CS;
+  int32_t* s = (int32_t*)(x0 + g.n);
+  short* nbr = (short*)(s + g.n);
+  const cst_t<T>* Wp = (const cst_t<T>*)Wpk;
+  for (int t = tid; t < g.n * g.FF; t += nt) {
+    const int p = t / g.FF, k = t % g.FF;
+    const int i = p / g.Ly, j = p % g.Ly, ky = k / g.Fy, kx = k % g.Fy;
+    nbr[t] = (short)(((i + ky) % g.Lx) * g.Ly + (j + kx) % g.Ly);
+  }
+  for (int i = tid; i < g.n; i += nt) {
+    s[i] = walkers[(int64_t)chain * g.n + i];
+    x0[i] = (T)(2 * s[i] - 1);
+  }
+  // per layer: this lane's window element (a, b) and its nine input codes
+  // (>= 0: the slot in the previous layer's window; < 0: -1 - the packed
+  // shift ((aq + 16) << 8) | (bq + 16) of the stored position)
+  int code[MCW3_MAXL][9];
+  int wa[MCW3_MAXL], wb[MCW3_MAXL];
+#pragma unroll
+  for (int l = 0; l < MCW3_MAXL; ++l) {
+    const int ey = 1 + 2 * (l + 1), pex = l == 0 ? 1 : 1 + 2 * l, pey = pex;
+    const int ja = lane / ey, jb = lane - (lane / ey) * ey;
+    wa[l] = ja;
+    wb[l] = jb;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      const int aq = ja - k / 3, bq = jb - k % 3;
+      const bool in = l > 0 && aq >= 0 && aq < pex && bq >= 0 && bq < pey;
+      code[l][k] = in ? g.n + aq * pey + bq : -1 - (((aq + 16) << 8) | (bq + 16));
+    }
+  }
+  __syncthreads();
+  win_forward<CM, G, T>(g, Wp, pk, x0, U, CM * CS, CS, nbr, tid, nt, [] { __syncthreads(); });
+  const int total = a.therm_steps + a.samples_per_chain * a.sweep_steps;
+  int dsite = 0;
+  double du = 0.0;
+  auto batch = [&](int base) {
+    if (base + lane < total)
+      mc_draw(a.seed, (uint32_t)(base + lane), (uint32_t)chain, a.call_index, a.stream, g.n, dsite, du);
+  };
+  batch(0);
+  int site = __builtin_amdgcn_readlane(dsite, 0);
+  __syncthreads();
+  int64_t nacc = 0;
+  for (int step = 0; step < total; ++step) {
+    const int r = step & 63;
+    const int ulo = __builtin_amdgcn_readlane((int)__double2loint(du), r);
+    const int uhi = __builtin_amdgcn_readlane((int)__double2hiint(du), r);
+    const double u = __hiloint2double(uhi, ulo);
+    const int si = site / g.Ly, sj = site - (site / g.Ly) * g.Ly;
+    double diff = 0.0;
+#pragma unroll
+    for (int l = 0; l < MCW3_MAXL; ++l) {
+      if (l < g.nl) {
+        const int ey = 1 + 2 * (l + 1), wsz = ey * ey;
+        const cst_t<T>* K = Wp + pk.koff[l];
+        const cst_t<T>* B = Wp + pk.boff[l] + w * CG;
+        T* Ul = U + (size_t)l * CM * CS + (size_t)w * CG * CS;
+        if (lane < wsz) {
+          int q[9];
+#pragma unroll
+          for (int k = 0; k < 9; ++k) {
+            const int e = code[l][k];
+            const int t = -1 - e;
+            const int qi = wrap1(si - ((t >> 8) - 16), g.Lx), qj = wrap1(sj - ((t & 255) - 16), g.Ly);
+            q[k] = e >= 0 ? e : qi * g.Ly + qj;
+          }
+          T z[CG];
+#pragma unroll
+          for (int c = 0; c < CG; ++c) z[c] = B[c];
+          if (l == 0) {
+            lds_t<T>* X0 = (lds_t<T>*)x0;
+            T xv[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) xv[k] = X0[q[k]];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) xv[k] = q[k] == site ? -xv[k] : xv[k];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) {
+              const cst_t<T>* wk = K + (k * G + w) * CG;
+#pragma unroll
+              for (int c = 0; c < CG; ++c) z[c] = fma(xv[k], wk[c], z[c]);
+            }
+          } else {
+            lds_t<T>* X = (lds_t<T>*)(U + (size_t)(l - 1) * CM * CS);
+#pragma unroll
+            for (int k = 0; k < 9; ++k) {
+              T xv[CM];
+#pragma unroll
+              for (int ci = 0; ci < CM; ++ci) xv[ci] = X[q[k] + ci * CS];
+              const cst_t<T>* wk = K + (size_t)(k * G + w) * CM * CG;
+#pragma unroll
+              for (int ci = 0; ci < CM; ++ci)
+#pragma unroll
+                for (int c = 0; c < CG; ++c) z[c] = fma(xv[ci], wk[ci * CG + c], z[c]);
+            }
+          }
+          if (l < L1) {
+#pragma unroll
+            for (int c = 0; c < CG; ++c) Ul[c * CS + g.n + lane] = log_cosh(z[c]);
+          } else {
+            const int p = wrap1(si - wa[l], g.Lx) * g.Ly + wrap1(sj - wb[l], g.Ly);
+#pragma unroll
+            for (int c = 0; c < CG; ++c) {
+              const T v = log_cosh(z[c]);
+              Ul[c * CS + g.n + lane] = v;
+              diff += (double)(v - Ul[c * CS + p]);
+            }
+          }
+        }
+        if (l < L1) __syncthreads();
+      }
+    }
+    diff = wave_sum(diff);
+    if (lane == 0) red[w] = diff;
+    __syncthreads();
+    double dsum = 0.0;
+#pragma unroll
+    for (int i = 0; i < G; ++i) dsum += red[i];
+    const double dl = dsum * g.inv_nrm;
+    const bool acc = u < exp(a.mu * dl);  // random.bernoulli(key, P), sampler.py:336-340
+    if (acc) {
+      // wave w commits its own channels of every layer's window
+#pragma unroll
+      for (int l = 0; l < MCW3_MAXL; ++l) {
+        const int ey = 1 + 2 * (l + 1);
+        if (l < g.nl && lane < ey * ey) {
+          const int p = wrap1(si - wa[l], g.Lx) * g.Ly + wrap1(sj - wb[l], g.Ly);
+          T* Ul = U + (size_t)l * CM * CS + (size_t)w * CG * CS;
+#pragma unroll
+          for (int c = 0; c < CG; ++c) Ul[c * CS + p] = Ul[c * CS + g.n + lane];
+        }
+      }
+      if (tid == 0) {
+        s[site] = 1 - s[site];
+        x0[site] = -x0[site];
+      }
+      ++nacc;
+    }
+    if (step + 1 < total) {
+      if (((step + 1) & 63) == 0) batch(step + 1);
+      site = __builtin_amdgcn_readlane(dsite, (step + 1) & 63);
+    }
+    __syncthreads();
+    const int k = step + 1 - a.therm_steps;
+    if (k > 0 && k % a.sweep_steps == 0) {
+      const int64_t t = k / a.sweep_steps - 1;
+      int32_t* outc = configs + (t * a.num_chains + chain) * g.n;
+      for (int i = tid; i < g.n; i += nt) outc[i] = s[i];
+    }
+  }
+  for (int i = tid; i < g.n; i += nt) walkers[(int64_t)chain * g.n + i] = s[i];
+  if (tid == 0) {
+    atomicAdd((unsigned long long*)&counters[0], (unsigned long long)total);
+    atomicAdd((unsigned long long*)&counters[1], (unsigned long long)nacc);
+  }
+}
+
 // ------------------------------------------------------------------ weights
 // p_i = exp(scale * Re(logpsi_i) - shift); one block, deterministic order.
 __global__ void __launch_bounds__(1024) weights_sum(const double* __restrict__ lp, int64_t n, double scale,
@@ -413,7 +611,9 @@ int jv_mc_sample(const jv_net* net, const double* params, const jv_mc_args* args
     // waves per chain: each owns cm / groups output channels (one channel per
     // wave, eight waves, measured slower on the 10x10 CNN: 3.69 against 2.99 ms)
     const int groups = cm >= 8 ? 4 : 2;
-    const size_t lds = mc_win_bytes(cg, cm, cs, mixed ? 4 : 8);
+    // 3x3 filters with windows of at most 64 elements: input codes in registers
+    const bool w3 = mc_win3_ok(cg);
+    const size_t lds = w3 ? mc_win3_bytes(cg, cm, cs, mixed ? 4 : 8) : mc_win_bytes(cg, cm, cs, mixed ? 4 : 8);
     const CnnPack pk = make_cnn_pack(cg, cm, groups);
     ScopedTimer tm("mc_sweep", st);
     const void* wpk = mixed ? (const void*)pack_cnn_weights<float>(cg, pk, params, st, 43)
@@ -423,11 +623,19 @@ int jv_mc_sample(const jv_net* net, const double* params, const jv_mc_args* args
     const void* fn = nullptr;
 #define JV_MCW(CMv, Gv, CSv, Tv)                                                                        \
   if (cm == CMv && cs == CSv && (sizeof(Tv) == 4) == mixed) {                                          \
-    fn = (const void*)mc_cnn_win<CMv, Gv, CSv, Tv>;                                                     \
-    rc = set_lds_limit(fn, lds);                                                                        \
-    if (rc) return rc;                                                                                  \
-    hipLaunchKernelGGL((mc_cnn_win<CMv, Gv, CSv, Tv>), dim3((unsigned)a.num_chains), dim3(64 * Gv), lds, st, \
-                       cg, (const Tv*)wpk, pk, a, walkers, configs, counters);                          \
+    if (w3) {                                                                                           \
+      fn = (const void*)mc_cnn_win3<CMv, Gv, CSv, Tv>;                                                  \
+      rc = set_lds_limit(fn, lds);                                                                      \
+      if (rc) return rc;                                                                                \
+      hipLaunchKernelGGL((mc_cnn_win3<CMv, Gv, CSv, Tv>), dim3((unsigned)a.num_chains), dim3(64 * Gv), lds, \
+                         st, cg, (const Tv*)wpk, pk, a, walkers, configs, counters);                    \
+    } else {                                                                                            \
+      fn = (const void*)mc_cnn_win<CMv, Gv, CSv, Tv>;                                                   \
+      rc = set_lds_limit(fn, lds);                                                                      \
+      if (rc) return rc;                                                                                \
+      hipLaunchKernelGGL((mc_cnn_win<CMv, Gv, CSv, Tv>), dim3((unsigned)a.num_chains), dim3(64 * Gv), lds, \
+                         st, cg, (const Tv*)wpk, pk, a, walkers, configs, counters);                    \
+    }                                                                                                   \
   }
 #define JV_MCW_CS(CMv, Gv)                                                                              \
   JV_MCW(CMv, Gv, 128, double) JV_MCW(CMv, Gv, 256, double) JV_MCW(CMv, Gv, 512, double)               \
