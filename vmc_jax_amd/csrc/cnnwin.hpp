@@ -181,4 +181,59 @@ __device__ inline void win_preact0(const CnnGeom& g, const cst_t<T>* __restrict_
   }
 }
 
+// 3x3-filter versions of win_preact / win_preact0 for layer LC of a network
+// whose windows are (1 + 2 (l + 1))^2: the same inputs and the same order of
+// fused multiply-adds (kernel row, kernel column, input channel, output
+// channel), with the window shapes as constants and the nine input locations
+// computed before any load.
+__device__ inline int wrap_once(int q, int L) { return q < 0 ? q + L : (q >= L ? q - L : q); }
+
+template <int CM, int LC, typename T>
+__device__ inline void win_preact3(const CnnGeom& g, const cst_t<T>* __restrict__ K, const cst_t<T>* __restrict__ B,
+                                   const T* Xb, const T* Xw, int si, int sj, int a, int b, T* z) {
+  constexpr int pex = 1 + 2 * LC, pey = pex, pwsz = pex * pey;
+#pragma unroll
+  for (int c = 0; c < CM; ++c) z[c] = B[c];
+#pragma unroll 1
+  for (int k = 0; k < 9; ++k) {
+    const int ky = k / 3, kx = k - 3 * (k / 3);
+    const int aq = a - ky, bq = b - kx;
+    const bool in = aq >= 0 && aq < pex && bq >= 0 && bq < pey;
+    const int qi = wrap_once(si - aq, g.Lx), qj = wrap_once(sj - bq, g.Ly);
+    lds_t<T>* x = (lds_t<T>*)(in ? Xw + (aq * pey + bq) : Xb + (qi * g.Ly + qj));
+    const int xs = in ? pwsz : g.n;
+    const cst_t<T>* w = K + k * CM * CM;
+#pragma unroll
+    for (int ci = 0; ci < CM; ++ci) {
+      const T xv = x[ci * xs];
+#pragma unroll
+      for (int c = 0; c < CM; ++c) z[c] = fma(xv, w[ci * CM + c], z[c]);
+    }
+  }
+}
+
+template <int CM, typename T>
+__device__ inline void win_preact03(const CnnGeom& g, const cst_t<T>* __restrict__ K, const cst_t<T>* __restrict__ B,
+                                    const T* x0, T xflip, int site, int si, int sj, int a, int b, T* z) {
+#pragma unroll
+  for (int c = 0; c < CM; ++c) z[c] = B[c];
+  T xv[9];
+#pragma unroll
+  for (int ky = 0; ky < 3; ++ky) {
+    const int qi = wrap_once(si - a + ky, g.Lx);
+#pragma unroll
+    for (int kx = 0; kx < 3; ++kx) {
+      const int q = qi * g.Ly + wrap_once(sj - b + kx, g.Ly);
+      const T v = ((lds_t<T>*)x0)[q];
+      xv[ky * 3 + kx] = q == site ? xflip : v;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    const cst_t<T>* w = K + k * CM;
+#pragma unroll
+    for (int c = 0; c < CM; ++c) z[c] = fma(xv[k], w[c], z[c]);
+  }
+}
+
 }  // namespace jv

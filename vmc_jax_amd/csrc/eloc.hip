@@ -3,6 +3,8 @@
 // incremental (RBM) or receptive-field-limited (CNN) re-evaluation.
 #include <algorithm>
 
+#include <type_traits>
+
 #include "cnnwin.hpp"
 #include "ops.hpp"
 #include "rbm.hpp"
@@ -227,7 +229,7 @@ __host__ __device__ inline size_t elo_win_bytes(const CnnGeom& g, int cm, int no
   const size_t d = 4 * (size_t)nops + 16 + (size_t)fch * win_ex(g, g.nl - 1) * win_ey(g, g.nl - 1);
   size_t t = (size_t)g.n + (size_t)g.nl * cm * g.n;
   for (int l = 0; l < g.nl - 1; ++l) t += (size_t)fch * cm * win_ex(g, l) * win_ey(g, l);
-  const size_t i32 = (size_t)g.n + 2 * (size_t)nops + 16;
+  const size_t i32 = (size_t)g.n + 3 * (size_t)nops + 16;
   const size_t sh = (size_t)g.n * g.FF;
   return 8 * d + (size_t)tsize * t + 4 * i32 + 2 * sh;
 }
@@ -270,8 +272,10 @@ inline int elo_win_chunk(const CnnGeom& g, int cm, int nops, int threads, int ts
 
 // T = double: the FP64 form; T = float: the mixed-precision form (FP32
 // activations and window recomputation, log psi, the per-flip changes and
-// O_loc accumulated in FP64).
-template <int CM, typename T>
+// O_loc accumulated in FP64).  F3: 3x3 filters and at most three layers, so
+// window shapes are compile-time constants (index arithmetic by constants,
+// unrolled kernel offsets); the arithmetic of every flip is the same.
+template <int CM, typename T, bool F3>
 __global__ void __launch_bounds__(256) eloc_cnn_win(CnnGeom g, const T* __restrict__ Wpk, CnnPack pk, OpDev op,
                                                     const int32_t* __restrict__ cfg,
                                                     const double* __restrict__ lps, int64_t n, int fch,
@@ -299,7 +303,8 @@ __global__ void __launch_bounds__(256) eloc_cnn_win(CnnGeom g, const T* __restri
   int32_t* cs = (int32_t*)wp;
   int32_t* code = cs + g.n;
   int32_t* flips = code + nops;
-  int32_t* wcnt = flips + nops;
+  int32_t* finfo = flips + nops;  // per flip: (row << 16) | (column << 1) | new value
+  int32_t* wcnt = finfo + nops;
   short* nbr = (short*)(wcnt + 16);
   const cst_t<T>* Wp = (const cst_t<T>*)Wpk;
   for (int t = tid; t < g.n * g.FF; t += nt) {
@@ -373,25 +378,43 @@ __global__ void __launch_bounds__(256) eloc_cnn_win(CnnGeom g, const T* __restri
       nflip += tot;
       __syncthreads();
     }
+    for (int i = tid; i < nflip; i += nt) {
+      const int cv = code[flips[i]];
+      const int site = cv >> 1, si = site / g.Ly;
+      finfo[i] = (si << 16) | ((site - si * g.Ly) << 1) | (cv & 1);
+    }
+    __syncthreads();
     for (int f0 = 0; f0 < nflip; f0 += fch) {
       const int nf = min(fch, nflip - f0);
-      for (int l = 0; l < g.nl; ++l) {
-        const int ey = win_ey(g, l), wsz = win_ex(g, l) * ey;
-        const int pex = win_ex(g, l - 1), pey = win_ey(g, l - 1);
+      // one layer of the chunk; LC >= 0: layer LC of a 3x3 network (shapes known)
+      auto layer = [&](auto LC, int lrt) {
+        constexpr int lc = decltype(LC)::value;
+        const int l = lc >= 0 ? lc : lrt;
+        const int ey = lc >= 0 ? 1 + 2 * (lc + 1) : win_ey(g, l);
+        const int wsz = lc >= 0 ? ey * ey : win_ex(g, l) * ey;
+        const int pex = lc >= 0 ? 1 + 2 * lc : win_ex(g, l - 1), pey = lc >= 0 ? pex : win_ey(g, l - 1);
         const cst_t<T>* K = Wp + pk.koff[l];
         const cst_t<T>* B = Wp + pk.boff[l];
         for (int t = tid; t < nf * wsz; t += nt) {
           const int f = t / wsz, j = t - f * wsz;
-          const int cv = code[flips[f0 + f]];
-          const int site = cv >> 1;
-          const int si = site / g.Ly, sj = site - si * g.Ly;
+          const int fi = finfo[f0 + f];
+          const int si = fi >> 16, sj = (fi >> 1) & 0x7fff;
+          const int site = si * g.Ly + sj;
           const int a = j / ey, b = j - a * ey;
           T z[CM];
-          if (l == 0)
-            win_preact0<CM, T>(g, K, B, x0, (T)(2 * (cv & 1) - 1), site, si, sj, a, b, z);
-          else
-            win_preact<CM, T>(g, K, B, A + (size_t)(l - 1) * lsz, W[l - 1] + (size_t)f * CM * pex * pey, pex, pey,
-                              si, sj, a, b, z);
+          if constexpr (lc >= 0) {
+            if constexpr (lc == 0)
+              win_preact03<CM, T>(g, K, B, x0, (T)(2 * (fi & 1) - 1), site, si, sj, a, b, z);
+            else
+              win_preact3<CM, lc, T>(g, K, B, A + (size_t)(l - 1) * lsz, W[l - 1] + (size_t)f * CM * pex * pey, si,
+                                     sj, a, b, z);
+          } else {
+            if (l == 0)
+              win_preact0<CM, T>(g, K, B, x0, (T)(2 * (fi & 1) - 1), site, si, sj, a, b, z);
+            else
+              win_preact<CM, T>(g, K, B, A + (size_t)(l - 1) * lsz, W[l - 1] + (size_t)f * CM * pex * pey, pex,
+                                pey, si, sj, a, b, z);
+          }
           if (l < L1) {
             lds_w<T>* Wo = (lds_w<T>*)(W[l] + (size_t)f * CM * wsz + j);
 #pragma unroll
@@ -405,6 +428,13 @@ __global__ void __launch_bounds__(256) eloc_cnn_win(CnnGeom g, const T* __restri
           }
         }
         __syncthreads();
+      };
+      if constexpr (F3) {
+        layer(std::integral_constant<int, 0>(), 0);
+        if (g.nl > 1) layer(std::integral_constant<int, 1>(), 1);
+        if (g.nl > 2) layer(std::integral_constant<int, 2>(), 2);
+      } else {
+        for (int l = 0; l < g.nl; ++l) layer(std::integral_constant<int, -1>(), l);
       }
       for (int f = wv; f < nf; f += nw) {
         double d = 0.0;
@@ -518,16 +548,24 @@ int jv_eloc(const jv_net* net, const double* params, const jv_optable* op, const
     const size_t lds = elo_win_bytes(cg, cm, op->num_ops, fch, ts);
     if (lds <= 160 * 1024) {
       const CnnPack pk = make_cnn_pack(cg, cm, 1);
+      const bool f3 = cg.Fx == 3 && cg.Fy == 3 && cg.nl <= 3;
       ScopedTimer tm("eloc", st);
       const int64_t blocks = std::min<int64_t>(n, 1 << 20);
 #define JV_ELO(CMv, Tv, slot)                                                                            \
   case CMv: {                                                                                            \
     const Tv* wpk = pack_cnn_weights<Tv>(cg, pk, params, st, slot);                                      \
     if (!wpk) return JV_E_NOMEM;                                                                         \
-    rc = set_lds_limit((const void*)eloc_cnn_win<CMv, Tv>, lds);                                         \
-    if (rc) return rc;                                                                                   \
-    hipLaunchKernelGGL((eloc_cnn_win<CMv, Tv>), dim3((unsigned)blocks), dim3(256), lds, st, cg, wpk, pk, d, \
-                       configs, logpsi_s, n, fch, out);                                                  \
+    if (f3) {                                                                                            \
+      rc = set_lds_limit((const void*)eloc_cnn_win<CMv, Tv, true>, lds);                                 \
+      if (rc) return rc;                                                                                 \
+      hipLaunchKernelGGL((eloc_cnn_win<CMv, Tv, true>), dim3((unsigned)blocks), dim3(256), lds, st, cg, wpk, pk, \
+                         d, configs, logpsi_s, n, fch, out);                                             \
+    } else {                                                                                             \
+      rc = set_lds_limit((const void*)eloc_cnn_win<CMv, Tv, false>, lds);                                \
+      if (rc) return rc;                                                                                 \
+      hipLaunchKernelGGL((eloc_cnn_win<CMv, Tv, false>), dim3((unsigned)blocks), dim3(256), lds, st, cg, wpk, \
+                         pk, d, configs, logpsi_s, n, fch, out);                                         \
+    }                                                                                                    \
     break;                                                                                               \
   }
       if (mixed) {
