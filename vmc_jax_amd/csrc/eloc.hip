@@ -550,6 +550,8 @@ int jv_eloc(const jv_net* net, const double* params, const jv_optable* op, const
       const CnnPack pk = make_cnn_pack(cg, cm, 1);
       const bool f3 = cg.Fx == 3 && cg.Fy == 3 && cg.nl <= 3;
       ScopedTimer tm("eloc", st);
+      // one workgroup per sample (grid-stride beyond 2^20; capping the grid to
+      // leave CUs to the solve beside it was measured slower)
       const int64_t blocks = std::min<int64_t>(n, 1 << 20);
 #define JV_ELO(CMv, Tv, slot)                                                                            \
   case CMv: {                                                                                            \

@@ -158,6 +158,180 @@ __global__ void __launch_bounds__(256) cnn_fwd_grad(CnnGeom g, const double* __r
   }
 }
 
+// ------------------------------------------------------------------ CNN kernel, planar and register-blocked
+// The same forward pass and gradients for networks whose channels fit CM
+// (padded with zero channels) and whose planes fit LDS: activations planar
+// (A_l: CM planes of n values), tanh(z) and then the back-propagated deltas
+// interleaved per position (D_l[p][c]), weights packed as for the window
+// kernels and read as wave-uniform scalar operands.  Per thread:
+//  * forward: one position, all CM output channels (win_forward's order);
+//  * kernel gradient dW[k][ci][co] = sum_p delta[p][co] x[ci][nbr(p, k)]: one
+//    (k, ci) pair and one slice of the positions, all CM output channels
+//    (one gather of x and one contiguous read of delta[p] per term), slices
+//    summed in slice order;
+//  * delta_{l-1}[q][ci] = tanh(z)[q][ci] sum_{k, co} delta_l[inv(q, k)][co]
+//    K[k][ci][co]: one position, all CM input channels, in the reference
+//    kernel's order (k, then co) for every ci.
+// Slots of kernel-gradient partial sums (items of CM values): one per thread,
+// or one per (k, ci) pair when the pairs outnumber the threads.
+__host__ __device__ inline int cnn_p_items(const CnnGeom& g, int cm, int nt) { return nt > g.FF * cm ? nt : g.FF * cm; }
+__host__ __device__ inline size_t cnn_p_lds(const CnnGeom& g, int cm, int nt, int tsize) {
+  const size_t t = (size_t)g.n + 2 * (size_t)g.nl * cm * g.n + (size_t)cnn_p_items(g, cm, nt) * cm;
+  return 8 * 8 + (size_t)tsize * t + 2 * 2 * (size_t)g.n * g.FF;
+}
+
+template <int CM, typename T>
+__global__ void __launch_bounds__(256) cnn_fwd_grad_p(CnnGeom g, const T* __restrict__ Wpk, CnnPack pk,
+                                                      const int32_t* __restrict__ cfg, int64_t n, int layout,
+                                                      T* __restrict__ grad, double* __restrict__ logpsi) {
+  extern __shared__ double lds[];
+  const bool want_grad = grad != nullptr;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int L1 = g.nl - 1, nn = g.n;
+  double* red = lds;
+  T* x0 = (T*)(red + 8);
+  T* A = x0 + nn;                          // A + l * CM * n: planes [c][p]
+  T* D = A + (size_t)g.nl * CM * nn;       // D + l * n * CM: [p][c]
+  T* part = D + (size_t)g.nl * CM * nn;    // [nt][CM]
+  short* nbr = (short*)(part + (size_t)cnn_p_items(g, CM, nt) * CM);
+  short* inv = nbr + nn * g.FF;
+  const cst_t<T>* Wp = (const cst_t<T>*)Wpk;
+  cnn_build_tables(g, nbr, inv, tid, nt);
+  __syncthreads();
+  for (int64_t s = blockIdx.x; s < n; s += gridDim.x) {
+    const int32_t* c = cfg + s * nn;
+    for (int i = tid; i < nn; i += nt) x0[i] = (T)(2 * c[i] - 1);
+    __syncthreads();
+    for (int l = 0; l < g.nl; ++l) {
+      const cst_t<T>* K = Wp + pk.koff[l];
+      const cst_t<T>* B = Wp + pk.boff[l];
+      lds_t<T>* X = (lds_t<T>*)(l == 0 ? x0 : A + (size_t)(l - 1) * CM * nn);
+      T* Al = A + (size_t)l * CM * nn;
+      T* Dl = D + (size_t)l * CM * nn;
+      for (int p = tid; p < nn; p += nt) {
+        T z[CM];
+#pragma unroll
+        for (int co = 0; co < CM; ++co) z[co] = B[co];
+        for (int k = 0; k < g.FF; ++k) {
+          const int q = nbr[p * g.FF + k];
+          if (l == 0) {
+            const T xv = X[q];
+            const cst_t<T>* w = K + k * CM;
+#pragma unroll
+            for (int co = 0; co < CM; ++co) z[co] = fma(xv, w[co], z[co]);
+          } else {
+            const cst_t<T>* w = K + k * CM * CM;
+#pragma unroll
+            for (int ci = 0; ci < CM; ++ci) {
+              const T xv = X[ci * nn + q];
+#pragma unroll
+              for (int co = 0; co < CM; ++co) z[co] = fma(xv, w[ci * CM + co], z[co]);
+            }
+          }
+        }
+#pragma unroll
+        for (int co = 0; co < CM; ++co) Al[co * nn + p] = log_cosh(z[co]);
+        if (want_grad) {
+#pragma unroll
+          for (int co = 0; co < CM; ++co) Dl[p * CM + co] = act_tanh(z[co]);
+        }
+      }
+      __syncthreads();
+    }
+    double pl = 0.0;
+    for (int t = tid; t < g.C[L1] * nn; t += nt) pl += (double)A[(size_t)L1 * CM * nn + t];
+    const double lp = block_sum(pl, red) * g.inv_nrm;
+    if (logpsi && tid == 0) {
+      logpsi[2 * s] = lp;
+      logpsi[2 * s + 1] = 0.0;
+    }
+    if (!want_grad) continue;
+    T* row = grad + s * (int64_t)g.P * (layout == 0 ? 1 : 2);
+    auto put = [&](int64_t col, T v) {
+      if (layout == 0) row[col] = v;
+      else {
+        row[2 * col] = v;
+        row[2 * col + 1] = (T)0;
+      }
+    };
+    {
+      T* Dl = D + (size_t)L1 * CM * nn;
+      for (int t = tid; t < nn * CM; t += nt) Dl[t] *= (T)g.inv_nrm;
+    }
+    __syncthreads();
+    for (int l = L1; l >= 0; --l) {
+      const int cinr = g.Cin[l], coutr = g.C[l];
+      lds_t<T>* X = (lds_t<T>*)(l == 0 ? x0 : A + (size_t)(l - 1) * CM * nn);
+      lds_t<T>* dl = (lds_t<T>*)(D + (size_t)l * CM * nn);
+      const int pairs = g.FF * cinr;
+      const int ps = max(1, min(nn, nt / pairs));  // position slices
+      // kernel-gradient partial sums: item (slice, pair), all CM output channels
+      for (int it = tid; it < pairs * ps; it += nt) {
+        const int sl = it / pairs, pr = it - sl * pairs;
+        const int k = pr / cinr, ci = pr - k * cinr;
+        const int p0 = (int)((int64_t)sl * nn / ps), p1 = (int)((int64_t)(sl + 1) * nn / ps);
+        T acc[CM];
+#pragma unroll
+        for (int co = 0; co < CM; ++co) acc[co] = (T)0;
+        for (int p = p0; p < p1; ++p) {
+          const T xv = X[ci * nn + nbr[p * g.FF + k]];
+#pragma unroll
+          for (int co = 0; co < CM; ++co) acc[co] = fma(dl[p * CM + co], xv, acc[co]);
+        }
+#pragma unroll
+        for (int co = 0; co < CM; ++co) part[it * CM + co] = acc[co];
+      }
+      // deltas of the layer below (every position, all CM input channels)
+      if (l > 0) {
+        const cst_t<T>* K = Wp + pk.koff[l];
+        T* dp = D + (size_t)(l - 1) * CM * nn;
+        for (int q = tid; q < nn; q += nt) {
+          T acc[CM];
+#pragma unroll
+          for (int ci = 0; ci < CM; ++ci) acc[ci] = (T)0;
+          for (int k = 0; k < g.FF; ++k) {
+            lds_t<T>* d = dl + (int)inv[q * g.FF + k] * CM;
+            const cst_t<T>* w = K + k * CM * CM;
+#pragma unroll
+            for (int co = 0; co < CM; ++co) {
+              const T dv = d[co];
+#pragma unroll
+              for (int ci = 0; ci < CM; ++ci) acc[ci] = fma(dv, w[ci * CM + co], acc[ci]);
+            }
+          }
+          // written after every thread has read delta_l of this layer: dp is
+          // another layer's buffer, read by no thread in this phase
+#pragma unroll
+          for (int ci = 0; ci < CM; ++ci) dp[q * CM + ci] *= acc[ci];
+        }
+      }
+      __syncthreads();
+      // sum the slices in slice order; bias gradients over all positions
+      for (int t = tid; t < pairs * coutr; t += nt) {
+        const int pr = t / coutr, co = t - pr * coutr;
+        T v = part[pr * CM + co];
+        for (int sl = 1; sl < ps; ++sl) v += part[(sl * pairs + pr) * CM + co];
+        put(g.koff[l] + t, v);
+      }
+      if (g.boff[l] >= 0) {
+        for (int co = tid; co < coutr; co += nt) {
+          T acc = 0;
+          for (int p = 0; p < nn; ++p) acc += dl[p * CM + co];
+          put(g.boff[l] + co, acc);
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// Launches cnn_fwd_grad_p when the padded channel count has an instantiation
+// and its LDS fits (*done = true); *done = false when it does not apply.
+// Returns the status (packed weights in workspace slot `slot`).
+template <typename T>
+int cnn_p_launch(const CnnGeom& g, const double* params, const int32_t* configs, int64_t n, int layout, T* grad,
+                 double* logpsi, int64_t blocks, hipStream_t st, int slot, bool* done);
+
 // ------------------------------------------------------------------ packed CNN weights
 template <typename T>
 __global__ void cnn_pack(CnnGeom g, CnnPack pk, const double* __restrict__ Pg, T* __restrict__ out) {
@@ -213,6 +387,36 @@ int cnn_cmax(const CnnGeom& g) {
   if (m <= 32) return 32;
   return 0;
 }
+
+template <typename T>
+int cnn_p_launch(const CnnGeom& g, const double* params, const int32_t* configs, int64_t n, int layout, T* grad,
+                 double* logpsi, int64_t blocks, hipStream_t st, int slot, bool* done) {
+  *done = false;
+  const int cm = cnn_cmax(g);
+  if (cm == 0) return JV_OK;
+  const size_t lds = cnn_p_lds(g, cm, 256, (int)sizeof(T));
+  if (lds > 160 * 1024) return JV_OK;
+  const CnnPack pk = make_cnn_pack(g, cm, 1);
+  const T* wpk = pack_cnn_weights<T>(g, pk, params, st, slot);
+  if (!wpk) return JV_E_NOMEM;
+  int rc = JV_OK;
+#define JV_CNNP(CMv)                                                                                      \
+  if (cm == CMv) {                                                                                        \
+    rc = set_lds_limit((const void*)cnn_fwd_grad_p<CMv, T>, lds);                                         \
+    if (rc) return rc;                                                                                    \
+    hipLaunchKernelGGL((cnn_fwd_grad_p<CMv, T>), dim3((unsigned)blocks), dim3(256), lds, st, g, wpk, pk,   \
+                       configs, n, layout, grad, logpsi);                                                 \
+    JV_LAUNCH_CHECK("cnn_fwd_grad_p");                                                                    \
+    *done = true;                                                                                         \
+  }
+  JV_CNNP(4) JV_CNNP(8) JV_CNNP(16) JV_CNNP(32)
+#undef JV_CNNP
+  return rc;
+}
+template int cnn_p_launch<double>(const CnnGeom&, const double*, const int32_t*, int64_t, int, double*, double*,
+                                  int64_t, hipStream_t, int, bool*);
+template int cnn_p_launch<float>(const CnnGeom&, const double*, const int32_t*, int64_t, int, float*, double*,
+                                 int64_t, hipStream_t, int, bool*);
 
 }  // namespace jv
 
@@ -300,11 +504,14 @@ int jv_net_gradients(const jv_net* net, const double* params, const int32_t* con
     return JV_OK;
   }
   if (make_cnn_geom(net, cg)) {
+    int64_t blocks = std::min<int64_t>(n, 8192);
+    ScopedTimer tm(grad ? "net_grad" : "net_forward", st);
+    bool done = false;
+    int prc = cnn_p_launch<double>(cg, params, configs, n, layout, grad, logpsi, blocks, st, 48, &done);
+    if (prc || done) return prc;
     size_t lds = cnn_fwd_lds(cg, grad != nullptr);
     int rc = set_lds_limit((const void*)cnn_fwd_grad<double>, lds);
     if (rc) return rc;
-    int64_t blocks = std::min<int64_t>(n, 8192);
-    ScopedTimer tm(grad ? "net_grad" : "net_forward", st);
     hipLaunchKernelGGL(cnn_fwd_grad<double>, dim3((unsigned)blocks), dim3(256), lds, st, cg, params, configs, n,
                        layout, grad, logpsi);
     JV_LAUNCH_CHECK("cnn_fwd_grad");
@@ -320,11 +527,14 @@ int jv_net_gradients_f32(const jv_net* net, const double* params, const int32_t*
   hipStream_t st = (hipStream_t)stream;
   CnnGeom cg;
   if (!make_cnn_geom(net, cg)) return fail(JV_E_ARG, "jv_net_gradients_f32: only the CNN has an FP32 form");
+  const int64_t blocks = std::min<int64_t>(n, 8192);
+  ScopedTimer tm("net_grad", st);
+  bool done = false;
+  int prc = cnn_p_launch<float>(cg, params, configs, n, 0, grad, logpsi, blocks, st, 49, &done);
+  if (prc || done) return prc;
   const size_t lds = cnn_fwd_lds(cg, true, 4);
   int rc = set_lds_limit((const void*)cnn_fwd_grad<float>, lds);
   if (rc) return rc;
-  const int64_t blocks = std::min<int64_t>(n, 8192);
-  ScopedTimer tm("net_grad", st);
   hipLaunchKernelGGL(cnn_fwd_grad<float>, dim3((unsigned)blocks), dim3(256), lds, st, cg, params, configs, n, 0, grad,
                      logpsi);
   JV_LAUNCH_CHECK("cnn_fwd_grad");
