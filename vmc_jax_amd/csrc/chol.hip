@@ -40,6 +40,7 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 // their later columns.
 __global__ void __launch_bounds__(256) chol_panel(double* A0, double* A1, int p, int k0, int* info,
                                                   double* __restrict__ dscr) {
+  chain_priority();
   double* A = blockIdx.y ? A1 : A0;
   __shared__ double Ld[CB][CB + 1];
   const int tid = threadIdx.x;
@@ -148,6 +149,7 @@ __global__ void __launch_bounds__(256) chol_panel(double* A0, double* A1, int p,
 // C[4 r + l / 16][l % 16]).
 __global__ void __launch_bounds__(256) chol_update(double* A0, double* A1, int p, int k0,
                                                    const double* __restrict__ dscr) {
+  chain_priority();
   double* A = blockIdx.y ? A1 : A0;
   if (blockIdx.x == 0) {  // the factored diagonal block of panel k0 (a full CB x CB block here)
     const double* src = dscr + (size_t)blockIdx.y * CB * CB;
@@ -223,6 +225,7 @@ __global__ void inv_identity(double* __restrict__ W, int64_t p) {
 }
 
 __global__ void __launch_bounds__(256) inv_row(const double* __restrict__ L, int p, int i0, double* W) {
+  chain_priority();
   __shared__ double Ls[CB][CB + 1];
   const int tid = threadIdx.x, lane = tid % 64;
   const int c0 = CB * blockIdx.x;  // tile column block j <= i
@@ -270,6 +273,7 @@ __global__ void __launch_bounds__(256) inv_row(const double* __restrict__ L, int
 
 // tiles (k, j): k over the block rows below i0 (blockIdx.x / (i + 1)), j <= i
 __global__ void __launch_bounds__(256) inv_update(const double* __restrict__ L, int p, int i0, double* W) {
+  chain_priority();
   const int nj = i0 / CB + 1;
   const int kt = blockIdx.x / nj, jt = blockIdx.x % nj;
   const int r0 = i0 + CB * (kt + 1), c0 = CB * jt;

@@ -187,6 +187,13 @@ __device__ inline void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
 }
 
 // Site in [0, n) and a 53-bit uniform in [0, 1) for (step, chain, call, stream).
+// Raises the issuing wave's arbitration priority.  Kernels on the solve's
+// dependency chain (centring, Gram, Cholesky factor and inverse, O_k) run
+// beside the compute-bound local-energy kernel on another stream; with the
+// higher priority their few latency-bound waves are issued first when both
+// are ready, so the chain is not slowed by the waves it shares CUs with.
+__device__ inline void chain_priority() { __builtin_amdgcn_s_setprio(2); }
+
 __device__ inline void mc_draw(uint64_t seed, uint32_t step, uint32_t chain, uint32_t call,
                                uint32_t stream, int n, int& site, double& u) {
   uint32_t c[4] = {step, chain, call, stream};

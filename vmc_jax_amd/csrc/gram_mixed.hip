@@ -109,6 +109,7 @@ constexpr int G6ROW = GT + 2;
 template <bool VEC>
 __global__ void __launch_bounds__(256) gram64_tn(const double* __restrict__ X, int64_t k_total, int64_t q,
                                                  int64_t kchunk, int ntile_side, double* __restrict__ part) {
+  chain_priority();
   __shared__ double As[G6K][G6ROW];
   __shared__ double Bs[G6K][G6ROW];
   int tm, tn, tile, slice;
@@ -194,6 +195,7 @@ __global__ void __launch_bounds__(256) gram64_tn(const double* __restrict__ X, i
 template <typename T>
 __global__ void __launch_bounds__(256) gram3_split(const T* __restrict__ X, int64_t k_total, int64_t q, int64_t qp,
                                                    int64_t nkb, __bf16* __restrict__ Y) {
+  chain_priority();
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nkb * qp * (GK / 8)) return;
   const int64_t c = t % qp, rest = t / qp;
@@ -220,6 +222,7 @@ __global__ void __launch_bounds__(256) gram3_split(const T* __restrict__ X, int6
 // loads per thread and stored to the LDS planes without conversion.
 __global__ void __launch_bounds__(256) gram3_pre(const __bf16* __restrict__ Y, int64_t nkb, int64_t qp, int64_t kbchunk,
                                                  int ntile_side, float* __restrict__ part) {
+  chain_priority();
   extern __shared__ __align__(16) unsigned char gram_lds[];
   __bf16* Ap = (__bf16*)gram_lds;          // [2][GT][GROW]: hi then lo
   __bf16* Bp = Ap + 2 * GPLANE;
@@ -306,6 +309,7 @@ __global__ void __launch_bounds__(256) gram3_pre(const __bf16* __restrict__ Y, i
 template <typename TP>
 __global__ void __launch_bounds__(256) gram3_reduce(const TP* __restrict__ part, int splits, int ntile_side,
                                                     int64_t p, int cpx, double* __restrict__ S) {
+  chain_priority();
   __shared__ double M[32][33];
   int tm, tn;
   upper_tile(blockIdx.x / 16, ntile_side, tm, tn);

@@ -39,6 +39,7 @@ constexpr int CHUNK_ROWS = 256;
 template <typename TIn>
 __global__ void col_wsum_partial(const TIn* __restrict__ O, const double* __restrict__ w, int64_t n,
                                  int64_t p, int cpx, double* __restrict__ part) {
+  chain_priority();
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= p) return;
   int64_t r0 = (int64_t)blockIdx.y * CHUNK_ROWS, r1 = std::min<int64_t>(n, r0 + CHUNK_ROWS);
@@ -57,6 +58,7 @@ __global__ void col_wsum_partial(const TIn* __restrict__ O, const double* __rest
 }
 
 __global__ void col_reduce(const double* __restrict__ part, int nchunks, int64_t len, double* __restrict__ out) {
+  chain_priority();
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= len) return;
   double a = 0.0;
@@ -68,6 +70,7 @@ template <typename TIn>
 __global__ void center_apply(const TIn* __restrict__ O, const double* __restrict__ w,
                              const double* __restrict__ mean, int64_t n, int64_t p, int cpx,
                              TIn* __restrict__ D) {
+  chain_priority();
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int64_t len = p * (cpx ? 2 : 1);
   if (t >= n * len) return;

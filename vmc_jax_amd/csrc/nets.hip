@@ -184,6 +184,7 @@ template <int CM, typename T>
 __global__ void __launch_bounds__(256) cnn_fwd_grad_p(CnnGeom g, const T* __restrict__ Wpk, CnnPack pk,
                                                       const int32_t* __restrict__ cfg, int64_t n, int layout,
                                                       T* __restrict__ grad, double* __restrict__ logpsi) {
+  chain_priority();
   extern __shared__ double lds[];
   const bool want_grad = grad != nullptr;
   const int tid = threadIdx.x, nt = blockDim.x;
