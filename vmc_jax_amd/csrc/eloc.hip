@@ -237,16 +237,20 @@ __host__ __device__ inline size_t elo_win_bytes(const CnnGeom& g, int cm, int no
 // Flips per chunk: the fewest estimated VALU cycles per flip among the sizes
 // whose LDS fits two workgroups per CU (at least one flip).
 // Workgroups per CU the chunk size aims at (LDS per workgroup <= 160 KiB /
-// this): two for the FP64 kernel, four for the FP32 one (measured on the
-// 10x10 CNN: FP64 4.39 ms at two, 4.63 at three, 8.78 at four; FP32 2.97 ms
-// at two, 2.44 at four); VMC_JAX_AMD_ELOC_WGS_PER_CU overrides both.
+// this): two for both precisions.  Measured on the 10x10 CNN with the MinSR
+// solve beside the kernel (the default step): FP64 121 steps/s at two, 91 at
+// three, 75 at four; FP32 182 steps/s at two against 146 at four, although
+// the FP32 kernel alone is faster at four (2.44 against 2.97 ms): with
+// larger, fewer workgroups the solve's kernels find CU slots sooner.
+// VMC_JAX_AMD_ELOC_WGS_PER_CU overrides it.
 inline int elo_wgs_per_cu(int tsize) {
   static const int env = [] {
     const char* s = getenv("VMC_JAX_AMD_ELOC_WGS_PER_CU");
     const int k = s ? atoi(s) : 0;
     return k < 0 ? 0 : (k > 8 ? 8 : k);
   }();
-  return env > 0 ? env : (tsize == 4 ? 4 : 2);
+  (void)tsize;
+  return env > 0 ? env : 2;
 }
 
 inline int elo_win_chunk(const CnnGeom& g, int cm, int nops, int threads, int tsize = 8) {
