@@ -176,6 +176,9 @@ def test_mc_cnn_bit_exact_tfim10x10():
         assert rel(to_np(lp)[0], lpo) < 1e-12
         assert float(ms.acceptance_ratio()[0]) == mo.acceptance_ratio()
         assert 0.05 < mo.acceptance_ratio() < 0.95
+        # the sampling kernel's log psi (from its maintained activations) equals
+        # a fresh forward pass of the emitted configurations exactly
+        assert torch.equal(lp, psi(s))
 
 
 def test_minsr_overlapped_cholesky_route_tfim10x10():

@@ -145,6 +145,47 @@ __host__ __device__ inline double log_cosh(double x) {
   for (int k = 1; k < 10; ++k) w = fma(w, z, LC_ATANH[k]);
   return (a - LN2) + 2.0 * fma(s * z, w, s);
 }
+// log cosh(x) and tanh(x) from one exponential: with y = -2 min(|x|, 40) =
+// n ln2 + r and e^r = 1 + r P(r) (the polynomial of log_cosh before its last
+// step), e^y - 1 = r P(r) when n = 0 (no cancellation for small |x|) and
+// 2^n e^r - 1 otherwise; tanh |x| = -(e^y - 1) / (2 + (e^y - 1)), the
+// quotient by the hardware reciprocal with two Newton steps and a residual
+// correction (2 + (e^y - 1) lies in [1, 2]).  For |x| >= 40 tanh rounds to +-1.
+__device__ inline double log_cosh_tanh(double x, double& th) {
+  const double a = fabs(x);
+  const double y = -2.0 * fmin(a, 40.0);
+  const double n = rint(y * 1.4426950408889634);
+  double r = fma(-n, 6.93147180559945286227e-01, y);
+  r = fma(-n, 2.31904681384629955842e-17, r);
+  double p = LC_EXP[0];
+#pragma unroll
+  for (int k = 1; k < 11; ++k) p = fma(p, r, LC_EXP[k]);
+  const double em1r = r * p;             // e^r - 1
+  const double er = fma(p, r, LC_EXP[11]);  // e^r
+  const double t = ldexp(er, (int)n);
+  const double em1 = n == 0.0 ? em1r : t - 1.0;
+  // tanh
+  const double dt = 2.0 + em1;
+  double qt = __builtin_amdgcn_rcp(dt);
+  qt = fma(qt, fma(-dt, qt, 1.0), qt);
+  qt = fma(qt, fma(-dt, qt, 1.0), qt);
+  double ht = -em1 * qt;
+  ht = fma(fma(dt, -ht, -em1), qt, ht);
+  th = x < 0.0 ? -ht : ht;
+  // log cosh, as log_cosh
+  const double d = 2.0 + t;
+  double q = __builtin_amdgcn_rcp(d);
+  q = fma(q, fma(-d, q, 1.0), q);
+  q = fma(q, fma(-d, q, 1.0), q);
+  double s = t * q;
+  s = fma(fma(-d, s, t), q, s);
+  const double z = s * s;
+  double w = LC_ATANH[0];
+#pragma unroll
+  for (int k = 1; k < 10; ++k) w = fma(w, z, LC_ATANH[k]);
+  return (a - LN2) + 2.0 * fma(s * z, w, s);
+}
+
 // FP32 log cosh of the mixed-precision network kernels:
 // |x| - ln 2 + ln(1 + e^{-2|x|}) with the hardware base-2 exponential and
 // logarithm (v_exp_f32 / v_log_f32); absolute error a few 1e-8 for |x| < 10,
@@ -160,6 +201,10 @@ __device__ inline float tanh_f32(float x) {
   const float t = __builtin_amdgcn_exp2f(-2.8853900817779268f * fabsf(x));
   const float r = (1.0f - t) / (1.0f + t);
   return x < 0.0f ? -r : r;
+}
+__device__ inline float log_cosh_tanh(float x, float& th) {
+  th = tanh_f32(x);
+  return log_cosh(x);
 }
 
 // tanh = d log_cosh / dx, via exp(-2|Re x|) so it never overflows.
@@ -188,10 +233,11 @@ __device__ inline void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
 
 // Site in [0, n) and a 53-bit uniform in [0, 1) for (step, chain, call, stream).
 // Raises the issuing wave's arbitration priority.  Kernels on the solve's
-// dependency chain (centring, Gram, Cholesky factor and inverse, O_k) run
-// beside the compute-bound local-energy kernel on another stream; with the
-// higher priority their few latency-bound waves are issued first when both
-// are ready, so the chain is not slowed by the waves it shares CUs with.
+// dependency chain (centring, Gram, Cholesky factor and inverse, O_k, the
+// Householder reduction and the tridiagonal divide and conquer) run beside
+// the compute-bound local-energy kernel on another stream; with the higher
+// priority their few latency-bound waves are issued first when both are
+// ready, so the chain is not slowed by the waves it shares CUs with.
 __device__ inline void chain_priority() { __builtin_amdgcn_s_setprio(2); }
 
 __device__ inline void mc_draw(uint64_t seed, uint32_t step, uint32_t chain, uint32_t call,

@@ -94,6 +94,7 @@ __device__ inline double dc_key(double v) { return v != v ? __builtin_inf() : v;
 __global__ void __launch_bounds__(64) dc_leaves(int D, const double* dg,
                                                const double* __restrict__ eg, int n, double* __restrict__ Q,
                                                double* w) {
+  chain_priority();
   __shared__ double d[DC_LEAF], ee[DC_LEAF], Zs[DC_LEAF][DC_LEAF + 1];
   __shared__ int rk[DC_LEAF];
   const int s0 = (int)(((int64_t)blockIdx.x * n) >> D);
@@ -206,6 +207,7 @@ __device__ inline double wave_max(double v) {
 __global__ void __launch_bounds__(64) dc_prep(int lvk, const double* __restrict__ e, int n,
                                              const double* __restrict__ Q, const double* __restrict__ w, DcBufs b,
                                              double* __restrict__ zt) {
+  chain_priority();
   extern __shared__ double lds[];
   const DcNode nd = dc_node(n, lvk, blockIdx.x);
   const int s0 = nd.start, n1 = nd.n1, n2 = nd.n2, nn = nd.n1 + nd.n2, lane = threadIdx.x;
@@ -347,6 +349,7 @@ __global__ void __launch_bounds__(64) dc_prep(int lvk, const double* __restrict_
 
 // Givens rotations of eigenvector columns, in deflation order; thread = row.
 __global__ void dc_rotate(int lvk, int n, const DcBufs b, double* __restrict__ Q) {
+  chain_priority();
   const DcNode nd = dc_node(n, lvk, blockIdx.y);
   const int nn = nd.n1 + nd.n2, s0 = nd.start;
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -368,6 +371,7 @@ __global__ void dc_rotate(int lvk, int n, const DcBufs b, double* __restrict__ Q
 // One wave per root j of node y: bisection on tau in the coordinate of the
 // nearer pole, f(tau) = 1 + rho sum z_i^2 / ((d_i - d_org) - tau).
 __global__ void __launch_bounds__(256) dc_secular(int lvk, int n, DcBufs b) {
+  chain_priority();
   extern __shared__ double sec[];  // poles [k], squared couplings [k]
   const DcNode nd = dc_node(n, lvk, blockIdx.y);
   const int s0 = nd.start;
@@ -429,6 +433,7 @@ __global__ void __launch_bounds__(256) dc_secular(int lvk, int n, DcBufs b) {
 // Gu-Eisenstat weights; thread = pole i.  The roots' data (pole, origin
 // pole, tau) pass through LDS in chunks of 1024, in root order.
 __global__ void __launch_bounds__(256) dc_zhat(int lvk, int n, DcBufs b) {
+  chain_priority();
   __shared__ double sd[1024], so[1024], st[1024];
   const DcNode nd = dc_node(n, lvk, blockIdx.y);
   const int s0 = nd.start;
@@ -462,6 +467,7 @@ __global__ void __launch_bounds__(256) dc_zhat(int lvk, int n, DcBufs b) {
 __global__ void __launch_bounds__(256) dc_vectors(int lvk, int n, const DcBufs b,
                                                  const double* __restrict__ Q, double* __restrict__ U,
                                                  double* __restrict__ G) {
+  chain_priority();
   const DcNode nd = dc_node(n, lvk, blockIdx.y);
   const int s0 = nd.start, nn = nd.n1 + nd.n2;
   const int j = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
@@ -499,6 +505,7 @@ __global__ void __launch_bounds__(256) dc_vectors(int lvk, int n, const DcBufs b
 // C[4 q + l / 16][l % 16]); tiles and chunks beyond k are skipped.
 __global__ void __launch_bounds__(256) dc_merge_gemm(int lvk, int n, DcBufs b, const double* __restrict__ G,
                                                     const double* __restrict__ U, double* __restrict__ C) {
+  chain_priority();
   const DcNode nd = dc_node(n, lvk, blockIdx.y);
   const int s0 = nd.start, m = nd.n1 + nd.n2;
   const int k = b.cnt[2 * s0];
@@ -556,6 +563,7 @@ __global__ void __launch_bounds__(256) dc_merge_gemm(int lvk, int n, DcBufs b, c
 // network in LDS, padded to a power of two with +inf keys and indices >= nn.
 // Dynamic LDS: 12 bytes per padded entry.
 __global__ void __launch_bounds__(256) dc_rank(int lvk, int n, DcBufs b, double* __restrict__ w) {
+  chain_priority();
   extern __shared__ double lv[];
   const DcNode nd = dc_node(n, lvk, blockIdx.x);
   const int s0 = nd.start, nn = nd.n1 + nd.n2;
@@ -597,6 +605,7 @@ __global__ void __launch_bounds__(256) dc_rank(int lvk, int n, DcBufs b, double*
 // >= k) at position rank[t]; thread = row.
 __global__ void dc_scatter(int lvk, int n, const DcBufs b, const double* __restrict__ C,
                            const double* __restrict__ G, double* __restrict__ Q) {
+  chain_priority();
   const DcNode nd = dc_node(n, lvk, blockIdx.z);
   const int s0 = nd.start, nn = nd.n1 + nd.n2;
   const int r = blockIdx.x * blockDim.x + threadIdx.x;

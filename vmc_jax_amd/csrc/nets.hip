@@ -172,13 +172,6 @@ __global__ void __launch_bounds__(256) cnn_fwd_grad(CnnGeom g, const double* __r
 //  * delta_{l-1}[q][ci] = tanh(z)[q][ci] sum_{k, co} delta_l[inv(q, k)][co]
 //    K[k][ci][co]: one position, all CM input channels, in the reference
 //    kernel's order (k, then co) for every ci.
-// Slots of kernel-gradient partial sums (items of CM values): one per thread,
-// or one per (k, ci) pair when the pairs outnumber the threads.
-__host__ __device__ inline int cnn_p_items(const CnnGeom& g, int cm, int nt) { return nt > g.FF * cm ? nt : g.FF * cm; }
-__host__ __device__ inline size_t cnn_p_lds(const CnnGeom& g, int cm, int nt, int tsize) {
-  const size_t t = (size_t)g.n + 2 * (size_t)g.nl * cm * g.n + (size_t)cnn_p_items(g, cm, nt) * cm;
-  return 8 * 8 + (size_t)tsize * t + 2 * 2 * (size_t)g.n * g.FF;
-}
 
 template <int CM, typename T>
 __global__ void __launch_bounds__(256) cnn_fwd_grad_p(CnnGeom g, const T* __restrict__ Wpk, CnnPack pk,
@@ -230,11 +223,16 @@ __global__ void __launch_bounds__(256) cnn_fwd_grad_p(CnnGeom g, const T* __rest
             }
           }
         }
-#pragma unroll
-        for (int co = 0; co < CM; ++co) Al[co * nn + p] = log_cosh(z[co]);
         if (want_grad) {
 #pragma unroll
-          for (int co = 0; co < CM; ++co) Dl[p * CM + co] = act_tanh(z[co]);
+          for (int co = 0; co < CM; ++co) {
+            T th;
+            Al[co * nn + p] = log_cosh_tanh(z[co], th);
+            Dl[p * CM + co] = th;
+          }
+        } else {
+#pragma unroll
+          for (int co = 0; co < CM; ++co) Al[co * nn + p] = log_cosh(z[co]);
         }
       }
       __syncthreads();

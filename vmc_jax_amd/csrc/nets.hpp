@@ -268,6 +268,15 @@ __device__ inline void cnn_forward_pos(const CnnGeom& g, const double* __restric
 // of layer l that depend on s are ((si - a) mod Lx, (sj - b) mod Ly) for
 // 0 <= a < ex(l), 0 <= b < ey(l), ex(l) = 1 + (l+1)(Fx-1); window index a*ey + b.
 // Valid while ex(nl-1) <= Lx and ey(nl-1) <= Ly (no wrap-around aliasing).
+// LDS of the planar CNN forward / gradient kernel (cnn_fwd_grad_p): slots of
+// kernel-gradient partial sums (items of cm values), one per thread or one per
+// (k, ci) pair when the pairs outnumber the threads, plus the planes.
+__host__ __device__ inline int cnn_p_items(const CnnGeom& g, int cm, int nt) { return nt > g.FF * cm ? nt : g.FF * cm; }
+__host__ __device__ inline size_t cnn_p_lds(const CnnGeom& g, int cm, int nt, int tsize) {
+  const size_t t = (size_t)g.n + 2 * (size_t)g.nl * cm * g.n + (size_t)cnn_p_items(g, cm, nt) * cm;
+  return 8 * 8 + (size_t)tsize * t + 2 * 2 * (size_t)g.n * g.FF;
+}
+
 __device__ __host__ inline int win_ex(const CnnGeom& g, int l) { return l < 0 ? 1 : 1 + (l + 1) * (g.Fx - 1); }
 __device__ __host__ inline int win_ey(const CnnGeom& g, int l) { return l < 0 ? 1 : 1 + (l + 1) * (g.Fy - 1); }
 __device__ __host__ inline bool win_ok(const CnnGeom& g) {

@@ -369,11 +369,16 @@ __host__ __device__ inline size_t mc_win3_bytes(const CnnGeom& g, int cm, int cs
 // Wrap a row (column) index shifted by at most one period.
 __device__ inline int wrap1(int q, int L) { return q < 0 ? q + L : (q >= L ? q - L : q); }
 
+// logpsi (optional): log psi of every emitted configuration, summed from the
+// maintained activations (which equal a fresh forward pass of the current
+// configuration exactly: every window is recomputed with the forward's
+// operations in the forward's order) in the order of the forward kernels'
+// own sum, so the values equal jv_net_forward's.
 template <int CM, int G, int CS, typename T>
 __global__ void __launch_bounds__(64 * G) mc_cnn_win3(CnnGeom g, const T* __restrict__ Wpk, CnnPack pk,
                                                       jv_mc_args a, int32_t* __restrict__ walkers,
                                                       int32_t* __restrict__ configs,
-                                                      int64_t* __restrict__ counters) {
+                                                      int64_t* __restrict__ counters, double* __restrict__ logpsi) {
   constexpr int CG = CM / G;
   extern __shared__ double lds[];
   const int tid = threadIdx.x, nt = blockDim.x, lane = tid % WAVE;
@@ -531,6 +536,18 @@ __global__ void __launch_bounds__(64 * G) mc_cnn_win3(CnnGeom g, const T* __rest
       const int64_t t = k / a.sweep_steps - 1;
       int32_t* outc = configs + (t * a.num_chains + chain) * g.n;
       for (int i = tid; i < g.n; i += nt) outc[i] = s[i];
+      if (logpsi) {
+        double pl = 0.0;
+        for (int e = tid; e < g.C[L1] * g.n; e += nt) {
+          const int c = e / g.n, p = e - c * g.n;
+          pl += (double)U[((size_t)L1 * CM + c) * CS + p];
+        }
+        const double lp = block_sum(pl, red) * g.inv_nrm;
+        if (tid == 0) {
+          logpsi[2 * (t * a.num_chains + chain)] = lp;
+          logpsi[2 * (t * a.num_chains + chain) + 1] = 0.0;
+        }
+      }
     }
   }
   for (int i = tid; i < g.n; i += nt) walkers[(int64_t)chain * g.n + i] = s[i];
@@ -595,6 +612,7 @@ int jv_mc_sample(const jv_net* net, const double* params, const jv_mc_args* args
   JV_LAUNCH_CHECK("zero_i64");
   RbmGeom rg;
   CnnGeom cg;
+  bool fused_lp = false;  // log psi of the emitted samples written by the sampling kernel
   if (make_rbm_geom(net, rg)) {
     const int nw = 4;
     size_t lds = nw * 8 * mc_rbm_lds_doubles(rg);
@@ -608,11 +626,16 @@ int jv_mc_sample(const jv_net* net, const double* params, const jv_mc_args* args
              mc_win_bytes(cg, cnn_cmax(cg), mc_win_cs(cg), net->precision == JV_PREC_MIXED ? 4 : 8) <= 160 * 1024) {
     const bool mixed = net->precision == JV_PREC_MIXED;
     const int cm = cnn_cmax(cg), cs = mc_win_cs(cg);
-    // waves per chain: each owns cm / groups output channels (one channel per
-    // wave, eight waves, measured slower on the 10x10 CNN: 3.69 against 2.99 ms)
+    // waves per chain: each owns cm / groups output channels
+    // (one channel per wave, eight waves per chain, measured slower with both
+    // kernels: 3.69 against 2.99 ms and 3.50 against 1.93 ms per step)
     const int groups = cm >= 8 ? 4 : 2;
     // 3x3 filters with windows of at most 64 elements: input codes in registers
     const bool w3 = mc_win3_ok(cg);
+    // the emitted samples' log psi from the chains' own activations: equal to
+    // the FP64 forward kernel's when the workgroup sums with its 256 threads
+    // (the mixed form keeps the FP64 forward for them)
+    fused_lp = w3 && !mixed && groups == 4 && logpsi != nullptr && cnn_p_lds(cg, cm, 256, 8) <= 160 * 1024;
     const size_t lds = w3 ? mc_win3_bytes(cg, cm, cs, mixed ? 4 : 8) : mc_win_bytes(cg, cm, cs, mixed ? 4 : 8);
     const CnnPack pk = make_cnn_pack(cg, cm, groups);
     ScopedTimer tm("mc_sweep", st);
@@ -622,13 +645,14 @@ int jv_mc_sample(const jv_net* net, const double* params, const jv_mc_args* args
     int rc = JV_OK;
     const void* fn = nullptr;
 #define JV_MCW(CMv, Gv, CSv, Tv)                                                                        \
-  if (cm == CMv && cs == CSv && (sizeof(Tv) == 4) == mixed) {                                          \
+  if (cm == CMv && groups == Gv && cs == CSv && (sizeof(Tv) == 4) == mixed) {                          \
     if (w3) {                                                                                           \
       fn = (const void*)mc_cnn_win3<CMv, Gv, CSv, Tv>;                                                  \
       rc = set_lds_limit(fn, lds);                                                                      \
       if (rc) return rc;                                                                                \
       hipLaunchKernelGGL((mc_cnn_win3<CMv, Gv, CSv, Tv>), dim3((unsigned)a.num_chains), dim3(64 * Gv), lds, \
-                         st, cg, (const Tv*)wpk, pk, a, walkers, configs, counters);                    \
+                         st, cg, (const Tv*)wpk, pk, a, walkers, configs, counters,                     \
+                         fused_lp ? logpsi : nullptr);                                                 \
     } else {                                                                                            \
       fn = (const void*)mc_cnn_win<CMv, Gv, CSv, Tv>;                                                   \
       rc = set_lds_limit(fn, lds);                                                                      \
@@ -662,7 +686,7 @@ int jv_mc_sample(const jv_net* net, const double* params, const jv_mc_args* args
   }
   // log psi of the emitted samples from a fresh forward pass (sampler.py:293-296)
   int64_t ns = (int64_t)a.samples_per_chain * a.num_chains;
-  if (logpsi && ns > 0) return jv_net_forward(net, params, configs, ns, logpsi, stream);
+  if (logpsi && ns > 0 && !fused_lp) return jv_net_forward(net, params, configs, ns, logpsi, stream);
   return JV_OK;
 }
 

@@ -68,6 +68,7 @@ __device__ inline double trd_block_sum(double v, double* red) {
 // Global reads are batched per thread (all loads of a batch are issued before
 // any is used): each step is latency bound, not bandwidth bound.
 __global__ void __launch_bounds__(TRD_T) trd_step(TrdArgs a, int k) {
+  chain_priority();
   extern __shared__ double sh[];
   const int n = a.n, G = a.G, tid = threadIdx.x, g = blockIdx.x;
   const bool qwg = g == G;
@@ -272,6 +273,7 @@ __device__ inline cd cmulc(cd a, cd b) { return {a.re * b.re + a.im * b.im, a.im
 // w_{k-1} = tau_{k-1} y_{k-1} - alpha' v_{k-1} from global memory.
 template <int R, bool VG>
 __global__ void __launch_bounds__(TRD_T) trd_step_c(TrdcArgs a, int k) {
+  chain_priority();
   extern __shared__ double sh[];
   const int n = a.n, G = a.G, tid = threadIdx.x, g = blockIdx.x;
   int j0 = g + ((k + 1 - g + G - 1) / G) * G;
@@ -495,6 +497,7 @@ constexpr int WY_B = 32;
 // Gram Y = V^T V accumulated over 256-row chunks staged in LDS.
 __global__ void __launch_bounds__(256) trd_wy_t(const double* __restrict__ V, const double* __restrict__ tau,
                                                 int n, int nref, double* __restrict__ Tout) {
+  chain_priority();
   __shared__ double Vs[WY_B][257];
   __shared__ double Y[WY_B][WY_B + 1];
   __shared__ double T[WY_B][WY_B + 1];
@@ -569,6 +572,7 @@ __global__ void __launch_bounds__(256) trd_wy_t(const double* __restrict__ V, co
 template <bool TR>
 __global__ void __launch_bounds__(TRD_T) trd_apply_wy(const double* __restrict__ V, const double* __restrict__ Tg,
                                                       int n, int nref, double* __restrict__ x) {
+  chain_priority();
   __shared__ double part[TRD_W][WY_B];
   __shared__ double wv[WY_B], tv[WY_B];
   const int tid = threadIdx.x, w = tid / WAVE, lane = tid % WAVE;
