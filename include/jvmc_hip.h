@@ -428,6 +428,23 @@ int jv_minsr_solve_dual_deferred_mixed(const float* data, const double* e_raw, c
                                        int64_t n, int64_t p, int32_t is_complex, double rcond, double diag_shift,
                                        int32_t try_cholesky, double* update, jv_solve_info* info, void* stream);
 
+/* The dual solve in two calls, so that the caller can start the local
+ * estimator after the Gram: jv_minsr_dual_gram forms S = D^dagger D (+ shift I
+ * for real data) into the caller's buffer S (p x p, complex interleaved
+ * p x 2p when is_complex) on the stream: FP64 data (f32 = 0) through the FP64
+ * MFMA Gram, FP32 data (f32 = 1, real only) through the compensated BF16 Gram
+ * of the mixed form.  jv_minsr_solve_dual_deferred_gram is then
+ * jv_minsr_solve_dual_deferred (f32 = 0) or its mixed form (f32 = 1) on that
+ * S, which it consumes.  Same update as the one-call forms.  No separate
+ * reference function (minsr.py:82-165 computes E_loc before the solve; the
+ * order does not change the result). */
+int jv_minsr_dual_gram(const void* data, int32_t f32, int64_t n, int64_t p, int32_t is_complex, double diag_shift,
+                       double* S, void* stream);
+int jv_minsr_solve_dual_deferred_gram(const void* data, int32_t f32, double* S, const double* e_raw,
+                                      const double* w, void* e_ready, int64_t n, int64_t p, int32_t is_complex,
+                                      double rcond, double diag_shift, int32_t try_cholesky, double* update,
+                                      jv_solve_info* info, void* stream);
+
 /* TDVP solve (tdvp.py:134-213).  data: centred O in jVMC's flat layout, complex
  * [n][pr]; e: centred E_loc, complex [n]; w: weights [n]; x = rhsPrefactor
  * (complex, host); make_real 0 = 'real', 1 = 'imag'.  use_snr selects the SNR

@@ -98,6 +98,32 @@ def test_mixed_eloc_and_gradients_tfim10x10():
     assert np.linalg.norm(g - Go) <= 1e-5 * np.linalg.norm(Go)
 
 
+@pytest.mark.parametrize("shape,ch", [((6, 6), (8, 8, 8)), ((12, 12), (8, 8)), ((7, 9), (4, 8, 4))])
+def test_mixed_eloc_incremental_windows(shape, ch):
+    """The mixed form's incremental window update on other lattices and
+    depths: 36 and 63 flips (a last chunk shorter than the others), two
+    layers, uneven channel counts (padded to CM = 8) and a non-square lattice;
+    per-sample |dE_loc| <= 1e-5 |E_loc| + 1e-5 against the FP64 oracle."""
+    import sys, os
+    sys.path.insert(0, os.path.dirname(__file__))
+    import vmc_jax_amd as jv
+    from oracle import nets as onets
+    from helpers import tfim_pair
+    rng = np.random.default_rng(sum(shape) + len(ch))
+    net = jv.nets.CNN(F=(3, 3), channels=ch, strides=(1, 1))
+    psi = jv.NQS(net, precision="mixed")
+    psi(torch.zeros((1, 1) + shape, dtype=torch.int32))
+    P = net.he_normal(psi.desc, rng)
+    psi.set_parameters(P)
+    opsi = onets.NQS(onets.CNN(shape, F=(3, 3), channels=ch, strides=(1, 1)), P)
+    H, Ho = tfim_pair(shape, -1.0, -3.0)
+    cs = rng.integers(0, 2, size=(96,) + shape).astype(np.int32)
+    s = torch.as_tensor(cs[None], device="cuda")
+    El = H.get_O_loc(s, psi, psi(s), 0.0).cpu().numpy().reshape(-1)
+    Eo = Ho.get_O_loc(cs, opsi, opsi(cs), 0.0)
+    assert np.all(np.abs(El - Eo) <= 1e-5 * np.abs(Eo) + 1e-5)
+
+
 def test_mixed_metropolis_runs_tfim10x10():
     """The FP32 Metropolis kernel: emitted configurations are valid spins, log psi
     of the emitted samples equals the FP64 forward, and the acceptance rate is

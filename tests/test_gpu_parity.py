@@ -7,6 +7,8 @@ stream; linear-algebra outputs (S, F, T, eigenvalues) to 1e-10 relative and
 solver updates to 1e-7 relative (the pseudo-inverse amplifies rounding by the
 condition number kept under the cutoff).
 """
+import ctypes
+
 import numpy as np
 import pytest
 import torch
@@ -185,6 +187,32 @@ def test_eloc_cnn(shape):
     e = to_np(H.get_O_loc(dev(s[None]), psi, psi(dev(s[None]))))[0]
     eo = Ho.get_O_loc(s, opsi, opsi(s))
     assert rel(e, eo) < 1e-11
+
+
+@pytest.mark.parametrize("shape,ch", [((10, 10), (8, 8, 8)), ((6, 6), (4, 4, 4)), ((7, 9), (8, 8))])
+def test_eloc_cnn_incremental_windows_fp64(shape, ch, monkeypatch):
+    """FP64 E_loc with incremental windows (VMC_JAX_AMD_ELOC_DELTA=1: stored
+    pre-activations plus the changed inputs) against window recomputation (the
+    FP64 default) and the oracle: the two orders of summation agree to a few
+    ulps of log psi."""
+    rng = np.random.default_rng(sum(shape) + 41)
+    net = jv.nets.CNN(F=(3, 3), channels=ch, strides=(1, 1))
+    psi = jv.NQS(net)
+    psi(torch.zeros((1, 1) + shape, dtype=torch.int32))
+    P = net.he_normal(psi.desc, rng)
+    psi.set_parameters(P)
+    opsi = onets.NQS(onets.CNN(shape, F=(3, 3), channels=ch, strides=(1, 1)), P)
+    H, Ho = tfim_pair(shape, -1.0, -3.0)
+    cs = rng.integers(0, 2, size=(128,) + shape).astype(np.int32)
+    s = dev(cs[None])
+    lp = psi(s)
+    monkeypatch.setenv("VMC_JAX_AMD_ELOC_DELTA", "0")
+    e0 = to_np(H.get_O_loc(s, psi, lp))[0]
+    monkeypatch.setenv("VMC_JAX_AMD_ELOC_DELTA", "1")
+    e1 = to_np(H.get_O_loc(s, psi, lp))[0]
+    monkeypatch.delenv("VMC_JAX_AMD_ELOC_DELTA")
+    assert rel(e1, e0) < 1e-13
+    assert rel(e1, Ho.get_O_loc(cs, opsi, opsi(cs))) < 1e-11
 
 
 def test_eloc_window_kernel_equals_generic_kernel():
@@ -570,16 +598,33 @@ def test_minsr_dual_deferred_equals_dual(shift, shape, ch, N, P, tol, stedc):
     G = jv.stats.SampledObs(psi.gradients(s, layout=0), p)
     eq = jv.util.MinSR(None, pinvTol=1e-10, diagonalShift=shift, solver="dual")
     u1 = to_np(eq.solve(El, G, False))
+    w = p.reshape(-1).contiguous()
+    # the Gram first (jv_minsr_dual_gram), then E_loc beside the solve
+    # (jv_minsr_solve_dual_deferred_gram), as MinSR.__call__ runs them
+    D, S = eq._dual_gram(G)
     side = torch.cuda.Stream()
     side.wait_stream(torch.cuda.current_stream())
     with torch.cuda.stream(side):
         e2 = H.get_O_loc(s, psi, lp, 0.0).reshape(-1).contiguous()
         ready = torch.cuda.Event()
         ready.record(side)
-    u2 = to_np(eq._solve_deferred(e2, ready, G, p.reshape(-1).contiguous()))
+    u2 = to_np(eq._solve_deferred(e2, ready, D, S, w))
     torch.cuda.current_stream().wait_stream(side)
     assert psi.numParameters == P and eq.last_info.rank > 0
     assert rel(u2, u1) < tol
+    # the one-call deferred form (Gram inside, E_loc already in flight)
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        e3 = H.get_O_loc(s, psi, lp, 0.0).reshape(-1).contiguous()
+        ready3 = torch.cuda.Event()
+        ready3.record(side)
+    u3 = torch.empty(P, dtype=torch.float64, device="cuda")
+    info = jv._lib.JvSolveInfo()
+    jv._lib.call("jv_minsr_solve_dual_deferred", jv._lib.ptr(D), jv._lib.ptr(e3), jv._lib.ptr(w),
+                 ready3._as_parameter_, D.shape[0], P, 0, 1e-10, float(shift), eq._cholesky_hint(), jv._lib.ptr(u3),
+                 ctypes.byref(info), jv._lib.stream_ptr())
+    torch.cuda.current_stream().wait_stream(side)
+    assert rel(to_np(u3), u1) < tol
 
 
 def test_minsr_dual_overlapped_call_matches_oracle(stedc):

@@ -30,6 +30,24 @@ constexpr int CB = 64;  // panel width
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
+// Lane S of each group of four lanes to the whole group (DPP quad_perm
+// broadcast: a VALU move, no LDS round trip).
+template <int S>
+__device__ inline double quad_bcast(double v) {
+  constexpr int ctrl = S * 0x55;  // quad_perm [S, S, S, S]
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), ctrl, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), ctrl, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ inline double quad_bcast(double v, int s) {
+  switch (s & 3) {
+    case 0: return quad_bcast<0>(v);
+    case 1: return quad_bcast<1>(v);
+    case 2: return quad_bcast<2>(v);
+    default: return quad_bcast<3>(v);
+  }
+}
+
 // panel k0: diagonal block (blockIdx.x == 0) and row blocks below it.  256
 // threads.  The diagonal block is factored right-looking by one wave, row t
 // in the registers of lane t (per column: pivot and scaling, then the
@@ -63,8 +81,11 @@ __global__ void __launch_bounds__(256) chol_panel(double* A0, double* A1, int p,
   // The diagonal block is factored by wave 0 alone, lane t holding row t in
   // registers (columns fully unrolled); per column the scaled column passes
   // through LDS.  Lanes of one wave see each other's LDS writes in program
-  // order, so the steps need no workgroup barrier.
+  // order, so the steps need no workgroup barrier.  (A version with the rows
+  // in LDS and a runtime loop over the pivots, 24 KB of code against 117 KB,
+  // took three times as long: 330k against 104k cycles per block.)
   __shared__ double colj[CB];
+  __shared__ double rdiag[CB];  // 1 / L_jj for the row solve
   if (tid < 64) {
     const int t = tid;
     int first_bad = 0;  // 1-based column of the first non-positive pivot
@@ -98,11 +119,16 @@ __global__ void __launch_bounds__(256) chol_panel(double* A0, double* A1, int p,
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       }
     }
+    double dt = 1.0;
     if (t < b) {
 #pragma unroll
       for (int c = 0; c < CB; ++c)
-        if (c <= t) Ld[t][c] = x[c];
+        if (c <= t) {
+          Ld[t][c] = x[c];
+          if (c == t) dt = x[c];
+        }
     }
+    rdiag[t] = 1.0 / dt;
     if (blockIdx.x == 0 && t == 0 && first_bad != 0 && info[blockIdx.y] == 0) info[blockIdx.y] = k0 + first_bad;
   }
   __syncthreads();
@@ -127,11 +153,12 @@ __global__ void __launch_bounds__(256) chol_panel(double* A0, double* A1, int p,
     const double v = row[q + 4 * k];  // row points at a valid row even when !live
     x[k] = live ? v : 0.0;
   }
-  const int base = (threadIdx.x % 64) & ~3;
+  // x_j = (a_j - sum_{l<j} x_l L_jl) / L_jj by the reciprocal; the owner of
+  // column j broadcasts x_j to its row's four lanes by DPP
 #pragma unroll
   for (int j = 0; j < CB; ++j) {
-    if (q == (j & 3)) x[j >> 2] /= Ld[j][j];
-    const double xj = __shfl(x[j >> 2], base + (j & 3), 64);
+    if (q == (j & 3)) x[j >> 2] *= rdiag[j];
+    const double xj = quad_bcast(x[j >> 2], j);
 #pragma unroll
     for (int k = 0; k < CB / 4; ++k)
       if (q + 4 * k > j) x[k] -= xj * Ld[q + 4 * k][j];
@@ -227,7 +254,8 @@ __global__ void inv_identity(double* __restrict__ W, int64_t p) {
 __global__ void __launch_bounds__(256) inv_row(const double* __restrict__ L, int p, int i0, double* W) {
   chain_priority();
   __shared__ double Ls[CB][CB + 1];
-  const int tid = threadIdx.x, lane = tid % 64;
+  __shared__ double rdl[CB];  // 1 / L_rr
+  const int tid = threadIdx.x;
   const int c0 = CB * blockIdx.x;  // tile column block j <= i
   const int nb = min(CB, p - i0);
   {
@@ -241,7 +269,9 @@ __global__ void __launch_bounds__(256) inv_row(const double* __restrict__ L, int
 #pragma unroll
     for (int u = 0; u < (CB * CB) / 256; ++u) {
       const int e = tid + 256 * u, r = e / CB, c = e % CB;
-      Ls[r][c] = (r < nb && c <= r) ? v[u] : (r == c ? 1.0 : 0.0);
+      const double lv = (r < nb && c <= r) ? v[u] : (r == c ? 1.0 : 0.0);
+      Ls[r][c] = lv;
+      if (r == c) rdl[r] = 1.0 / lv;
     }
   }
   __syncthreads();
@@ -255,11 +285,12 @@ __global__ void __launch_bounds__(256) inv_row(const double* __restrict__ L, int
     const double v = W[(size_t)min(gi, p - 1) * p + min(gc, p - 1)];
     x[u] = (live && gi < p) ? v : 0.0;
   }
-  const int base = lane & ~3;
+  // forward substitution by the reciprocal diagonal; the owner of row r
+  // broadcasts x_r to its column's four lanes by DPP
 #pragma unroll
   for (int r = 0; r < CB; ++r) {
-    if (q == (r & 3)) x[r >> 2] /= Ls[r][r];
-    const double xr = __shfl(x[r >> 2], base + (r & 3), 64);
+    if (q == (r & 3)) x[r >> 2] *= rdl[r];
+    const double xr = quad_bcast(x[r >> 2], r);
 #pragma unroll
     for (int u = 0; u < CB / 4; ++u)
       if (q + 4 * u > r) x[u] -= Ls[q + 4 * u][r] * xr;

@@ -77,10 +77,12 @@ int cnn_cmax(const CnnGeom& g);
 // layer l occupies A + l*lstride as CM planes of cstride doubles (positions
 // [0, n)).  All CM planes are written (padded channels hold log_cosh(0));
 // callers sum only the real channels of the last layer.
+// With Zs, the pre-activations of layers l >= 1 are kept as well, at
+// Zs + (l - 1) * lstride in the same planar layout.
 template <int CM, int G, typename T, typename Sync>
 __device__ inline void win_forward(const CnnGeom& g, const cst_t<T>* __restrict__ Wp, const CnnPack& pk,
                                    const T* x0, T* A, int lstride, int cstride, const short* nbr,
-                                   int tid, int nt, Sync sync) {
+                                   int tid, int nt, Sync sync, T* Zs = nullptr) {
   for (int l = 0; l < g.nl; ++l) {
     const cst_t<T>* K = Wp + pk.koff[l];
     const cst_t<T>* B = Wp + pk.boff[l];
@@ -111,6 +113,10 @@ __device__ inline void win_forward(const CnnGeom& g, const cst_t<T>* __restrict_
       }
 #pragma unroll
       for (int c = 0; c < CM; ++c) Al[c * cstride + p] = log_cosh(z[c]);
+      if (Zs != nullptr && l >= 1) {
+#pragma unroll
+        for (int c = 0; c < CM; ++c) Zs[(l - 1) * lstride + c * cstride + p] = z[c];
+      }
     }
     sync();
   }
@@ -208,6 +214,33 @@ __device__ inline void win_preact3(const CnnGeom& g, const cst_t<T>* __restrict_
       const T xv = x[ci * xs];
 #pragma unroll
       for (int c = 0; c < CM; ++c) z[c] = fma(xv, w[ci * CM + c], z[c]);
+    }
+  }
+}
+
+// Change of the pre-activations of window element (a, b) of layer LC >= 1 of
+// a 3x3 network after one flip: z += sum over the kernel offsets whose input
+// lies inside the layer-(LC-1) window of W[k] dX, with dX the change of that
+// layer's activations (window buffer D: element e, channel ci of flip f at
+// D[(ci * pwsz + e) * fs + f]).  Inputs outside the window did not change and
+// are skipped; the offsets taken depend on (a, b) alone, so lanes that share
+// a window element share the branch.
+template <int CM, int LC, typename T>
+__device__ inline void win_delta3(const cst_t<T>* __restrict__ K, const T* D, int fs, int f, int a, int b, T* z) {
+  constexpr int pex = 1 + 2 * LC, pwsz = pex * pex;  // the layer-(LC-1) window
+#pragma unroll 1
+  for (int k = 0; k < 9; ++k) {
+    const int ky = k / 3, kx = k - 3 * (k / 3);
+    const int aq = a - ky, bq = b - kx;
+    if (aq >= 0 && aq < pex && bq >= 0 && bq < pex) {
+      lds_t<T>* x = (lds_t<T>*)(D + (aq * pex + bq) * fs + f);
+      const cst_t<T>* w = K + k * CM * CM;
+#pragma unroll
+      for (int ci = 0; ci < CM; ++ci) {
+        const T xv = x[ci * pwsz * fs];
+#pragma unroll
+        for (int c = 0; c < CM; ++c) z[c] = fma(xv, w[ci * CM + c], z[c]);
+      }
     }
   }
 }

@@ -225,9 +225,11 @@ __device__ inline cd classify_string(const OpDev& op, int o, const int32_t* s, i
 // by one wave; the sample's O_loc is a block reduction over strings.
 // LDS: FP64 string data, reductions and per-flip partial sums; activations and
 // window buffers in the arithmetic type (tsize bytes: 8 FP64, 4 mixed FP32).
-__host__ __device__ inline size_t elo_win_bytes(const CnnGeom& g, int cm, int nops, int fch, int tsize = 8) {
+// delta (incremental windows): the pre-activations of layers >= 1 are kept
+// as well.
+__host__ __device__ inline size_t elo_win_bytes(const CnnGeom& g, int cm, int nops, int fch, int tsize, bool delta) {
   const size_t d = 4 * (size_t)nops + 16 + (size_t)fch * win_ex(g, g.nl - 1) * win_ey(g, g.nl - 1);
-  size_t t = (size_t)g.n + (size_t)g.nl * cm * g.n;
+  size_t t = (size_t)g.n + (size_t)(delta ? 2 * g.nl - 1 : g.nl) * cm * g.n;
   for (int l = 0; l < g.nl - 1; ++l) t += (size_t)fch * cm * win_ex(g, l) * win_ey(g, l);
   const size_t i32 = (size_t)g.n + 3 * (size_t)nops + 16;
   const size_t sh = (size_t)g.n * g.FF;
@@ -235,34 +237,61 @@ __host__ __device__ inline size_t elo_win_bytes(const CnnGeom& g, int cm, int no
 }
 
 // Flips per chunk: the fewest estimated VALU cycles per flip among the sizes
-// whose LDS fits two workgroups per CU (at least one flip).
-// Workgroups per CU the chunk size aims at (LDS per workgroup <= 160 KiB /
-// this): two for both precisions.  Measured on the 10x10 CNN with the MinSR
-// solve beside the kernel (the default step): FP64 121 steps/s at two, 91 at
-// three, 75 at four; FP32 182 steps/s at two against 146 at four, although
-// the FP32 kernel alone is faster at four (2.44 against 2.97 ms): with
-// larger, fewer workgroups the solve's kernels find CU slots sooner.
-// VMC_JAX_AMD_ELOC_WGS_PER_CU overrides it.
-inline int elo_wgs_per_cu(int tsize) {
-  static const int env = [] {
-    const char* s = getenv("VMC_JAX_AMD_ELOC_WGS_PER_CU");
-    const int k = s ? atoi(s) : 0;
-    return k < 0 ? 0 : (k > 8 ? 8 : k);
-  }();
-  (void)tsize;
-  return env > 0 ? env : 2;
+// whose LDS fits the workgroups per CU below next to a reserve (at least one
+// flip); the launch also asks for at least a (wgs + 1)-th of the LDS, so no
+// further workgroup of this kernel fills the reserve.  Defaults, measured on
+// the 10x10 CNN step, where the kernel runs beside MinSR's solve:
+//  - FP64: window recomputation, two workgroups per CU, no reserve (the solve
+//    then starts beside it: 128 steps/s; with incremental windows 122, with
+//    three workgroups per CU 91, with four 75);
+//  - mixed: incremental windows, two workgroups per CU and a 34 KiB reserve,
+//    room for one 33 KiB workgroup of the Cholesky kernels that run beside it
+//    (202 steps/s; three workgroups per CU 184, no reserve and three
+//    workgroups 150).
+// Overrides: VMC_JAX_AMD_ELOC_DELTA, VMC_JAX_AMD_ELOC_WGS_PER_CU and
+// VMC_JAX_AMD_ELOC_RESERVE_KB (FP64), the same names with _MIXED (mixed form);
+// the first is read at every launch.
+inline int elo_env_int(const char* name, int dflt) {
+  const char* s = getenv(name);
+  return s ? atoi(s) : dflt;
+}
+inline bool elo_delta(int tsize) {
+  return elo_env_int(tsize == 4 ? "VMC_JAX_AMD_ELOC_DELTA_MIXED" : "VMC_JAX_AMD_ELOC_DELTA", tsize == 4 ? 1 : 0) != 0;
+}
+inline int elo_wgs_per_cu(bool delta, int tsize) {
+  (void)delta;
+  static const int env = elo_env_int("VMC_JAX_AMD_ELOC_WGS_PER_CU", 0);
+  static const int env_m = elo_env_int("VMC_JAX_AMD_ELOC_WGS_PER_CU_MIXED", 0);
+  const int e = tsize == 4 ? env_m : env;
+  return e > 0 ? (e > 8 ? 8 : e) : 2;
+}
+inline size_t elo_reserve(int tsize) {
+  static const int env = elo_env_int("VMC_JAX_AMD_ELOC_RESERVE_KB", -1);
+  static const int env_m = elo_env_int("VMC_JAX_AMD_ELOC_RESERVE_KB_MIXED", -1);
+  const int e = tsize == 4 ? env_m : env;
+  return e >= 0 ? (size_t)e * 1024 : (tsize == 4 ? (size_t)34 * 1024 : 0);
 }
 
-inline int elo_win_chunk(const CnnGeom& g, int cm, int nops, int threads, int tsize = 8) {
+// With delta, a window element of layer l >= 1 takes on average
+// 9 ex(l-1)^2 / ex(l)^2 of the nine kernel offsets (the inputs inside the
+// previous layer's window).  VMC_JAX_AMD_ELOC_CHUNK (FP64) and
+// VMC_JAX_AMD_ELOC_CHUNK_MIXED force the chunk size.
+inline int elo_win_chunk(const CnnGeom& g, int cm, int nops, int threads, int tsize, bool delta) {
+  static const int forced_d = elo_env_int("VMC_JAX_AMD_ELOC_CHUNK", 0);
+  static const int forced_m = elo_env_int("VMC_JAX_AMD_ELOC_CHUNK_MIXED", 0);
+  const int forced = tsize == 4 ? forced_m : forced_d;
+  if (forced > 0 && elo_win_bytes(g, cm, nops, forced, tsize, delta) <= (size_t)160 * 1024) return forced;
   int best = 1;
   double best_cost = 1e300;
-  const size_t cap = (size_t)160 * 1024 / elo_wgs_per_cu(tsize);
+  const size_t cap = ((size_t)160 * 1024 - elo_reserve(tsize)) / elo_wgs_per_cu(delta, tsize);
   for (int f = 1; f <= 64; ++f) {
-    if (f > 1 && elo_win_bytes(g, cm, nops, f, tsize) > cap) break;
+    if (f > 1 && elo_win_bytes(g, cm, nops, f, tsize, delta) > cap) break;
     double cost = 0.0;
     for (int l = 0; l < g.nl; ++l) {
       const int wsz = win_ex(g, l) * win_ey(g, l);
-      const double item = (double)g.FF * (l == 0 ? 1 : cm) * cm + 45.0 * cm;
+      double taps = g.FF;
+      if (delta && l >= 1) taps = 9.0 * (win_ex(g, l - 1) * win_ey(g, l - 1)) / wsz;
+      const double item = taps * (l == 0 ? 1 : cm) * cm + 45.0 * cm;
       cost += (double)((f * wsz + threads - 1) / threads) * item;
     }
     cost /= f;
@@ -278,8 +307,16 @@ inline int elo_win_chunk(const CnnGeom& g, int cm, int nops, int threads, int ts
 // activations and window recomputation, log psi, the per-flip changes and
 // O_loc accumulated in FP64).  F3: 3x3 filters and at most three layers, so
 // window shapes are compile-time constants (index arithmetic by constants,
-// unrolled kernel offsets); the arithmetic of every flip is the same.
-template <int CM, typename T, bool F3>
+// unrolled kernel offsets).  Without DL each window element is recomputed
+// from its inputs in the order of a forward pass; with DL (incremental
+// windows) layers >= 1 are updated from the stored pre-activations and the
+// changes of the inputs inside the previous window (on average 3.2 and 4.6 of
+// the nine kernel offsets for layers 1 and 2 of a three-layer network), with
+// the flip index fastest so that a wave's lanes share window elements.  On
+// the 10x10 CNN the incremental form is faster alone (FP64 3.59 against 4.16
+// ms, FP32 2.04 against 2.63 ms); which form wins beside MinSR's solve is in
+// the notes on the chunk size above.
+template <int CM, typename T, bool F3, bool DL>
 __global__ void __launch_bounds__(256) eloc_cnn_win(CnnGeom g, const T* __restrict__ Wpk, CnnPack pk, OpDev op,
                                                     const int32_t* __restrict__ cfg,
                                                     const double* __restrict__ lps, int64_t n, int fch,
@@ -290,6 +327,7 @@ __global__ void __launch_bounds__(256) eloc_cnn_win(CnnGeom g, const T* __restri
   const int nops = op.num_ops, L1 = g.nl - 1;
   const int lsz = CM * g.n;  // values per stored layer
   const int wlast = win_ex(g, L1) * win_ey(g, L1);
+  constexpr bool DELTA = F3 && DL;
   double* mre = lds;
   double* mim = mre + nops;
   double* cre = mim + nops;
@@ -298,10 +336,12 @@ __global__ void __launch_bounds__(256) eloc_cnn_win(CnnGeom g, const T* __restri
   double* part = red + 16;  // per flip [wlast]
   T* x0 = (T*)(part + (size_t)fch * wlast);
   T* A = x0 + g.n;  // layer l as [c][n] at A + l*lsz
+  // DELTA: pre-activations of layers l >= 1 at Zs + (l-1)*lsz
+  T* Zs = DELTA ? A + (size_t)g.nl * lsz : nullptr;
   T* W[JV_MAX_LAYERS];
-  T* wp = A + (size_t)g.nl * lsz;
+  T* wp = A + (size_t)(DELTA ? 2 * g.nl - 1 : g.nl) * lsz;
   for (int l = 0; l < L1; ++l) {
-    W[l] = wp;  // per flip [c][wsz]
+    W[l] = wp;  // per flip [c][wsz]; DELTA: activation changes as [c][wsz][fch]
     wp += (size_t)fch * CM * win_ex(g, l) * win_ey(g, l);
   }
   int32_t* cs = (int32_t*)wp;
@@ -324,7 +364,7 @@ __global__ void __launch_bounds__(256) eloc_cnn_win(CnnGeom g, const T* __restri
       x0[i] = (T)(2 * c[i] - 1);
     }
     __syncthreads();
-    win_forward<CM, 1, T>(g, Wp, pk, x0, A, lsz, g.n, nbr, tid, nt, sync);
+    win_forward<CM, 1, T>(g, Wp, pk, x0, A, lsz, g.n, nbr, tid, nt, sync, Zs);
     double pl = 0.0;
     for (int t = tid; t < g.C[L1] * g.n; t += nt) pl += (double)A[(size_t)L1 * lsz + t];
     const double lp = block_sum(pl, red) * g.inv_nrm;
@@ -400,13 +440,38 @@ __global__ void __launch_bounds__(256) eloc_cnn_win(CnnGeom g, const T* __restri
         const cst_t<T>* K = Wp + pk.koff[l];
         const cst_t<T>* B = Wp + pk.boff[l];
         for (int t = tid; t < nf * wsz; t += nt) {
-          const int f = t / wsz, j = t - f * wsz;
+          // DELTA: the flip index runs fastest, so the lanes of a wave share
+          // few window elements (and with them the kernel offsets win_delta3 takes)
+          const int f = DELTA ? t % nf : t / wsz, j = DELTA ? t / nf : t - f * wsz;
           const int fi = finfo[f0 + f];
           const int si = fi >> 16, sj = (fi >> 1) & 0x7fff;
           const int site = si * g.Ly + sj;
           const int a = j / ey, b = j - a * ey;
           T z[CM];
-          if constexpr (lc >= 0) {
+          if constexpr (DELTA && lc >= 0) {
+            // layer 0 from the spins; layers >= 1 incrementally: the stored
+            // pre-activation plus the weighted changes of the inputs
+            if constexpr (lc == 0) {
+              win_preact03<CM, T>(g, K, B, x0, (T)(2 * (fi & 1) - 1), site, si, sj, a, b, z);
+            } else {
+              lds_t<T>* zo = (lds_t<T>*)(Zs + (size_t)(l - 1) * lsz + win_site(g, si, sj, a, b));
+#pragma unroll
+              for (int cc = 0; cc < CM; ++cc) z[cc] = zo[cc * g.n];
+              win_delta3<CM, lc, T>(K, W[l - 1], fch, f, a, b, z);
+            }
+            lds_t<T>* Ab = (lds_t<T>*)(A + (size_t)l * lsz + win_site(g, si, sj, a, b));
+            if (l < L1) {
+              lds_w<T>* Wo = (lds_w<T>*)(W[l] + (size_t)j * fch + f);
+#pragma unroll
+              for (int cc = 0; cc < CM; ++cc) Wo[(size_t)cc * wsz * fch] = log_cosh(z[cc]) - Ab[cc * g.n];
+            } else {
+              double acc = 0.0;
+#pragma unroll
+              for (int cc = 0; cc < CM; ++cc) acc += (double)(log_cosh(z[cc]) - Ab[cc * g.n]);
+              part[j * fch + f] = acc;
+            }
+            continue;
+          } else if constexpr (lc >= 0) {
             if constexpr (lc == 0)
               win_preact03<CM, T>(g, K, B, x0, (T)(2 * (fi & 1) - 1), site, si, sj, a, b, z);
             else
@@ -442,7 +507,7 @@ __global__ void __launch_bounds__(256) eloc_cnn_win(CnnGeom g, const T* __restri
       }
       for (int f = wv; f < nf; f += nw) {
         double d = 0.0;
-        for (int j = lane; j < wlast; j += WAVE) d += part[f * wlast + j];
+        for (int j = lane; j < wlast; j += WAVE) d += part[DELTA ? j * fch + f : f * wlast + j];
         d = wave_sum(d);
         if (lane == 0) {
           const int o = flips[f0 + f];
@@ -548,11 +613,15 @@ int jv_eloc(const jv_net* net, const double* params, const jv_optable* op, const
     const int cm = cnn_cmax(cg);
     const bool mixed = net->precision == JV_PREC_MIXED;
     const int ts = mixed ? 4 : 8;
-    const int fch = elo_win_chunk(cg, cm, op->num_ops, 256, ts);
-    const size_t lds = elo_win_bytes(cg, cm, op->num_ops, fch, ts);
+    const bool f3 = cg.Fx == 3 && cg.Fy == 3 && cg.nl <= 3;
+    const bool delta = f3 && elo_delta(ts);  // the kernel's DELTA
+    const int fch = elo_win_chunk(cg, cm, op->num_ops, 256, ts, delta);
+    // at least a (wgs + 1)-th of the LDS, so that no further workgroup of this
+    // kernel takes the reserve
+    const size_t lds = std::max(elo_win_bytes(cg, cm, op->num_ops, fch, ts, delta),
+                                (size_t)160 * 1024 / (elo_wgs_per_cu(delta, ts) + 1) + 512);
     if (lds <= 160 * 1024) {
       const CnnPack pk = make_cnn_pack(cg, cm, 1);
-      const bool f3 = cg.Fx == 3 && cg.Fy == 3 && cg.nl <= 3;
       ScopedTimer tm("eloc", st);
       // one workgroup per sample (grid-stride beyond 2^20; capping the grid to
       // leave CUs to the solve beside it was measured slower)
@@ -561,16 +630,21 @@ int jv_eloc(const jv_net* net, const double* params, const jv_optable* op, const
   case CMv: {                                                                                            \
     const Tv* wpk = pack_cnn_weights<Tv>(cg, pk, params, st, slot);                                      \
     if (!wpk) return JV_E_NOMEM;                                                                         \
-    if (f3) {                                                                                            \
-      rc = set_lds_limit((const void*)eloc_cnn_win<CMv, Tv, true>, lds);                                 \
+    if (delta) {                                                                                         \
+      rc = set_lds_limit((const void*)eloc_cnn_win<CMv, Tv, true, true>, lds);                           \
       if (rc) return rc;                                                                                 \
-      hipLaunchKernelGGL((eloc_cnn_win<CMv, Tv, true>), dim3((unsigned)blocks), dim3(256), lds, st, cg, wpk, pk, \
-                         d, configs, logpsi_s, n, fch, out);                                             \
-    } else {                                                                                             \
-      rc = set_lds_limit((const void*)eloc_cnn_win<CMv, Tv, false>, lds);                                \
-      if (rc) return rc;                                                                                 \
-      hipLaunchKernelGGL((eloc_cnn_win<CMv, Tv, false>), dim3((unsigned)blocks), dim3(256), lds, st, cg, wpk, \
+      hipLaunchKernelGGL((eloc_cnn_win<CMv, Tv, true, true>), dim3((unsigned)blocks), dim3(256), lds, st, cg, wpk, \
                          pk, d, configs, logpsi_s, n, fch, out);                                         \
+    } else if (f3) {                                                                                     \
+      rc = set_lds_limit((const void*)eloc_cnn_win<CMv, Tv, true, false>, lds);                          \
+      if (rc) return rc;                                                                                 \
+      hipLaunchKernelGGL((eloc_cnn_win<CMv, Tv, true, false>), dim3((unsigned)blocks), dim3(256), lds, st, cg, \
+                         wpk, pk, d, configs, logpsi_s, n, fch, out);                                    \
+    } else {                                                                                             \
+      rc = set_lds_limit((const void*)eloc_cnn_win<CMv, Tv, false, false>, lds);                         \
+      if (rc) return rc;                                                                                 \
+      hipLaunchKernelGGL((eloc_cnn_win<CMv, Tv, false, false>), dim3((unsigned)blocks), dim3(256), lds, st, cg, \
+                         wpk, pk, d, configs, logpsi_s, n, fch, out);                                    \
     }                                                                                                    \
     break;                                                                                               \
   }

@@ -1662,13 +1662,17 @@ __global__ void neg_copy(const double* __restrict__ in, int64_t len, double* __r
 // Storage and Gram arithmetic of the dual solve's data: FP64 data with the FP64
 // MFMA Gram (the reference's precision), or the mixed form's FP32 data with
 // the compensated BF16 Gram.
+// S_given: the Gram (with the shift) formed beforehand by dual_gram into a
+// caller's buffer, consumed by the solve.
 struct DualData {
   const void* ptr;
   bool f32;
+  double* S_given = nullptr;
 };
 static int minsr_dual(DualData data, const double* e, const double* e_raw, const double* w,
                       hipEvent_t e_ready, int64_t n, int64_t p, int32_t is_complex, double rcond, double diag_shift,
                       bool try_chol, double* update, hipStream_t st);
+static int dual_gram(DualData data, int64_t n, int64_t p, int cpx, double diag_shift, double* S, hipStream_t st);
 int gram_pp_bf16x3(const void* D, int f32, int64_t n, int64_t p, int cpx, double* S, hipStream_t st, int slot);
 static int minsr_dual_finish_info(int rc, jv_solve_info* info, int64_t p, double rcond, bool deferred,
                                   hipStream_t st);
@@ -1706,6 +1710,26 @@ int jv_minsr_solve_dual_deferred_mixed(const float* data, const double* e_raw, c
   hipStream_t st = (hipStream_t)stream;
   return minsr_dual_finish_info(minsr_dual({data, true}, nullptr, e_raw, w, (hipEvent_t)e_ready, n, p, 0, rcond,
                                            diag_shift, try_cholesky != 0, update, st),
+                                info, p, rcond, true, st);
+}
+
+int jv_minsr_dual_gram(const void* data, int32_t f32, int64_t n, int64_t p, int32_t is_complex, double diag_shift,
+                       double* S, void* stream) {
+  if (!data || !S || n < 1 || p < 1) return fail(JV_E_ARG, "jv_minsr_dual_gram: bad arguments");
+  if (f32 && is_complex) return fail(JV_E_ARG, "jv_minsr_dual_gram: FP32 data is real only");
+  return dual_gram({data, f32 != 0}, n, p, is_complex, diag_shift, S, (hipStream_t)stream);
+}
+
+int jv_minsr_solve_dual_deferred_gram(const void* data, int32_t f32, double* S, const double* e_raw,
+                                      const double* w, void* e_ready, int64_t n, int64_t p, int32_t is_complex,
+                                      double rcond, double diag_shift, int32_t try_cholesky, double* update,
+                                      jv_solve_info* info, void* stream) {
+  if (!data || !S || !e_raw || !w || !update || n < 1 || p < 1)
+    return fail(JV_E_ARG, "jv_minsr_solve_dual_deferred_gram: bad arguments");
+  if (f32 && is_complex) return fail(JV_E_ARG, "jv_minsr_solve_dual_deferred_gram: FP32 data is real only");
+  hipStream_t st = (hipStream_t)stream;
+  return minsr_dual_finish_info(minsr_dual({data, f32 != 0, S}, nullptr, e_raw, w, (hipEvent_t)e_ready, n, p,
+                                           is_complex, rcond, diag_shift, try_cholesky != 0, update, st),
                                 info, p, rcond, true, st);
 }
 
@@ -1803,12 +1827,23 @@ static int chol_route(const double* S, int64_t p, double rcond, int* flags, bool
 // path the Householder reduction does not need b = D^T e: the producer of e_raw
 // overlaps the Gram and the reduction, and the centring, b and Q^T b run on the
 // auxiliary stream during dstedc.
+// S = D^dagger D (+ shift): the same nonzero spectrum as T = D D^dagger (+ shift).
+static int dual_gram(DualData data, int64_t n, int64_t p, int cpx, double diag_shift, double* S, hipStream_t st) {
+  int rc = data.f32 ? gram_pp_bf16x3(data.ptr, 1, n, p, cpx, S, st, 19)
+                    : jv_gram_pp((const double*)data.ptr, n, p, cpx, S, (void*)st);
+  if (rc) return rc;
+  if (!cpx && diag_shift != 0.0) {
+    hipLaunchKernelGGL(add_diag, dim3((unsigned)((p + 255) / 256)), dim3(256), 0, st, S, p, diag_shift, 0);
+    JV_LAUNCH_CHECK("add_diag");
+  }
+  return JV_OK;
+}
+
 static int minsr_dual(DualData data, const double* e, const double* e_raw, const double* w,
                       hipEvent_t e_ready, int64_t n, int64_t p, int32_t is_complex, double rcond, double diag_shift,
                       bool try_chol, double* update, hipStream_t st) {
-  void* stream = (void*)st;
   const int cpx = is_complex;
-  double* S = (double*)workspace(sizeof(double) * (size_t)p * p * (cpx ? 2 : 1), 6);
+  double* S = data.S_given ? data.S_given : (double*)workspace(sizeof(double) * (size_t)p * p * (cpx ? 2 : 1), 6);
   double* ev = (double*)workspace(sizeof(double) * (size_t)p, 7);
   double* y = (double*)workspace(sizeof(double) * (size_t)p * 2, 8);
   double* x = (double*)workspace(sizeof(double) * (size_t)p * 2, 9);
@@ -1816,14 +1851,8 @@ static int minsr_dual(DualData data, const double* e, const double* e_raw, const
   int* rank = (int*)workspace(64, 11);
   double* b = (double*)workspace(sizeof(double) * (size_t)p * 2, 15);
   if (!S || !ev || !y || !x || !er || !rank || !b) return fail(JV_E_NOMEM, "jv_minsr_solve_dual: workspace allocation failed");
-  // S = D^dagger D (+ shift): same nonzero spectrum as T = D D^dagger (+ shift)
-  int rc = data.f32 ? gram_pp_bf16x3(data.ptr, 1, n, p, cpx, S, st, 19)
-                    : jv_gram_pp((const double*)data.ptr, n, p, cpx, S, stream);
+  int rc = data.S_given ? JV_OK : dual_gram(data, n, p, cpx, diag_shift, S, st);
   if (rc) return rc;
-  if (!cpx && diag_shift != 0.0) {
-    hipLaunchKernelGGL(add_diag, dim3((unsigned)((p + 255) / 256)), dim3(256), 0, st, S, p, diag_shift, 0);
-    JV_LAUNCH_CHECK("add_diag");
-  }
   const bool deferred = e == nullptr;
   // deferred: E_loc mean (2 doubles) and variance, read back with the solve info
   double* emean = deferred ? (double*)workspace(sizeof(double) * 4, 50) : nullptr;

@@ -7,6 +7,7 @@ gradients G give T = 2 G G^dagger for the [G, iG] layout, and the update is
 expanded back to jVMC's flat layout.
 """
 import ctypes
+import os
 
 import numpy as np
 import torch
@@ -115,7 +116,6 @@ class MinSR:
     # Real dual solves compute E_loc on a side stream while the solve builds and
     # reduces the Gram matrix (set False to run the phases one after another).
     overlap = True
-
     def _overlapped(self, psi, n, outp):
         """The real dual solve runs with E_loc still in flight (not when per-phase
         timing is requested: phases then overlap)."""
@@ -123,17 +123,35 @@ class MinSR:
             return False
         return self.solver == "dual" or (self.solver == "auto" and n > psi.numParameters)
 
-    def _solve_deferred(self, Eraw, ready, gradients, p):
+    def _eloc_first(self, psi):
+        """Where E_loc starts on the side stream.  FP64: right after sampling,
+        beside O_k, the Gram and the factorisation (10x10 CNN: 128 steps/s
+        against 125 when it starts after the Gram).  Mixed form (FP32 O):
+        after the Gram, beside the factorisation alone (202 steps/s against
+        195).  VMC_JAX_AMD_MINSR_ELOC_FIRST = 0 / 1 overrides."""
+        env = os.environ.get("VMC_JAX_AMD_MINSR_ELOC_FIRST")
+        if env is not None:
+            return env == "1"
+        return not psi.fp32_gradients
+
+    def _dual_gram(self, gradients):
+        """S = D^T D (+ diagonalShift) of the centred data on the current
+        stream (FP32 data of the mixed form: the compensated BF16 Gram)."""
         D = gradients._data.contiguous()
+        n, P = D.shape
+        S = torch.empty((P, P), dtype=torch.float64, device=D.device)
+        _lib.call("jv_minsr_dual_gram", _lib.ptr(D), int(D.dtype == torch.float32), n, P, 0,
+                  float(self.diagonalShift), _lib.ptr(S), _lib.stream_ptr())
+        return D, S
+
+    def _solve_deferred(self, Eraw, ready, D, S, p):
         n, P = D.shape
         upd = torch.empty(P, dtype=torch.float64, device=D.device)
         info = _lib.JvSolveInfo()
         hint = self._cholesky_hint()
-        # FP32 data (the mixed form): compensated-BF16 Gram, FP64 solve
-        fn = "jv_minsr_solve_dual_deferred_mixed" if D.dtype == torch.float32 else "jv_minsr_solve_dual_deferred"
-        _lib.call(fn, _lib.ptr(D), _lib.ptr(Eraw), _lib.ptr(p), ready._as_parameter_,
-                  n, P, 0, float(self.pinvTol), float(self.diagonalShift), hint, _lib.ptr(upd), ctypes.byref(info),
-                  _lib.stream_ptr())
+        _lib.call("jv_minsr_solve_dual_deferred_gram", _lib.ptr(D), int(D.dtype == torch.float32), _lib.ptr(S),
+                  _lib.ptr(Eraw), _lib.ptr(p), ready._as_parameter_, n, P, 0, float(self.pinvTol),
+                  float(self.diagonalShift), hint, _lib.ptr(upd), ctypes.byref(info), _lib.stream_ptr())
         self.last_info = info
         self._note_route(hint, P)
         return upd
@@ -150,20 +168,30 @@ class MinSR:
         sampleConfigs, sampleLogPsi, p = self.sampler.sample(numSamples=numSamples)
         timer.stop("sampling")
         if self._overlapped(psi, sampleConfigs.shape[1], outp):
-            # E_loc on a side stream; the solve waits for it only where it needs it
+            # O_k and the Gram first, then E_loc on a side stream beside the
+            # latency-bound factorisation (or reduction) of the Gram; the solve
+            # waits for E_loc only where it needs it.  Starting E_loc after
+            # the Gram keeps the two throughput-bound kernels from sharing the
+            # CUs in an order the hardware picks.
             main = torch.cuda.current_stream()
             if getattr(self, "_side", None) is None:
                 self._side = torch.cuda.Stream()
             side = self._side
+            eloc_first = self._eloc_first(psi)
+            if not eloc_first:
+                sampleGradients = SampledObs(psi.gradients(sampleConfigs, layout=0), p)
+                D, S = self._dual_gram(sampleGradients)
             side.wait_stream(main)
             with torch.cuda.stream(side):
                 Eraw = hamiltonian.get_O_loc(sampleConfigs, psi, sampleLogPsi, t)
                 Eraw = Eraw.reshape(-1).to(torch.complex128).contiguous()
                 ready = torch.cuda.Event()
                 ready.record(side)
-            sampleGradients = SampledObs(psi.gradients(sampleConfigs, layout=0), p)
+            if eloc_first:
+                sampleGradients = SampledObs(psi.gradients(sampleConfigs, layout=0), p)
+                D, S = self._dual_gram(sampleGradients)
             w = p.reshape(-1).to(torch.float64).contiguous()
-            update = self._solve_deferred(Eraw, ready, sampleGradients, w)
+            update = self._solve_deferred(Eraw, ready, D, S, w)
             main.wait_stream(side)
             Eraw.record_stream(main)
             Eloc = None  # mean and variance come with the solve info
