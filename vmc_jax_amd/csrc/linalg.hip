@@ -625,17 +625,20 @@ __global__ void __launch_bounds__(256) sum_finish(const double* __restrict__ par
 
 // Margin of the Cholesky test.  If the factorisation of a symmetric p x p
 // matrix A completes in floating point, L L^T = A + dA with
-// ||dA||_2 <= p gamma_{p+1} ||A||_2, gamma_k = k u / (1 - k u), u = 2^-53
+// ||dA||_2 <= p gamma_{p+2} ||A||_2, gamma_k = k u / (1 - k u), u = 2^-53
 // (Higham, Accuracy and Stability of Numerical Algorithms, 2nd ed., Thm 10.3
-// with || |L| |L^T| ||_2 <= p ||A||_2), so lambda_min(A) >= -||dA||_2.  For
-// A = S' - c I (S' = S, or S + a E after a deflation) and
-// c = F (rcond + d + x) / (1 - d), d = p gamma_{p+1}, F >= ||S'||_F >= lambda_max,
+// with || |L| |L^T| ||_2 <= p ||A||_2; gamma_{p+1} there, one rounding more
+// here because the entries below a diagonal block are multiplied by the
+// rounded reciprocal of the pivot instead of divided by it), so
+// lambda_min(A) >= -||dA||_2.  For A = S' - c I (S' = S, or S + a E after a
+// deflation) and
+// c = F (rcond + d + x) / (1 - d), d = p gamma_{p+2}, F >= ||S'||_F >= lambda_max,
 // a completed factorisation gives lambda_min(S') >= c - d (F + c) >=
 // (rcond + x) F: every eigenvalue of S' exceeds the pseudo-inverse cutoff
 // rcond lambda_max plus the deflation's perturbation bound x F.
 __device__ inline double chol_margin(double F, int64_t p, double rcond, double x) {
   const double u = 1.1102230246251565e-16;
-  const double g = (double)(p + 1) * u / (1.0 - (double)(p + 1) * u);
+  const double g = (double)(p + 2) * u / (1.0 - (double)(p + 2) * u);
   const double d = (double)p * g;
   return F * (rcond + d + x) / (1.0 - d);
 }
