@@ -124,15 +124,19 @@ class MinSR:
         return self.solver == "dual" or (self.solver == "auto" and n > psi.numParameters)
 
     def _eloc_first(self, psi):
-        """Where E_loc starts on the side stream.  FP64: right after sampling,
-        beside O_k, the Gram and the factorisation (10x10 CNN: 128 steps/s
-        against 125 when it starts after the Gram).  Mixed form (FP32 O):
-        after the Gram, beside the factorisation alone (202 steps/s against
-        195).  VMC_JAX_AMD_MINSR_ELOC_FIRST = 0 / 1 overrides."""
+        """Where E_loc starts on the side stream (10x10 CNN measurements).
+        FP64, when the Cholesky route is tried: right after sampling, beside
+        O_k, the Gram and the factorisation (128 steps/s against 125 when it
+        starts after the Gram).  FP64, when the attempt is skipped (a cut
+        spectrum expected): after the Gram, beside the latency-bound
+        Householder reduction (55.6 against 50.6 steps/s).  Mixed form (FP32
+        O): after the Gram (202 against 195 steps/s on the Cholesky route,
+        equal on cut spectra).  VMC_JAX_AMD_MINSR_ELOC_FIRST = 0 / 1
+        overrides."""
         env = os.environ.get("VMC_JAX_AMD_MINSR_ELOC_FIRST")
         if env is not None:
             return env == "1"
-        return not psi.fp32_gradients
+        return not psi.fp32_gradients and self._chol_skip == 0
 
     def _dual_gram(self, gradients):
         """S = D^T D (+ diagonalShift) of the centred data on the current
