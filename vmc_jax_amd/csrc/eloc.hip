@@ -265,6 +265,16 @@ inline int elo_wgs_per_cu(bool delta, int tsize) {
   const int e = tsize == 4 ? env_m : env;
   return e > 0 ? (e > 8 ? 8 : e) : 2;
 }
+// Threads per workgroup (256 default; 320, 384 or 512 through
+// VMC_JAX_AMD_ELOC_THREADS and VMC_JAX_AMD_ELOC_THREADS_MIXED).  On the 10x10
+// CNN 512 threads make the kernel alone 9-12 % faster (FP64 4.01 -> 3.66 ms,
+// mixed 2.09 -> 1.84) but, beside MinSR's solve, leave its Cholesky kernels
+// too few registers on each SIMD (FP64 128 -> 118.5 steps/s, mixed 203 ->
+// 181.5; 384 threads: 122.7 / 190.9).
+inline int elo_threads(int tsize) {
+  const int t = elo_env_int(tsize == 4 ? "VMC_JAX_AMD_ELOC_THREADS_MIXED" : "VMC_JAX_AMD_ELOC_THREADS", 256);
+  return t == 512 || t == 384 || t == 320 ? t : 256;
+}
 inline size_t elo_reserve(int tsize) {
   static const int env = elo_env_int("VMC_JAX_AMD_ELOC_RESERVE_KB", -1);
   static const int env_m = elo_env_int("VMC_JAX_AMD_ELOC_RESERVE_KB_MIXED", -1);
@@ -317,7 +327,7 @@ inline int elo_win_chunk(const CnnGeom& g, int cm, int nops, int threads, int ts
 // ms, FP32 2.04 against 2.63 ms); which form wins beside MinSR's solve is in
 // the notes on the chunk size above.
 template <int CM, typename T, bool F3, bool DL>
-__global__ void __launch_bounds__(256) eloc_cnn_win(CnnGeom g, const T* __restrict__ Wpk, CnnPack pk, OpDev op,
+__global__ void __launch_bounds__(512) eloc_cnn_win(CnnGeom g, const T* __restrict__ Wpk, CnnPack pk, OpDev op,
                                                     const int32_t* __restrict__ cfg,
                                                     const double* __restrict__ lps, int64_t n, int fch,
                                                     double* __restrict__ out) {
@@ -615,7 +625,8 @@ int jv_eloc(const jv_net* net, const double* params, const jv_optable* op, const
     const int ts = mixed ? 4 : 8;
     const bool f3 = cg.Fx == 3 && cg.Fy == 3 && cg.nl <= 3;
     const bool delta = f3 && elo_delta(ts);  // the kernel's DELTA
-    const int fch = elo_win_chunk(cg, cm, op->num_ops, 256, ts, delta);
+    const int thr = elo_threads(ts);
+    const int fch = elo_win_chunk(cg, cm, op->num_ops, thr, ts, delta);
     // at least a (wgs + 1)-th of the LDS, so that no further workgroup of this
     // kernel takes the reserve
     const size_t lds = std::max(elo_win_bytes(cg, cm, op->num_ops, fch, ts, delta),
@@ -633,17 +644,17 @@ int jv_eloc(const jv_net* net, const double* params, const jv_optable* op, const
     if (delta) {                                                                                         \
       rc = set_lds_limit((const void*)eloc_cnn_win<CMv, Tv, true, true>, lds);                           \
       if (rc) return rc;                                                                                 \
-      hipLaunchKernelGGL((eloc_cnn_win<CMv, Tv, true, true>), dim3((unsigned)blocks), dim3(256), lds, st, cg, wpk, \
+      hipLaunchKernelGGL((eloc_cnn_win<CMv, Tv, true, true>), dim3((unsigned)blocks), dim3(thr), lds, st, cg, wpk, \
                          pk, d, configs, logpsi_s, n, fch, out);                                         \
     } else if (f3) {                                                                                     \
       rc = set_lds_limit((const void*)eloc_cnn_win<CMv, Tv, true, false>, lds);                          \
       if (rc) return rc;                                                                                 \
-      hipLaunchKernelGGL((eloc_cnn_win<CMv, Tv, true, false>), dim3((unsigned)blocks), dim3(256), lds, st, cg, \
+      hipLaunchKernelGGL((eloc_cnn_win<CMv, Tv, true, false>), dim3((unsigned)blocks), dim3(thr), lds, st, cg, \
                          wpk, pk, d, configs, logpsi_s, n, fch, out);                                    \
     } else {                                                                                             \
       rc = set_lds_limit((const void*)eloc_cnn_win<CMv, Tv, false, false>, lds);                         \
       if (rc) return rc;                                                                                 \
-      hipLaunchKernelGGL((eloc_cnn_win<CMv, Tv, false, false>), dim3((unsigned)blocks), dim3(256), lds, st, cg, \
+      hipLaunchKernelGGL((eloc_cnn_win<CMv, Tv, false, false>), dim3((unsigned)blocks), dim3(thr), lds, st, cg, \
                          wpk, pk, d, configs, logpsi_s, n, fch, out);                                    \
     }                                                                                                    \
     break;                                                                                               \
