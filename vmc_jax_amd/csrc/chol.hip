@@ -30,6 +30,14 @@ constexpr int CB = 64;  // panel width
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
+// Lane `lane` (a compile-time constant after unrolling) of v to every lane,
+// as a scalar operand.
+__device__ inline double readlane_d(double v, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+
 // Lane S of each group of four lanes to the whole group (DPP quad_perm
 // broadcast: a VALU move, no LDS round trip).
 template <int S>
@@ -79,12 +87,13 @@ __global__ void __launch_bounds__(256) chol_panel(double* A0, double* A1, int p,
   }
   __syncthreads();
   // The diagonal block is factored by wave 0 alone, lane t holding row t in
-  // registers (columns fully unrolled); per column the scaled column passes
-  // through LDS.  Lanes of one wave see each other's LDS writes in program
-  // order, so the steps need no workgroup barrier.  (A version with the rows
-  // in LDS and a runtime loop over the pivots, 24 KB of code against 117 KB,
-  // took three times as long: 330k against 104k cycles per block.)
-  __shared__ double colj[CB];
+  // registers (columns fully unrolled).  The pivot and the scaled column
+  // entries L_cj reach the other lanes by readlane (scalar operands of the
+  // update), not through LDS: per column the dependent chain is the pivot,
+  // its square root, the quotient and one FMA.  (An LDS exchange of the
+  // column took 104k cycles per block; rows in LDS with a runtime loop over
+  // the pivots 330k; four waves with a quarter of each row and one barrier
+  // per column 112k.)
   __shared__ double rdiag[CB];  // 1 / L_jj for the row solve
   if (tid < 64) {
     const int t = tid;
@@ -95,7 +104,7 @@ __global__ void __launch_bounds__(256) chol_panel(double* A0, double* A1, int p,
 #pragma unroll
     for (int j = 0; j < CB; ++j) {
       if (j < b) {
-        const double dj = __shfl(x[j], j, 64);
+        const double dj = readlane_d(x[j], j);
         const double sj = sqrt(dj);
         if (first_bad == 0 && !(dj > 0.0)) first_bad = j + 1;
         double l = 0.0;
@@ -105,18 +114,11 @@ __global__ void __launch_bounds__(256) chol_panel(double* A0, double* A1, int p,
         } else if (t == j) {
           x[j] = sj;
         }
-        colj[t] = l;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
         for (int c = j + 1; c < CB; ++c) {
-          const double lc = colj[c];
+          const double lc = readlane_d(l, c);
           if (c <= t) x[c] -= l * lc;
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       }
     }
     double dt = 1.0;
