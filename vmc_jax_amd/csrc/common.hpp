@@ -119,8 +119,8 @@ constexpr double LC_ATANH[10] = {0.080820228059717655, 0.044003692430723371, 0.0
                                  0.066570721048274675, 0.076927850324793115, 0.090908947180664293,
                                  0.11111111358713932,  0.14285714283554379,  0.20000000000007292,
                                  0.33333333333333331};
-__host__ __device__ inline double log_cosh(double x) {
-  const double a = fabs(x);
+// e^{-2 min(a, 40)} for a >= 0: the exponential step of log_cosh.
+__host__ __device__ inline double exp_m2(double a) {
   const double y = -2.0 * fmin(a, 40.0);
   const double n = rint(y * 1.4426950408889634);
   double r = fma(-n, 6.93147180559945286227e-01, y);
@@ -128,7 +128,11 @@ __host__ __device__ inline double log_cosh(double x) {
   double p = LC_EXP[0];
 #pragma unroll
   for (int k = 1; k < 12; ++k) p = fma(p, r, LC_EXP[k]);
-  const double t = ldexp(p, (int)n);
+  return ldexp(p, (int)n);
+}
+__host__ __device__ inline double log_cosh(double x) {
+  const double a = fabs(x);
+  const double t = exp_m2(a);
   const double d = 2.0 + t;
 #if defined(__HIP_DEVICE_COMPILE__)
   double q = __builtin_amdgcn_rcp(d);
@@ -145,6 +149,50 @@ __host__ __device__ inline double log_cosh(double x) {
   for (int k = 1; k < 10; ++k) w = fma(w, z, LC_ATANH[k]);
   return (a - LN2) + 2.0 * fma(s * z, w, s);
 }
+// Natural logarithm of a finite P >= 1: P = 2^e m with m in [sqrt(1/2),
+// sqrt(2)), log m = 2 atanh(s), s = (m - 1) / (m + 1), |s| <= 0.1716, by the
+// atanh polynomial of log_cosh (fitted on s^2 <= 1/9) and the quotient by the
+// hardware reciprocal with two Newton steps and a residual correction
+// (m + 1 lies in [1.7, 2.5]).  Absolute error a few 1e-16 for P <= 2^8.
+__device__ inline double log_ge1(double P) {
+  int e;
+  double m = frexp(P, &e);  // [0.5, 1)
+  if (m < 0.70710678118654752) {
+    m *= 2.0;
+    e -= 1;
+  }
+  const double num = m - 1.0, d = m + 1.0;
+  double q = __builtin_amdgcn_rcp(d);
+  q = fma(q, fma(-d, q, 1.0), q);
+  q = fma(q, fma(-d, q, 1.0), q);
+  double s = num * q;
+  s = fma(fma(-d, s, num), q, s);
+  const double z = s * s;
+  double w = LC_ATANH[0];
+#pragma unroll
+  for (int k = 1; k < 10; ++k) w = fma(w, z, LC_ATANH[k]);
+  const double de = (double)e;
+  return fma(de, 6.93147180559945286227e-01, fma(de, 2.31904681384629955842e-17, 2.0 * fma(s * z, w, s)));
+}
+
+// Sum over the CM channels of one window element of log cosh(z_c) - A_c
+// (A_c = stored log cosh of the unflipped configuration, at stride cs):
+// sum (|z_c| - log 2 - A_c) + log prod (1 + e^{-2|z_c|}), one logarithm for
+// the element instead of one log1p per channel.  Each factor lies in (1, 2],
+// so the product (<= 2^CM) neither overflows nor loses range; the absolute
+// error (a few 1e-16 per factor) matches the per-channel form's.
+template <int CM, typename A>
+__device__ inline double log_cosh_change(const double* z, A* Ab, int cs) {
+  double prod = 1.0, lin = 0.0;
+#pragma unroll
+  for (int c = 0; c < CM; ++c) {
+    const double a = fabs(z[c]);
+    prod *= 1.0 + exp_m2(a);
+    lin += (a - LN2) - Ab[c * cs];
+  }
+  return lin + log_ge1(prod);
+}
+
 // log cosh(x) and tanh(x) from one exponential: with y = -2 min(|x|, 40) =
 // n ln2 + r and e^r = 1 + r P(r) (the polynomial of log_cosh before its last
 // step), e^y - 1 = r P(r) when n = 0 (no cancellation for small |x|) and

@@ -476,8 +476,12 @@ __global__ void __launch_bounds__(512) eloc_cnn_win(CnnGeom g, const T* __restri
               for (int cc = 0; cc < CM; ++cc) Wo[(size_t)cc * wsz * fch] = log_cosh(z[cc]) - Ab[cc * g.n];
             } else {
               double acc = 0.0;
+              if constexpr (std::is_same<T, double>::value) {
+                acc = log_cosh_change<CM>(z, Ab, g.n);
+              } else {
 #pragma unroll
-              for (int cc = 0; cc < CM; ++cc) acc += (double)(log_cosh(z[cc]) - Ab[cc * g.n]);
+                for (int cc = 0; cc < CM; ++cc) acc += (double)(log_cosh(z[cc]) - Ab[cc * g.n]);
+              }
               part[j * fch + f] = acc;
             }
             continue;
@@ -501,8 +505,12 @@ __global__ void __launch_bounds__(512) eloc_cnn_win(CnnGeom g, const T* __restri
           } else {
             lds_t<T>* Ab = (lds_t<T>*)(A + (size_t)L1 * lsz + win_site(g, si, sj, a, b));
             double acc = 0.0;
+            if constexpr (std::is_same<T, double>::value) {
+              acc = log_cosh_change<CM>(z, Ab, g.n);
+            } else {
 #pragma unroll
-            for (int cc = 0; cc < CM; ++cc) acc += (double)(log_cosh(z[cc]) - Ab[cc * g.n]);
+              for (int cc = 0; cc < CM; ++cc) acc += (double)(log_cosh(z[cc]) - Ab[cc * g.n]);
+            }
             part[f * wlast + j] = acc;
           }
         }
