@@ -196,18 +196,24 @@ __device__ inline int wrap_once(int q, int L) { return q < 0 ? q + L : (q >= L ?
 
 template <int CM, int LC, typename T>
 __device__ inline void win_preact3(const CnnGeom& g, const cst_t<T>* __restrict__ K, const cst_t<T>* __restrict__ B,
-                                   const T* Xb, const T* Xw, int si, int sj, int a, int b, T* z) {
+                                   const T* Xb, const T* Xw, const short* nb, int a, int b, T* z) {
+  // nb: the nine stored-layer positions read by this element's kernel
+  // offsets (the neighbour table's row of the element's own position).  An
+  // input inside the previous layer's window is read from Xw instead; both
+  // choices are element offsets from Xb, selected without a branch.
   constexpr int pex = 1 + 2 * LC, pey = pex, pwsz = pex * pey;
+  const int dw = (int)(Xw - Xb);
 #pragma unroll
   for (int c = 0; c < CM; ++c) z[c] = B[c];
 #pragma unroll 1
   for (int k = 0; k < 9; ++k) {
     const int ky = k / 3, kx = k - 3 * (k / 3);
     const int aq = a - ky, bq = b - kx;
-    const bool in = aq >= 0 && aq < pex && bq >= 0 && bq < pey;
-    const int qi = wrap_once(si - aq, g.Lx), qj = wrap_once(sj - bq, g.Ly);
-    lds_t<T>* x = (lds_t<T>*)(in ? Xw + (aq * pey + bq) : Xb + (qi * g.Ly + qj));
+    const bool in = (unsigned)aq < (unsigned)pex && (unsigned)bq < (unsigned)pey;
+    const int qb = ((const __attribute__((address_space(3))) short*)nb)[k];
+    const int off = in ? dw + (aq * pey + bq) : qb;
     const int xs = in ? pwsz : g.n;
+    lds_t<T>* x = (lds_t<T>*)(Xb + off);
     const cst_t<T>* w = K + k * CM * CM;
 #pragma unroll
     for (int ci = 0; ci < CM; ++ci) {
