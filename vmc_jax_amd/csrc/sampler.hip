@@ -348,12 +348,13 @@ __global__ void __launch_bounds__(64 * G) mc_cnn_win(CnnGeom g, const T* __restr
 // window element per lane in every layer).  What a proposal at site s reads
 // differs from mc_cnn_win only in how the input locations are found: a
 // window element's nine inputs are either inside the previous layer's window
-// (a fixed slot n + aq * pey + bq) or at the stored position of the site
-// shifted by (-aq, -bq); which of the two, and the slot or the shift, depend
-// on the lane and the layer only.  Every lane keeps these 9 codes per layer
-// in registers, so a proposal computes its input addresses with a few integer
-// operations and issues all input loads of a layer at once, with no table
-// pass and no table reads between the draws.
+// (a fixed slot n + aq * pey + bq) or at the stored position (si - a + ky,
+// sj - b + kx) (periodic) of tap (ky, kx); which of the two, and the slot,
+// depend on the lane and the layer only.  Every lane keeps its 9 slots per
+// layer in registers; per proposal and layer it wraps three rows and three
+// columns once, so each input address is one add and one select, and all
+// input loads of a layer are issued at once, with no table pass and no table
+// reads between the draws.
 constexpr int MCW3_MAXL = 3;
 
 __host__ __device__ inline bool mc_win3_ok(const CnnGeom& g) {
@@ -400,10 +401,9 @@ __global__ void __launch_bounds__(64 * G) mc_cnn_win3(CnnGeom g, const T* __rest
     s[i] = walkers[(int64_t)chain * g.n + i];
     x0[i] = (T)(2 * s[i] - 1);
   }
-  // per layer: this lane's window element (a, b) and its nine input codes
-  // (>= 0: the slot in the previous layer's window; < 0: -1 - the packed
-  // shift ((aq + 16) << 8) | (bq + 16) of the stored position)
-  int code[MCW3_MAXL][9];
+  // per layer: this lane's window element (a, b) and its nine input slots
+  // (>= 0: the slot in the previous layer's window; -1: the stored layer)
+  int slot[MCW3_MAXL][9];
   int wa[MCW3_MAXL], wb[MCW3_MAXL];
 #pragma unroll
   for (int l = 0; l < MCW3_MAXL; ++l) {
@@ -415,7 +415,7 @@ __global__ void __launch_bounds__(64 * G) mc_cnn_win3(CnnGeom g, const T* __rest
     for (int k = 0; k < 9; ++k) {
       const int aq = ja - k / 3, bq = jb - k % 3;
       const bool in = l > 0 && aq >= 0 && aq < pex && bq >= 0 && bq < pey;
-      code[l][k] = in ? g.n + aq * pey + bq : -1 - (((aq + 16) << 8) | (bq + 16));
+      slot[l][k] = in ? g.n + aq * pey + bq : -1;
     }
   }
   __syncthreads();
@@ -446,14 +446,21 @@ __global__ void __launch_bounds__(64 * G) mc_cnn_win3(CnnGeom g, const T* __rest
         const cst_t<T>* B = Wp + pk.boff[l] + w * CG;
         T* Ul = U + (size_t)l * CM * CS + (size_t)w * CG * CS;
         if (lane < wsz) {
+          // rows si - a + ky and columns sj - b + kx of the stored layer
+          // (a < Lx: the window fits the lattice, so one period at most)
+          int rw[3], cl[3];
+          rw[0] = wrap1(si - wa[l], g.Lx);
+          cl[0] = wrap1(sj - wb[l], g.Ly);
+#pragma unroll
+          for (int r = 1; r < 3; ++r) {
+            rw[r] = rw[r - 1] + 1 == g.Lx ? 0 : rw[r - 1] + 1;
+            cl[r] = cl[r - 1] + 1 == g.Ly ? 0 : cl[r - 1] + 1;
+          }
+#pragma unroll
+          for (int r = 0; r < 3; ++r) rw[r] *= g.Ly;
           int q[9];
 #pragma unroll
-          for (int k = 0; k < 9; ++k) {
-            const int e = code[l][k];
-            const int t = -1 - e;
-            const int qi = wrap1(si - ((t >> 8) - 16), g.Lx), qj = wrap1(sj - ((t & 255) - 16), g.Ly);
-            q[k] = e >= 0 ? e : qi * g.Ly + qj;
-          }
+          for (int k = 0; k < 9; ++k) q[k] = slot[l][k] >= 0 ? slot[l][k] : rw[k / 3] + cl[k % 3];
           T z[CG];
 #pragma unroll
           for (int c = 0; c < CG; ++c) z[c] = B[c];
@@ -488,7 +495,7 @@ __global__ void __launch_bounds__(64 * G) mc_cnn_win3(CnnGeom g, const T* __rest
 #pragma unroll
             for (int c = 0; c < CG; ++c) Ul[c * CS + g.n + lane] = log_cosh(z[c]);
           } else {
-            const int p = wrap1(si - wa[l], g.Lx) * g.Ly + wrap1(sj - wb[l], g.Ly);
+            const int p = rw[0] + cl[0];  // the element's own position
 #pragma unroll
             for (int c = 0; c < CG; ++c) {
               const T v = log_cosh(z[c]);
