@@ -445,6 +445,23 @@ int jv_minsr_solve_dual_deferred_gram(const void* data, int32_t f32, double* S, 
                                       double rcond, double diag_shift, int32_t try_cholesky, double* update,
                                       jv_solve_info* info, void* stream);
 
+/* jv_minsr_solve_dual_deferred_gram without the host synchronisation at its
+ * end: info is filled by a copy enqueued on the stream after the solve, so it
+ * is valid once the stream has passed the call (an event recorded after it
+ * has completed); info must stay allocated until then, and must be pinned host
+ * memory (hipHostMalloc) for the call not to block.  A Cholesky factorisation
+ * that failed on the accepted route, an error return of the blocking form, is
+ * reported as info->iterations = -1 - its info.  The host still waits inside
+ * the call for the Cholesky attempt's verdict (it selects the kernels that
+ * follow), not for E_loc or the rest of the solve.  Same update.  No
+ * separate reference function (minsr.py:82-165 returns the update and keeps
+ * the energy for later reads). */
+int jv_minsr_solve_dual_deferred_gram_async(const void* data, int32_t f32, double* S, const double* e_raw,
+                                            const double* w, void* e_ready, int64_t n, int64_t p,
+                                            int32_t is_complex, double rcond, double diag_shift,
+                                            int32_t try_cholesky, double* update, jv_solve_info* info,
+                                            void* stream);
+
 /* TDVP solve (tdvp.py:134-213).  data: centred O in jVMC's flat layout, complex
  * [n][pr]; e: centred E_loc, complex [n]; w: weights [n]; x = rhsPrefactor
  * (complex, host); make_real 0 = 'real', 1 = 'imag'.  use_snr selects the SNR

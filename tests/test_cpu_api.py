@@ -224,7 +224,7 @@ def test_minsr_cholesky_retry_policy(lib):
         eq.last_info = lib.JvSolveInfo()
         eq.last_info.iterations = iterations
         eq.last_info.rank = rank
-        eq._note_route(tried, full)
+        eq._note_route(tried, full, eq.last_info)
 
     assert eq._cholesky_hint() == 1
     solved(1, 0, 1100, 1240)          # refused: the spectrum was cut
@@ -240,3 +240,41 @@ def test_minsr_cholesky_retry_policy(lib):
     assert eq._cholesky_hint() == 0
     solved(0, 0, 1230, 1240)          # skipped and still cut: keep skipping
     assert eq._chol_skip == MinSR.CHOLESKY_RETRY - 2
+
+
+def test_minsr_deferred_outcome_is_read_lazily(lib):
+    """The overlapped solve's outcome (route, rank, E_loc mean and variance)
+    is read only when needed: the route bookkeeping runs when the next route
+    choice or last_info asks for it, the energy when ElocMean0 / ElocVar0 is
+    read, each once; a failed factorisation raises when read."""
+    from vmc_jax_amd.util.minsr import MinSR
+
+    class Stub:  # stands for the pinned record of a non-blocking solve
+        def __init__(self, hint, full, iterations, rank, e):
+            self.hint, self.full, self.reads = hint, full, 0
+            self.info = lib.JvSolveInfo()
+            self.info.iterations, self.info.rank = iterations, rank
+            self.info.e_mean[0], self.info.e_mean[1], self.info.e_var = e, 0.5, 2.0
+
+        def resolve(self):
+            self.reads += 1
+            if self.info.iterations < 0:
+                raise lib.HipError("factorisation failed")
+            return self.info
+
+    eq = MinSR(None, pinvTol=1e-8)
+    st = Stub(1, 1240, 0, 1100, -3.25)     # a refused attempt, energy -3.25
+    eq._pending = eq._energy_src = st
+    assert st.reads == 0 and eq.last_outcome() is st
+    assert eq._cholesky_hint() == 0         # the refusal is seen by the next choice
+    assert eq._chol_skip == MinSR.CHOLESKY_RETRY - 2 and st.reads == 1
+    assert eq.get_energy_mean() == -3.25 and eq.ElocMean0 == complex(-3.25, 0.5) and eq.ElocVar0 == 2.0
+    assert eq.last_info.rank == 1100
+    st2 = Stub(0, 1240, 0, 1240, 1.0)
+    eq._pending = eq._energy_src = st2
+    eq.ElocMean0 = 7.0                      # a later value replaces it after the read
+    assert eq.ElocMean0 == 7.0 and eq._pending is None and eq._chol_skip == 0
+    bad = Stub(1, 1240, -2, 0, 0.0)
+    eq._pending = bad
+    with pytest.raises(lib.HipError):
+        eq.last_info

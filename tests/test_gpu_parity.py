@@ -612,6 +612,28 @@ def test_minsr_dual_deferred_equals_dual(shift, shape, ch, N, P, tol, stedc):
     torch.cuda.current_stream().wait_stream(side)
     assert psi.numParameters == P and eq.last_info.rank > 0
     assert rel(u2, u1) < tol
+    # the call above is the non-blocking form (info copied to pinned memory in
+    # stream order, read by last_info); the blocking form on the same inputs
+    # gives the same update and the same info
+    i2 = eq.last_info
+    D4, S4 = eq._dual_gram(G)
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        e4 = H.get_O_loc(s, psi, lp, 0.0).reshape(-1).contiguous()
+        ready4 = torch.cuda.Event()
+        ready4.record(side)
+    u4 = torch.empty(P, dtype=torch.float64, device="cuda")
+    i4 = jv._lib.JvSolveInfo()
+    jv._lib.call("jv_minsr_solve_dual_deferred_gram", jv._lib.ptr(D4), 0, jv._lib.ptr(S4), jv._lib.ptr(e4),
+                 jv._lib.ptr(w), ready4._as_parameter_, D4.shape[0], P, 0, 1e-10, float(shift),
+                 int(i2.iterations == 1), jv._lib.ptr(u4), ctypes.byref(i4), jv._lib.stream_ptr())
+    torch.cuda.current_stream().wait_stream(side)
+    assert rel(to_np(u4), u2) < 1e-13
+    assert (i4.rank, i4.iterations) == (i2.rank, i2.iterations)
+    assert abs(i4.e_mean[0] - i2.e_mean[0]) <= 1e-13 * abs(i4.e_mean[0]) and i4.e_mean[1] == i2.e_mean[1]
+    assert abs(i4.e_var - i2.e_var) <= 1e-12 * abs(i4.e_var)
+    em = El.mean()[0].item()
+    assert abs(i2.e_mean[0] - em.real) <= 1e-12 * abs(em)
     # the one-call deferred form (Gram inside, E_loc already in flight)
     side.wait_stream(torch.cuda.current_stream())
     with torch.cuda.stream(side):

@@ -123,6 +123,8 @@ _SIGS = {
     "jv_minsr_dual_gram": [_vp, _i32, _i64, _i64, _i32, _dbl, _vp, _vp],
     "jv_minsr_solve_dual_deferred_gram": [_vp, _i32, _vp, _vp, _vp, _vp, _i64, _i64, _i32, _dbl, _dbl, _i32, _vp,
                                           ctypes.POINTER(JvSolveInfo), _vp],
+    "jv_minsr_solve_dual_deferred_gram_async": [_vp, _i32, _vp, _vp, _vp, _vp, _i64, _i64, _i32, _dbl, _dbl, _i32,
+                                                _vp, _vp, _vp],
     "jv_tdvp_solve": [_vp, _vp, _vp, _i64, _i64, ctypes.POINTER(_dbl), _i32, _dbl, _i32, _dbl, _dbl, _dbl,
                       _dbl, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(JvSolveInfo), _vp],
     "jv_tdvp_solve_holomorphic": [_vp, _vp, _vp, _i64, _i64, _vp, _vp, _i64, ctypes.POINTER(_dbl), _i32, _dbl, _i32,

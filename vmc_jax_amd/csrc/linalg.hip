@@ -1679,6 +1679,7 @@ static int dual_gram(DualData data, int64_t n, int64_t p, int cpx, double diag_s
 int gram_pp_bf16x3(const void* D, int f32, int64_t n, int64_t p, int cpx, double* S, hipStream_t st, int slot);
 static int minsr_dual_finish_info(int rc, jv_solve_info* info, int64_t p, double rcond, bool deferred,
                                   hipStream_t st);
+static int minsr_dual_post_info(int rc, jv_solve_info* info, double rcond, bool deferred, hipStream_t st);
 }  // namespace jv
 
 extern "C" {
@@ -1734,6 +1735,20 @@ int jv_minsr_solve_dual_deferred_gram(const void* data, int32_t f32, double* S, 
   return minsr_dual_finish_info(minsr_dual({data, f32 != 0, S}, nullptr, e_raw, w, (hipEvent_t)e_ready, n, p,
                                            is_complex, rcond, diag_shift, try_cholesky != 0, update, st),
                                 info, p, rcond, true, st);
+}
+
+int jv_minsr_solve_dual_deferred_gram_async(const void* data, int32_t f32, double* S, const double* e_raw,
+                                            const double* w, void* e_ready, int64_t n, int64_t p,
+                                            int32_t is_complex, double rcond, double diag_shift,
+                                            int32_t try_cholesky, double* update, jv_solve_info* info,
+                                            void* stream) {
+  if (!data || !S || !e_raw || !w || !update || !info || n < 1 || p < 1)
+    return fail(JV_E_ARG, "jv_minsr_solve_dual_deferred_gram_async: bad arguments");
+  if (f32 && is_complex) return fail(JV_E_ARG, "jv_minsr_solve_dual_deferred_gram_async: FP32 data is real only");
+  hipStream_t st = (hipStream_t)stream;
+  return minsr_dual_post_info(minsr_dual({data, f32 != 0, S}, nullptr, e_raw, w, (hipEvent_t)e_ready, n, p,
+                                         is_complex, rcond, diag_shift, try_cholesky != 0, update, st),
+                              info, rcond, true, st);
 }
 
 }  // extern "C"
@@ -2008,6 +2023,37 @@ static int minsr_dual_finish_info(int rc, jv_solve_info* info, int64_t p, double
   info->residual = 0.0;
   info->pinv_cutoff = rcond;
   (void)p;
+  return JV_OK;
+}
+
+// The outcome record of a non-blocking solve, written on the device from the
+// rank / route slot and (deferred) the E_loc mean and variance slot; a
+// Cholesky factorisation that failed on the accepted route is reported as
+// iterations = -1 - its info.
+__global__ void write_solve_info(const int* __restrict__ hr, const double* __restrict__ em, double rcond,
+                                 jv_solve_info* __restrict__ rec) {
+  if (threadIdx.x != 0) return;
+  rec->residual = 0.0;
+  rec->pinv_cutoff = rcond;
+  rec->iterations = (hr[1] == 1 && hr[3] != 0) ? -1 - hr[3] : hr[1];
+  rec->rank = hr[0];
+  rec->e_var = em ? em[2] : 0.0;
+  rec->tdvp_error = 0.0;
+  rec->e_mean[0] = em ? em[0] : 0.0;
+  rec->e_mean[1] = em ? em[1] : 0.0;
+}
+
+// minsr_dual_finish_info without the host synchronisation: the record is
+// formed on the device and copied to info in stream order.
+static int minsr_dual_post_info(int rc, jv_solve_info* info, double rcond, bool deferred, hipStream_t st) {
+  if (rc || !info) return rc;
+  const int* rank = (const int*)workspace(64, 11);
+  const double* emean = deferred ? (const double*)workspace(sizeof(double) * 4, 50) : nullptr;
+  jv_solve_info* rec = (jv_solve_info*)workspace(sizeof(jv_solve_info), 121);
+  if (!rank || (deferred && !emean) || !rec) return fail(JV_E_NOMEM, "jv_minsr_solve_dual: workspace allocation failed");
+  hipLaunchKernelGGL(write_solve_info, dim3(1), dim3(64), 0, st, rank, emean, rcond, rec);
+  JV_LAUNCH_CHECK("write_solve_info");
+  JV_HIP(hipMemcpyAsync(info, rec, sizeof(jv_solve_info), hipMemcpyDeviceToHost, st));
   return JV_OK;
 }
 }  // namespace jv

@@ -16,6 +16,32 @@ from .. import _lib
 from .tdvp import _Timer
 
 
+class _Outcome:
+    """The jv_solve_info of a non-blocking solve, copied by the library into
+    pinned host memory in stream order; read once the event recorded after
+    the solve has completed.  `hint` and `full` are the route bookkeeping's
+    arguments for that solve."""
+
+    def __init__(self, buf, event, hint, full):
+        self._buf, self._event, self.hint, self.full = buf, event, hint, full
+        self._info = None
+
+    def resolve(self):
+        if self._info is None:
+            self._event.synchronize()
+            info = _lib.JvSolveInfo.from_buffer_copy(ctypes.string_at(self._buf.data_ptr(),
+                                                                      ctypes.sizeof(_lib.JvSolveInfo)))
+            self._buf = self._event = None
+            if info.iterations < 0:
+                raise _lib.HipError("jv_minsr_solve_dual_deferred_gram_async failed: Cholesky factorisation "
+                                    "failed (info %d)" % (-1 - info.iterations))
+            self._info = info
+        return self._info
+
+    def energy_mean(self):
+        return float(self.resolve().e_mean[0])
+
+
 class MinSR:
     def __init__(self, sampler, pinvTol=1e-14, diagonalShift=0., diagonalizeOnDevice=True, solver="auto"):
         """``solver``: "T" diagonalises the N x N tangent kernel as the reference
@@ -30,8 +56,63 @@ class MinSR:
         self.diagonalShift = diagonalShift
         self.diagonalizeOnDevice = diagonalizeOnDevice
         self.metaData = None
+        self._pending = None  # _Outcome of the last overlapped solve, not yet read
+        self._energy_src = None  # _Outcome whose E_loc mean and variance are ElocMean0 / ElocVar0
         self.last_info = None
         self._chol_skip = 0
+
+    # The overlapped solve returns before the device has finished it, and its
+    # info is read when first needed: by the next solve's route choice (after
+    # the next step's sampling is queued), or by a read of last_info,
+    # ElocMean0 or ElocVar0.  The host thus never waits at the end of a step.
+    @property
+    def last_info(self):
+        pend = self._pending
+        if pend is not None:
+            self._pending = None
+            self._last_info = pend.resolve()
+            self._note_route(pend.hint, pend.full, self._last_info)
+        return self._last_info
+
+    @last_info.setter
+    def last_info(self, info):
+        if self._pending is not None:
+            self.last_info  # read (and release) the outstanding outcome first
+        self._last_info = info
+
+    def last_outcome(self):
+        """The last overlapped solve's outcome (resolved on first read), or None."""
+        return self._pending if self._pending is not None else self._energy_src
+
+    def _energy(self):
+        src = self._energy_src
+        if src is not None:
+            if src is self._pending:
+                self.last_info  # settles the route bookkeeping first
+            info = src.resolve()
+            self._energy_src = None
+            self._ElocMean0 = complex(info.e_mean[0], info.e_mean[1])
+            self._ElocVar0 = float(info.e_var)
+
+    @property
+    def ElocMean0(self):
+        self._energy()
+        return self._ElocMean0
+
+    @ElocMean0.setter
+    def ElocMean0(self, v):
+        self._energy()  # an outstanding outcome is read (and released) first
+        self._ElocMean0 = v
+
+    @property
+    def ElocVar0(self):
+        self._energy()
+        return self._ElocVar0
+
+    @ElocVar0.setter
+    def ElocVar0(self, v):
+        self._energy()
+        self._ElocVar0 = v
 
     # Solves try the Cholesky route first (exact when no eigenvalue falls below
     # the cutoff).  After a refused attempt, the attempt is skipped for the
@@ -49,19 +130,20 @@ class MinSR:
     try_cholesky_holomorphic = True
 
     def _cholesky_hint(self):
+        self.last_info  # the previous solve's route first
         if self._chol_skip > 0:
             self._chol_skip -= 1
             return 0
         return 1
 
-    def _note_route(self, tried, full):
+    def _note_route(self, tried, full, info):
         """Back-off of the dual solve; `full`: the rank of a spectrum the
         route accepts (P for the P x P Gram)."""
-        if self.last_info.iterations != 0:
+        if info.iterations != 0:
             return
         if tried:
             self._chol_skip = self.CHOLESKY_RETRY - 1
-        elif self.last_info.rank >= full:
+        elif info.rank >= full:
             self._chol_skip = 0
 
     def set_pinv_tol(self, tol):
@@ -107,7 +189,7 @@ class MinSR:
         self.last_info = info
         # back-off after a refused attempt; a spectrum the route accepts keeps
         # P (dual) or N - 1 (T form, the deflated centring vector) eigenvalues
-        self._note_route(hint, p if dual else n - 1)
+        self._note_route(hint, p if dual else n - 1, info)
         if cpx:
             # T_ref = 2 G G^dagger: pinv scales by 1/2; [v, -i v] layout per leaf
             return psi.expand_compact(0.5 * upd)
@@ -136,6 +218,7 @@ class MinSR:
         env = os.environ.get("VMC_JAX_AMD_MINSR_ELOC_FIRST")
         if env is not None:
             return env == "1"
+        self.last_info  # the previous solve's route first
         return not psi.fp32_gradients and self._chol_skip == 0
 
     def _dual_gram(self, gradients):
@@ -149,15 +232,18 @@ class MinSR:
         return D, S
 
     def _solve_deferred(self, Eraw, ready, D, S, p):
+        """The non-blocking solve: its info is read later (see last_info)."""
         n, P = D.shape
         upd = torch.empty(P, dtype=torch.float64, device=D.device)
-        info = _lib.JvSolveInfo()
         hint = self._cholesky_hint()
-        _lib.call("jv_minsr_solve_dual_deferred_gram", _lib.ptr(D), int(D.dtype == torch.float32), _lib.ptr(S),
-                  _lib.ptr(Eraw), _lib.ptr(p), ready._as_parameter_, n, P, 0, float(self.pinvTol),
-                  float(self.diagonalShift), hint, _lib.ptr(upd), ctypes.byref(info), _lib.stream_ptr())
-        self.last_info = info
-        self._note_route(hint, P)
+        buf = torch.empty(ctypes.sizeof(_lib.JvSolveInfo), dtype=torch.uint8, pin_memory=True)
+        _lib.call("jv_minsr_solve_dual_deferred_gram_async", _lib.ptr(D), int(D.dtype == torch.float32),
+                  _lib.ptr(S), _lib.ptr(Eraw), _lib.ptr(p), ready._as_parameter_, n, P, 0, float(self.pinvTol),
+                  float(self.diagonalShift), hint, _lib.ptr(upd), ctypes.c_void_p(buf.data_ptr()),
+                  _lib.stream_ptr())
+        done = torch.cuda.Event()
+        done.record()
+        self._pending = _Outcome(buf, done, hint, P)
         return upd
 
     def __call__(self, netParameters, t, *, psi, hamiltonian, **rhsArgs):
@@ -213,8 +299,7 @@ class MinSR:
         psi.set_parameters(tmpParameters)
         if rhsArgs.get("intStep", None) == 0:
             if Eloc is None:
-                self.ElocMean0 = complex(self.last_info.e_mean[0], self.last_info.e_mean[1])
-                self.ElocVar0 = float(self.last_info.e_var)
+                self._energy_src = self._pending  # read on first use
             else:
                 self.ElocMean0 = complex(Eloc.mean()[0].item())
                 self.ElocVar0 = float(Eloc.var()[0].item())
