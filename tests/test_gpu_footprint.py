@@ -91,6 +91,11 @@ def test_footprint_bounds_one_step(case, warm):
             out["energy"] = eq.get_energy_mean()
         except Exception as exc:  # reported on the main thread
             out["error"] = repr(exc)
+        finally:
+            # release this thread's workspaces and handles now: Python's join()
+            # can return before the thread's C++ thread_local destructor has
+            # freed them, which would then land in the next case's measurement
+            _lib.call("jv_shutdown")
 
     t = threading.Thread(target=body)
     t.start()

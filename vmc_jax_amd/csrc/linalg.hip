@@ -35,6 +35,10 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 // ------------------------------------------------------------------ centring
 // Column partial sums of w_n * O[n][j] over row chunks (deterministic order).
 constexpr int CHUNK_ROWS = 256;
+// Row chunk of the transposed GEMV's partial sums: 64 rows give 8192 x 1240
+// data 640 workgroups (5 column blocks x 128 chunks; 160 with 256 rows, too
+// few to keep the loads of 256 CUs in flight).
+constexpr int GEMV_T_ROWS = 64;
 
 template <typename TIn>
 __global__ void col_wsum_partial(const TIn* __restrict__ O, const double* __restrict__ w, int64_t n,
@@ -57,13 +61,43 @@ __global__ void col_wsum_partial(const TIn* __restrict__ O, const double* __rest
   }
 }
 
+// col_wsum_partial of a single column (p = 1, e.g. E_loc): one row per
+// thread (blockDim.x = CHUNK_ROWS) and a workgroup sum, written to the same
+// partial layout part[chunk * len + component].
+template <typename TIn>
+__global__ void vec_wsum_partial(const TIn* __restrict__ O, const double* __restrict__ w, int64_t n, int cpx,
+                                 double* __restrict__ part) {
+  chain_priority();
+  __shared__ double red[CHUNK_ROWS / WAVE];
+  const int64_t r = (int64_t)blockIdx.y * CHUNK_ROWS + threadIdx.x;
+  double ar = 0.0, ai = 0.0;
+  if (r < n) {
+    ar = w[r] * (double)O[cpx ? 2 * r : r];
+    if (cpx) ai = w[r] * (double)O[2 * r + 1];
+  }
+  ar = block_sum(ar, red);
+  if (cpx) ai = block_sum(ai, red);
+  if (threadIdx.x == 0) {
+    part[(int64_t)blockIdx.y * (cpx ? 2 : 1)] = ar;
+    if (cpx) part[(int64_t)blockIdx.y * 2 + 1] = ai;
+  }
+}
+
 __global__ void col_reduce(const double* __restrict__ part, int nchunks, int64_t len, double* __restrict__ out) {
   chain_priority();
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= len) return;
-  double a = 0.0;
-  for (int c = 0; c < nchunks; ++c) a += part[c * len + j];
-  out[j] = a;
+  // four chunks per iteration into independent sums (loads in flight together)
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  int c = 0;
+  for (; c + 3 < nchunks; c += 4) {
+    a0 += part[c * len + j];
+    a1 += part[(c + 1) * len + j];
+    a2 += part[(c + 2) * len + j];
+    a3 += part[(c + 3) * len + j];
+  }
+  for (; c < nchunks; ++c) a0 += part[c * len + j];
+  out[j] = (a0 + a1) + (a2 + a3);
 }
 
 template <typename TIn>
@@ -94,8 +128,11 @@ int center_with(const TIn* O, const double* w, int64_t n, int64_t p, int cpx, do
                 double* part, hipStream_t st) {
   int nchunks = (int)((n + CHUNK_ROWS - 1) / CHUNK_ROWS);
   int64_t len = p * (cpx ? 2 : 1);
-  hipLaunchKernelGGL(col_wsum_partial<TIn>, dim3((unsigned)((p + 255) / 256), nchunks), dim3(256), 0, st, O, w, n,
-                     p, cpx, part);
+  if (p == 1)
+    hipLaunchKernelGGL(vec_wsum_partial<TIn>, dim3(1, nchunks), dim3(CHUNK_ROWS), 0, st, O, w, n, cpx, part);
+  else
+    hipLaunchKernelGGL(col_wsum_partial<TIn>, dim3((unsigned)((p + 255) / 256), nchunks), dim3(256), 0, st, O, w,
+                       n, p, cpx, part);
   JV_LAUNCH_CHECK("col_wsum_partial");
   hipLaunchKernelGGL(col_reduce, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, st, part, nchunks, len, mean);
   JV_LAUNCH_CHECK("col_reduce");
@@ -456,18 +493,29 @@ __global__ void gemv_t_partial(const TM* __restrict__ M, const double* __restric
                                int64_t cols, int conjM, double* __restrict__ part) {
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= cols) return;
-  int64_t r0 = (int64_t)blockIdx.y * CHUNK_ROWS, r1 = std::min<int64_t>(rows, r0 + CHUNK_ROWS);
+  int64_t r0 = (int64_t)blockIdx.y * GEMV_T_ROWS, r1 = std::min<int64_t>(rows, r0 + GEMV_T_ROWS);
   double ar = 0.0, ai = 0.0;
-  for (int64_t r = r0; r < r1; ++r) {
-    if (CPX) {
+  if (CPX) {
+    for (int64_t r = r0; r < r1; ++r) {
       double mr = M[2 * (r * cols + j)], mi = M[2 * (r * cols + j) + 1];
       if (conjM) mi = -mi;
       double xr = x[2 * r], xi = x[2 * r + 1];
       ar += mr * xr - mi * xi;
       ai += mr * xi + mi * xr;
-    } else {
-      ar = fma((double)M[r * cols + j], x[r], ar);
     }
+  } else {
+    // four rows per iteration into independent sums, so their loads are in
+    // flight together
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int64_t r = r0;
+    for (; r + 3 < r1; r += 4) {
+      a0 = fma((double)M[r * cols + j], x[r], a0);
+      a1 = fma((double)M[(r + 1) * cols + j], x[r + 1], a1);
+      a2 = fma((double)M[(r + 2) * cols + j], x[r + 2], a2);
+      a3 = fma((double)M[(r + 3) * cols + j], x[r + 3], a3);
+    }
+    for (; r < r1; ++r) a0 = fma((double)M[r * cols + j], x[r], a0);
+    ar = (a0 + a1) + (a2 + a3);
   }
   if (CPX) {
     part[2 * (blockIdx.y * cols + j)] = ar;
@@ -485,7 +533,7 @@ __global__ void scale_vec(double* __restrict__ v, int64_t len, double alpha) {
 // y = M^T x for a real FP32 matrix M [rows][cols] (the mixed form's centred
 // data), accumulated in FP64.
 int gemv_t_f32(const float* M, const double* x, int64_t rows, int64_t cols, double* y, hipStream_t st, int ws_slot) {
-  const int nchunks = (int)((rows + CHUNK_ROWS - 1) / CHUNK_ROWS);
+  const int nchunks = (int)((rows + GEMV_T_ROWS - 1) / GEMV_T_ROWS);
   double* part = (double*)workspace(sizeof(double) * (size_t)nchunks * cols, ws_slot);
   if (!part) return fail(JV_E_NOMEM, "gemv: workspace allocation failed");
   hipLaunchKernelGGL((gemv_t_partial<false, float>), dim3((unsigned)((cols + 255) / 256), nchunks), dim3(256), 0, st, M,
@@ -505,7 +553,7 @@ int gemv(bool trans, const double* M, const double* x, int64_t rows, int64_t col
     JV_LAUNCH_CHECK("gemv_n");
     return JV_OK;
   }
-  int nchunks = (int)((rows + CHUNK_ROWS - 1) / CHUNK_ROWS);
+  int nchunks = (int)((rows + GEMV_T_ROWS - 1) / GEMV_T_ROWS);
   int64_t len = cols * (cpx ? 2 : 1);
   double* part = (double*)workspace(sizeof(double) * (size_t)nchunks * len, ws_slot);
   if (!part) return fail(JV_E_NOMEM, "gemv: workspace allocation failed");
@@ -1013,12 +1061,32 @@ __global__ void col_norm2_partial(const double* __restrict__ D, int64_t n, int64
   part[blockIdx.y * p + j] = a;
 }
 
+// col_norm2_partial of a single column: one row per thread and a workgroup sum.
+__global__ void vec_norm2_partial(const double* __restrict__ D, int64_t n, int cpx, double* __restrict__ part) {
+  __shared__ double red[CHUNK_ROWS / WAVE];
+  const int64_t r = (int64_t)blockIdx.y * CHUNK_ROWS + threadIdx.x;
+  double a = 0.0;
+  if (r < n) {
+    if (cpx) {
+      const double x = D[2 * r], y = D[2 * r + 1];
+      a = fma(x, x, y * y);
+    } else {
+      a = D[r] * D[r];
+    }
+  }
+  a = block_sum(a, red);
+  if (threadIdx.x == 0) part[blockIdx.y] = a;
+}
+
 int col_norm2(const double* D, int64_t n, int64_t p, int cpx, double* out, hipStream_t st, int slot) {
   int nchunks = (int)((n + CHUNK_ROWS - 1) / CHUNK_ROWS);
   double* part = (double*)workspace(sizeof(double) * (size_t)nchunks * p, slot);
   if (!part) return fail(JV_E_NOMEM, "col_norm2: workspace allocation failed");
-  hipLaunchKernelGGL(col_norm2_partial, dim3((unsigned)((p + 255) / 256), nchunks), dim3(256), 0, st, D, n, p, cpx,
-                     part);
+  if (p == 1)
+    hipLaunchKernelGGL(vec_norm2_partial, dim3(1, nchunks), dim3(CHUNK_ROWS), 0, st, D, n, cpx, part);
+  else
+    hipLaunchKernelGGL(col_norm2_partial, dim3((unsigned)((p + 255) / 256), nchunks), dim3(256), 0, st, D, n, p,
+                       cpx, part);
   JV_LAUNCH_CHECK("col_norm2_partial");
   hipLaunchKernelGGL(col_reduce, dim3((unsigned)((p + 255) / 256)), dim3(256), 0, st, part, nchunks, p, out);
   JV_LAUNCH_CHECK("col_reduce");

@@ -425,6 +425,7 @@ def footprint_bytes(num_samples, num_real_params, num_sites, complex_params, sol
             b += _bf16_gram_bytes(n, 2 * pc)
     elif solver == "minsr":
         b += n * pc * esz * 2                             # O and centred O
+        b += (n // 64 + 1) * pc * esz                     # O^T e partial sums (64-sample chunks)
         if n > pc:                                        # dual form
             m = pc
             b += m * m * esz * 4                          # S, eigenvectors, library eigendecomposition workspace
