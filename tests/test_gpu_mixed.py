@@ -143,6 +143,24 @@ def test_mixed_metropolis_runs_tfim10x10():
     assert abs(float(rates[0]) - float(rates[1])) <= 0.03
 
 
+def test_mixed_sample_logpsi_on_side_stream():
+    """MCSampler.sample(logpsi_stream=...) of the mixed form: the same chains
+    (configurations), the same FP64 log psi (computed on the other stream) and
+    the same weights as a plain call from a sampler with the same key."""
+    jv, shape, psi, _, P = _c3_pair("mixed")
+    a = jv.sampler.MCSampler(psi, shape, 23, updateProposer=jv.sampler.propose_spin_flip, numChains=128,
+                             numSamples=1024)
+    b = jv.sampler.MCSampler(psi, shape, 23, updateProposer=jv.sampler.propose_spin_flip, numChains=128,
+                             numSamples=1024)
+    side = torch.cuda.Stream()
+    for _ in range(2):  # the second call continues the chains
+        s1, l1, p1 = a.sample()
+        s2, l2, p2 = b.sample(logpsi_stream=side)
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        assert torch.equal(s1, s2) and torch.equal(l1, l2) and torch.equal(p1, p2)
+
+
 def test_mixed_minsr_step_tfim10x10():
     """One MinSR right-hand side of the mixed form (FP32 O, compensated-BF16
     Gram, FP64 solve) against the oracle's FP64 T-form MinSR on the same 2048

@@ -62,8 +62,20 @@ class MCSampler:
     def get_last_number_of_samples(self):
         return self.globNumSamples
 
-    def sample(self, parameters=None, numSamples=None, multipleOf=1):
-        """sampler.py:195-235."""
+    # sample() accepts logpsi_stream (see there)
+    defers_logpsi = True
+
+    def sample(self, parameters=None, numSamples=None, multipleOf=1, *, logpsi_stream=None):
+        """sampler.py:195-235.
+
+        logpsi_stream (not in the reference): for the mixed form (FP32
+        chains, whose emitted samples' FP64 log psi is a separate forward
+        pass) with |psi|^2 sampling (mu = 2, uniform weights), the forward
+        pass runs on that stream after the chains, and the returned log psi
+        is ready in its order only; the configurations and weights are ready
+        on the current stream.  The network's parameters must not change
+        before that stream has run the forward pass.  Ignored otherwise, and
+        when ``parameters`` is given."""
         if numSamples is None:
             numSamples = self.numSamples
         tmp = None
@@ -86,6 +98,24 @@ class MCSampler:
         a.seed = self.key & 0xFFFFFFFFFFFFFFFF
         a.call_index = self.callIndex & 0xFFFFFFFF
         a.stream = 0
+        defer = (logpsi_stream is not None and parameters is None and self.net._sym is None and self.net.fp32_gradients
+                 and 1.0 / self.logProbFactor - self.mu == 0.0 and spc * C > 0)
+        if defer:
+            _lib.call("jv_mc_sample", self.net.desc, _lib.ptr(self.net.params), a, _lib.ptr(self.states),
+                      _lib.ptr(configs), None, _lib.ptr(self.counters), _lib.stream_ptr())
+            self.callIndex += 1
+            n = spc * C
+            logpsi_stream.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(logpsi_stream):
+                configs.record_stream(logpsi_stream)
+                logpsi.record_stream(logpsi_stream)
+                _lib.call("jv_net_forward", self.net.desc, _lib.ptr(self.net.params), _lib.ptr(configs), n,
+                          _lib.ptr(logpsi), _lib.stream_ptr())
+                out = self.net._out(logpsi.reshape(1, -1))
+            # mu = 2: exp(0 * Re log psi) / n, as jv_mc_weights gives it
+            p = torch.full((n,), 1.0 / n, dtype=torch.float64, device=dev)
+            configs = configs.reshape((1, n) + self.sampleShape)
+            return configs, out, p.reshape(1, -1)
         if self.net._sym is not None:
             _lib.call("jv_sym_mc_sample", self.net.desc, _lib.ptr(self.net.params), self.net._sym[0], a,
                       _lib.ptr(self.states), _lib.ptr(configs), _lib.ptr(logpsi), _lib.ptr(self.counters),

@@ -255,9 +255,21 @@ class MinSR:
         timer = _Timer(outp)
         numSamples = rhsArgs.get("numSamples", None)
         timer.start("sampling")
-        sampleConfigs, sampleLogPsi, p = self.sampler.sample(numSamples=numSamples)
+        side = None
+        if self.overlap and outp is None and psi.fp32_gradients and getattr(self.sampler, "defers_logpsi", False):
+            # the mixed form's FP64 log psi of the samples (needed by E_loc
+            # only) on the side stream, beside O_k and the Gram
+            if getattr(self, "_side", None) is None:
+                self._side = torch.cuda.Stream()
+            side = self._side
+            sampleConfigs, sampleLogPsi, p = self.sampler.sample(numSamples=numSamples, logpsi_stream=side)
+        else:
+            sampleConfigs, sampleLogPsi, p = self.sampler.sample(numSamples=numSamples)
         timer.stop("sampling")
-        if self._overlapped(psi, sampleConfigs.shape[1], outp):
+        overlapped = self._overlapped(psi, sampleConfigs.shape[1], outp)
+        if side is not None and not overlapped:
+            torch.cuda.current_stream().wait_stream(side)
+        if overlapped:
             # O_k and the Gram first, then E_loc on a side stream beside the
             # latency-bound factorisation (or reduction) of the Gram; the solve
             # waits for E_loc only where it needs it.  Starting E_loc after
