@@ -196,31 +196,42 @@ __device__ inline int wrap_once(int q, int L) { return q < 0 ? q + L : (q >= L ?
 
 template <int CM, int LC, typename T>
 __device__ inline void win_preact3(const CnnGeom& g, const cst_t<T>* __restrict__ K, const cst_t<T>* __restrict__ B,
-                                   const T* Xb, const T* Xw, const short* nb, int a, int b, T* z) {
-  // nb: the nine stored-layer positions read by this element's kernel
-  // offsets (the neighbour table's row of the element's own position).  An
-  // input inside the previous layer's window is read from Xw instead; both
-  // choices are element offsets from Xb, selected without a branch.
+                                   const T* Xb, const T* Xw, int si, int sj, int a, int b, T* z) {
+  // Inputs inside the previous layer's window come from Xw, the others from
+  // the stored layer at row si - a + ky, column sj - b + kx (periodic; the
+  // window fits the lattice, so one period at most): the rows are wrapped
+  // once per kernel row, the three columns once, and the two candidates are
+  // element offsets from Xb selected without a branch.
   constexpr int pex = 1 + 2 * LC, pey = pex, pwsz = pex * pey;
   const int dw = (int)(Xw - Xb);
+  int cl[3];
+  cl[0] = wrap_once(sj - b, g.Ly);
+  cl[1] = cl[0] + 1 == g.Ly ? 0 : cl[0] + 1;
+  cl[2] = cl[1] + 1 == g.Ly ? 0 : cl[1] + 1;
+  int row = wrap_once(si - a, g.Lx) * g.Ly;
 #pragma unroll
   for (int c = 0; c < CM; ++c) z[c] = B[c];
 #pragma unroll 1
-  for (int k = 0; k < 9; ++k) {
-    const int ky = k / 3, kx = k - 3 * (k / 3);
-    const int aq = a - ky, bq = b - kx;
-    const bool in = (unsigned)aq < (unsigned)pex && (unsigned)bq < (unsigned)pey;
-    const int qb = ((const __attribute__((address_space(3))) short*)nb)[k];
-    const int off = in ? dw + (aq * pey + bq) : qb;
-    const int xs = in ? pwsz : g.n;
-    lds_t<T>* x = (lds_t<T>*)(Xb + off);
-    const cst_t<T>* w = K + k * CM * CM;
+  for (int ky = 0; ky < 3; ++ky) {
+    const int aq = a - ky;
+    const bool rin = (unsigned)aq < (unsigned)pex;
 #pragma unroll
-    for (int ci = 0; ci < CM; ++ci) {
-      const T xv = x[ci * xs];
+    for (int kx = 0; kx < 3; ++kx) {
+      const int bq = b - kx;
+      const bool in = rin && (unsigned)bq < (unsigned)pey;
+      const int off = in ? dw + (aq * pey + bq) : row + cl[kx];
+      const int xs = in ? pwsz : g.n;
+      lds_t<T>* x = (lds_t<T>*)(Xb + off);
+      const cst_t<T>* w = K + (ky * 3 + kx) * CM * CM;
 #pragma unroll
-      for (int c = 0; c < CM; ++c) z[c] = fma(xv, w[ci * CM + c], z[c]);
+      for (int ci = 0; ci < CM; ++ci) {
+        const T xv = x[ci * xs];
+#pragma unroll
+        for (int c = 0; c < CM; ++c) z[c] = fma(xv, w[ci * CM + c], z[c]);
+      }
     }
+    row += g.Ly;
+    if (row == g.n) row = 0;
   }
 }
 

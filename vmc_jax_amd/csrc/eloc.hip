@@ -489,8 +489,8 @@ __global__ void __launch_bounds__(512) eloc_cnn_win(CnnGeom g, const T* __restri
             if constexpr (lc == 0)
               win_preact03<CM, T>(g, K, B, x0, (T)(2 * (fi & 1) - 1), site, si, sj, a, b, z);
             else
-              win_preact3<CM, lc, T>(g, K, B, A + (size_t)(l - 1) * lsz, W[l - 1] + (size_t)f * CM * pex * pey,
-                                     nbr + win_site(g, si, sj, a, b) * 9, a, b, z);
+              win_preact3<CM, lc, T>(g, K, B, A + (size_t)(l - 1) * lsz, W[l - 1] + (size_t)f * CM * pex * pey, si,
+                                     sj, a, b, z);
           } else {
             if (l == 0)
               win_preact0<CM, T>(g, K, B, x0, (T)(2 * (fi & 1) - 1), site, si, sj, a, b, z);
