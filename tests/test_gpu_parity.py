@@ -189,7 +189,8 @@ def test_eloc_cnn(shape):
     assert rel(e, eo) < 1e-11
 
 
-@pytest.mark.parametrize("shape,ch", [((10, 10), (8, 8, 8)), ((6, 6), (4, 4, 4)), ((7, 9), (8, 8))])
+@pytest.mark.parametrize("shape,ch", [((10, 10), (8, 8, 8)), ((6, 6), (4, 4, 4)), ((7, 9), (8, 8)),
+                                      ((7, 9), (4, 8, 4))])  # last: a 7-wide window on 7 rows
 def test_eloc_cnn_incremental_windows_fp64(shape, ch, monkeypatch):
     """FP64 E_loc with incremental windows (VMC_JAX_AMD_ELOC_DELTA=1: stored
     pre-activations plus the changed inputs) against window recomputation (the
