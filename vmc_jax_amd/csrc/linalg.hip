@@ -40,24 +40,45 @@ constexpr int CHUNK_ROWS = 256;
 // few to keep the loads of 256 CUs in flight).
 constexpr int GEMV_T_ROWS = 64;
 
+// Column partial sums of one CHUNK_ROWS chunk per blockIdx.y: a workgroup of
+// COLS_PER_WG columns x ROW_GROUPS row groups (threadIdx.y), each thread summing
+// every ROW_GROUPS-th row of the chunk, the groups combined through LDS (four
+// times the workgroups of one thread per column: 8192 x 1240 data fill 640).
+constexpr int COLS_PER_WG = 64, ROW_GROUPS = 4;
+
 template <typename TIn>
-__global__ void col_wsum_partial(const TIn* __restrict__ O, const double* __restrict__ w, int64_t n,
-                                 int64_t p, int cpx, double* __restrict__ part) {
+__global__ void __launch_bounds__(COLS_PER_WG * ROW_GROUPS) col_wsum_partial(const TIn* __restrict__ O,
+                                                                             const double* __restrict__ w, int64_t n,
+                                                                             int64_t p, int cpx,
+                                                                             double* __restrict__ part) {
   chain_priority();
-  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= p) return;
-  int64_t r0 = (int64_t)blockIdx.y * CHUNK_ROWS, r1 = std::min<int64_t>(n, r0 + CHUNK_ROWS);
+  __shared__ double sr[ROW_GROUPS][COLS_PER_WG], si[ROW_GROUPS][COLS_PER_WG];
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int64_t j = (int64_t)blockIdx.x * COLS_PER_WG + tx;
+  const int64_t r0 = (int64_t)blockIdx.y * CHUNK_ROWS, r1 = std::min<int64_t>(n, r0 + CHUNK_ROWS);
   double ar = 0.0, ai = 0.0;
-  if (cpx) {
-    for (int64_t r = r0; r < r1; ++r) {
-      ar = fma(w[r], O[2 * (r * p + j)], ar);
-      ai = fma(w[r], O[2 * (r * p + j) + 1], ai);
+  if (j < p) {
+    if (cpx) {
+      for (int64_t r = r0 + ty; r < r1; r += ROW_GROUPS) {
+        ar = fma(w[r], O[2 * (r * p + j)], ar);
+        ai = fma(w[r], O[2 * (r * p + j) + 1], ai);
+      }
+    } else {
+      for (int64_t r = r0 + ty; r < r1; r += ROW_GROUPS) ar = fma(w[r], O[r * p + j], ar);
     }
-    part[2 * (blockIdx.y * p + j)] = ar;
-    part[2 * (blockIdx.y * p + j) + 1] = ai;
-  } else {
-    for (int64_t r = r0; r < r1; ++r) ar = fma(w[r], O[r * p + j], ar);
-    part[blockIdx.y * p + j] = ar;
+  }
+  sr[ty][tx] = ar;
+  si[ty][tx] = ai;
+  __syncthreads();
+  if (ty == 0 && j < p) {
+    ar = (sr[0][tx] + sr[1][tx]) + (sr[2][tx] + sr[3][tx]);
+    if (cpx) {
+      ai = (si[0][tx] + si[1][tx]) + (si[2][tx] + si[3][tx]);
+      part[2 * (blockIdx.y * p + j)] = ar;
+      part[2 * (blockIdx.y * p + j) + 1] = ai;
+    } else {
+      part[blockIdx.y * p + j] = ar;
+    }
   }
 }
 
@@ -131,8 +152,8 @@ int center_with(const TIn* O, const double* w, int64_t n, int64_t p, int cpx, do
   if (p == 1)
     hipLaunchKernelGGL(vec_wsum_partial<TIn>, dim3(1, nchunks), dim3(CHUNK_ROWS), 0, st, O, w, n, cpx, part);
   else
-    hipLaunchKernelGGL(col_wsum_partial<TIn>, dim3((unsigned)((p + 255) / 256), nchunks), dim3(256), 0, st, O, w,
-                       n, p, cpx, part);
+    hipLaunchKernelGGL(col_wsum_partial<TIn>, dim3((unsigned)((p + COLS_PER_WG - 1) / COLS_PER_WG), nchunks),
+                       dim3(COLS_PER_WG, ROW_GROUPS), 0, st, O, w, n, p, cpx, part);
   JV_LAUNCH_CHECK("col_wsum_partial");
   hipLaunchKernelGGL(col_reduce, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, st, part, nchunks, len, mean);
   JV_LAUNCH_CHECK("col_reduce");
@@ -1043,22 +1064,29 @@ int jv_minsr_solve(const double* data, const double* e, int64_t n, int64_t p, in
 // ====================================================================== TDVP
 namespace jv {
 
-__global__ void col_norm2_partial(const double* __restrict__ D, int64_t n, int64_t p, int cpx,
-                                  double* __restrict__ part) {
-  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= p) return;
-  int64_t r0 = (int64_t)blockIdx.y * CHUNK_ROWS, r1 = std::min<int64_t>(n, r0 + CHUNK_ROWS);
+// The same work split as col_wsum_partial.
+__global__ void __launch_bounds__(COLS_PER_WG * ROW_GROUPS) col_norm2_partial(const double* __restrict__ D, int64_t n,
+                                                                              int64_t p, int cpx,
+                                                                              double* __restrict__ part) {
+  __shared__ double sa[ROW_GROUPS][COLS_PER_WG];
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int64_t j = (int64_t)blockIdx.x * COLS_PER_WG + tx;
+  const int64_t r0 = (int64_t)blockIdx.y * CHUNK_ROWS, r1 = std::min<int64_t>(n, r0 + CHUNK_ROWS);
   double a = 0.0;
-  for (int64_t r = r0; r < r1; ++r) {
-    if (cpx) {
-      double x = D[2 * (r * p + j)], y = D[2 * (r * p + j) + 1];
-      a = fma(x, x, fma(y, y, a));
-    } else {
-      double x = D[r * p + j];
-      a = fma(x, x, a);
+  if (j < p) {
+    for (int64_t r = r0 + ty; r < r1; r += ROW_GROUPS) {
+      if (cpx) {
+        double x = D[2 * (r * p + j)], y = D[2 * (r * p + j) + 1];
+        a = fma(x, x, fma(y, y, a));
+      } else {
+        double x = D[r * p + j];
+        a = fma(x, x, a);
+      }
     }
   }
-  part[blockIdx.y * p + j] = a;
+  sa[ty][tx] = a;
+  __syncthreads();
+  if (ty == 0 && j < p) part[blockIdx.y * p + j] = (sa[0][tx] + sa[1][tx]) + (sa[2][tx] + sa[3][tx]);
 }
 
 // col_norm2_partial of a single column: one row per thread and a workgroup sum.
@@ -1085,8 +1113,8 @@ int col_norm2(const double* D, int64_t n, int64_t p, int cpx, double* out, hipSt
   if (p == 1)
     hipLaunchKernelGGL(vec_norm2_partial, dim3(1, nchunks), dim3(CHUNK_ROWS), 0, st, D, n, cpx, part);
   else
-    hipLaunchKernelGGL(col_norm2_partial, dim3((unsigned)((p + 255) / 256), nchunks), dim3(256), 0, st, D, n, p,
-                       cpx, part);
+    hipLaunchKernelGGL(col_norm2_partial, dim3((unsigned)((p + COLS_PER_WG - 1) / COLS_PER_WG), nchunks),
+                       dim3(COLS_PER_WG, ROW_GROUPS), 0, st, D, n, p, cpx, part);
   JV_LAUNCH_CHECK("col_norm2_partial");
   hipLaunchKernelGGL(col_reduce, dim3((unsigned)((p + 255) / 256)), dim3(256), 0, st, part, nchunks, p, out);
   JV_LAUNCH_CHECK("col_reduce");
