@@ -121,16 +121,21 @@ __global__ void col_reduce(const double* __restrict__ part, int nchunks, int64_t
   out[j] = (a0 + a1) + (a2 + a3);
 }
 
+// Columns across threads (x), rows strided across blockIdx.y: no division
+// per element.
 template <typename TIn>
 __global__ void center_apply(const TIn* __restrict__ O, const double* __restrict__ w,
                              const double* __restrict__ mean, int64_t n, int64_t p, int cpx,
                              TIn* __restrict__ D) {
   chain_priority();
-  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int64_t len = p * (cpx ? 2 : 1);
-  if (t >= n * len) return;
-  int64_t r = t / len, j = t % len;
-  D[t] = (TIn)(sqrt(w[r]) * ((double)O[t] - mean[j]));
+  const int64_t len = p * (cpx ? 2 : 1);
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= len) return;
+  const double m = mean[j];
+  for (int64_t r = blockIdx.y; r < n; r += gridDim.y) {
+    const int64_t t = r * len + j;
+    D[t] = (TIn)(sqrt(w[r]) * ((double)O[t] - m));
+  }
 }
 
 // Doubles of partial column sums the centring needs: one row of len values
@@ -157,7 +162,8 @@ int center_with(const TIn* O, const double* w, int64_t n, int64_t p, int cpx, do
   JV_LAUNCH_CHECK("col_wsum_partial");
   hipLaunchKernelGGL(col_reduce, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, st, part, nchunks, len, mean);
   JV_LAUNCH_CHECK("col_reduce");
-  hipLaunchKernelGGL(center_apply<TIn>, dim3((unsigned)((n * len + 255) / 256)), dim3(256), 0, st, O, w, mean, n,
+  hipLaunchKernelGGL(center_apply<TIn>, dim3((unsigned)((len + 255) / 256), (unsigned)std::max<int64_t>(1, std::min<int64_t>(n, 1024))),
+                     dim3(256), 0, st, O, w, mean, n,
                      p, cpx, D);
   JV_LAUNCH_CHECK("center_apply");
   return JV_OK;
