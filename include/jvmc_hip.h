@@ -452,7 +452,7 @@ int jv_minsr_solve_dual_deferred_gram(const void* data, int32_t f32, double* S, 
  * memory (hipHostMalloc) for the call not to block.  A Cholesky factorisation
  * that failed on the accepted route, an error return of the blocking form, is
  * reported as info->iterations = -1 - its info.  The host still waits inside
- * the call for the Cholesky attempt's verdict (it selects the kernels that
+ * the call for the Cholesky attempt's outcome (it selects the kernels that
  * follow), not for E_loc or the rest of the solve.  Same update.  No
  * separate reference function (minsr.py:82-165 returns the update and keeps
  * the energy for later reads). */
